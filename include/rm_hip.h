@@ -1,0 +1,156 @@
+/*
+ * rm_hip.h — C-ABI of the MI355X rendering backend (librm_hip.so).
+ *
+ * This is the drop-in boundary that replaces the reference's Metal device:
+ * every entry point below takes plain pointers and sizes, and the record
+ * layouts are the byte layouts the reference uploads to its GPU
+ * (src/engine/metal_types.h:3-57; sizes asserted at src/engine/metal.mm:13-17),
+ * so a caller that already marshals scenes for the reference can hand the
+ * same arrays over unchanged.
+ *
+ * All functions return 0 on success and a nonzero RM_ERR_* code otherwise; the
+ * message is available from rm_last_error(). Calls on one context are
+ * serialised by an internal mutex (metal.mm:104) and are synchronous: when a
+ * call returns, its outputs are in host memory.
+ */
+#ifndef RM_HIP_H
+#define RM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RM_OK = 0,
+    RM_ERR_NO_DEVICE = 1,   /* no HIP GPU (reference: "No Metal GPU found", metal.mm:47) */
+    RM_ERR_INVALID = 2,     /* bad argument (reference: metal.mm:98-101) */
+    RM_ERR_ALLOC = 3,       /* device allocation failed (metal.mm:185,191,267) */
+    RM_ERR_DEVICE = 4,      /* a HIP call or kernel failed (metal.mm:368-371) */
+    RM_ERR_NO_SCENE = 5     /* rm_render/rm_trace before rm_set_scene */
+};
+
+typedef struct { float x, y, z; } rm_vec3;
+
+/* Triangle record, 176 bytes (metal_types.h:5-11). */
+typedef struct {
+    rm_vec3 v0, v1, v2, n, nn, n0, n1, n2;
+    float ar, ag, ab, er, eg, eb, ior, rough, metallic, alpha;
+    int32_t shader;
+    float tu0, tv0, tu1, tv1, tu2, tv2;
+    int32_t texture_offset, texture_width, texture_height;
+} rm_triangle;
+
+/* Binary BVH node in preorder, 36 bytes (metal_types.h:17-20): internal nodes
+ * have count == 0, their left child follows them, and right is the index just
+ * past the node's subtree (the escape link built at metal.mm:129-144). */
+typedef struct {
+    rm_vec3 mn, mx;
+    int32_t right, offset, count;
+} rm_bvh_node;
+
+/* Camera, 52 bytes (metal_types.h:12-15): position, focal length in pixels and
+ * the row-major camera-to-world rotation m0..m8. */
+typedef struct {
+    float x, y, z, focal;
+    float m0, m1, m2, m3, m4, m5, m6, m7, m8;
+} rm_camera;
+
+/* Primary-hit G-buffer record, 76 bytes (metal_types.h:32-42). */
+typedef struct {
+    rm_vec3 pos, normal, albedo, wo, emission;
+    float rough, metallic, ior;
+    int32_t kind; /* 0 miss, 1 opaque, 3 dielectric */
+} rm_gbuffer;
+
+/* ReSTIR DI reservoir, 60 bytes (metal_types.h:46-56). */
+typedef struct {
+    rm_vec3 light_pos, light_normal, emission;
+    float alpha, p_hat, sum_ws, sum_wu;
+    uint32_t m, valid;
+} rm_reservoir;
+
+/* One scene as the reference uploads it (metal.mm:109-186). */
+typedef struct {
+    const rm_triangle* tris;   size_t ntri;
+    const rm_bvh_node* nodes;  size_t nnode;   /* preorder with escape links */
+    const int32_t* bvh_tri;                     /* ntri entries; leaves index it */
+    const int32_t* light_tri;  const float* light_cdf; size_t nlight;
+    float light_total;
+    const float* texels;       size_t ntexel;  /* linear RGB triples */
+} rm_scene_desc;
+
+/* Render settings after the host-side defaults of metal.mm:146-170 (the C++
+ * shim render_image_hip applies those and the environment variables). */
+typedef struct {
+    int32_t width, height, spp, bounces;
+    float resolution, clamp_value;
+    int32_t adapt_min, adapt_step;
+    float adapt_rel, adapt_abs;
+    int32_t nomis;
+    int32_t restir_candidates, restir_spatial, restir_mcap;
+    float restir_radius;
+    int32_t denoise;             /* RAYMOTION_DENOISE (metal.mm:269-271) */
+    int32_t denoise_iterations;  /* a-trous passes; the reference runs 2 */
+    int32_t temporal;            /* RAYMOTION_RESTIR_TEMPORAL (metal.mm:251-252) */
+    rm_camera camera;
+} rm_frame;
+
+/* Per-frame counters of the last rm_render (or rm_trace) call. */
+typedef struct {
+    int32_t path;         /* 0 megakernel (render_paths), 1 ReSTIR + denoise */
+    int32_t lanes;        /* megakernel lanes per pixel: 1, 8 or 32 */
+    int32_t has_history;  /* temporal reuse was active */
+    int32_t refit;        /* last rm_set_scene: 0 unchanged, 1 refit, 2 rebuild */
+    /* Per pass: megakernel, generate, spatial, shade, probe, and the total. */
+    uint64_t rays[6];        /* rays traced */
+    uint64_t node_visits[6]; /* 4-wide BVH nodes fetched */
+    uint64_t tri_tests[6];   /* ray/triangle tests */
+    float ms[8];          /* GPU time per pass: megakernel, generate, spatial, shade,
+                             temporal, atrous (all passes), tonemap, whole frame */
+    int32_t bvh_nodes;    /* 4-wide nodes resident */
+    int32_t stack_depth;  /* traversal stack entries the resident tree needs */
+} rm_stats;
+
+typedef struct rm_ctx rm_ctx;
+
+/* Device context: HIP device, stream, persistent buffers and temporal history.
+ * Replaces the reference's static Context (metal.mm:23-92,102). */
+int rm_create(rm_ctx** out, int hip_device);
+void rm_destroy(rm_ctx* ctx);
+/* Message for the last failure on ctx; ctx == NULL gives the last rm_create failure. */
+const char* rm_last_error(const rm_ctx* ctx);
+
+/* Uploads a scene (metal.mm:109-186 marshalling + upload, 194-231 refit or
+ * rebuild policy). Identical content is not re-uploaded. With the same
+ * triangle count and BVH topology the resident 4-wide tree is refit on the GPU,
+ * at most 31 times in a row; otherwise it is rebuilt from the given nodes. */
+int rm_set_scene(rm_ctx* ctx, const rm_scene_desc* scene);
+
+/* One frame (render_image_metal, metal.mm:95-414): rgb8 receives W*H*3 bytes,
+ * row-major, top row first; hdr (may be NULL) receives the W*H*3 linear values
+ * that were tonemapped. Either output may be NULL. */
+int rm_render(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, uint8_t* rgb8, float* hdr);
+
+/* Closest-hit (any == 0) or any-hit (any != 0) queries against the resident
+ * scene: the traversal of PathTracer::intersect (pathtrace.metal:119-163) as a
+ * standalone probe. origins/dirs: 3*n floats; tmax may be NULL (1e30). tri
+ * receives the hit triangle index (scene order) or -1; t/u/v may be NULL. */
+int rm_trace(rm_ctx* ctx, const float* origins, const float* dirs, const float* tmax, size_t n, int any,
+             int32_t* tri, float* t, float* u, float* v);
+
+/* Intermediate buffers of the last ReSTIR frame in the reference layouts:
+ * G-buffer, initial and merged reservoirs (W*H each) and the shaded HDR image
+ * before denoising (W*H*3). Any pointer may be NULL. */
+int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade);
+
+/* Per-pass GPU timing with HIP events (off by default). */
+int rm_set_profiling(rm_ctx* ctx, int enable);
+int rm_get_stats(rm_ctx* ctx, rm_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RM_HIP_H */

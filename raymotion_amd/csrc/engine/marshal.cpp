@@ -1,0 +1,131 @@
+// Scene marshalling and render-switch handling in front of the C-ABI.
+#include "marshal.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <string>
+#include <unordered_map>
+
+namespace raymotion {
+namespace {
+
+rm_vec3 pack(const Vec3& v) { return {float(v.x), float(v.y), float(v.z)}; }
+
+bool env_is(const char* name, const char* value) {
+    const char* v = std::getenv(name);
+    return v && std::string(v) == value;
+}
+
+}  // namespace
+
+rm_scene_desc MarshalledScene::desc(const Scene& scene) const {
+    rm_scene_desc d{};
+    d.tris = tris.data();
+    d.ntri = tris.size();
+    d.nodes = nodes.data();
+    d.nnode = nodes.size();
+    d.bvh_tri = scene.bvh_tri.data();
+    d.light_tri = scene.light_tri.data();
+    d.light_cdf = cdf.data();
+    d.nlight = scene.light_tri.size();
+    d.light_total = float(scene.light_total);
+    d.texels = texels.data();
+    d.ntexel = texels.size() / 3;
+    return d;
+}
+
+void marshal_scene(const Scene& scene, MarshalledScene& out) {
+    out.tris.resize(scene.tris.size());
+    out.texels.clear();
+    std::unordered_map<const Texture*, int> atlas;
+    for (size_t i = 0; i < scene.tris.size(); ++i) {
+        const Triangle& s = scene.tris[i];
+        rm_triangle& t = out.tris[i];
+        t.v0 = pack(s.v0); t.v1 = pack(s.v1); t.v2 = pack(s.v2);
+        t.n = pack(s.n); t.nn = pack(s.nn);
+        t.n0 = pack(s.n0); t.n1 = pack(s.n1); t.n2 = pack(s.n2);
+        t.ar = float(s.ar); t.ag = float(s.ag); t.ab = float(s.ab);
+        t.er = float(s.er); t.eg = float(s.eg); t.eb = float(s.eb);
+        t.ior = float(s.ior); t.rough = float(s.rough); t.metallic = float(s.metallic); t.alpha = float(s.alpha);
+        t.shader = s.shader;
+        t.tu0 = float(s.tu0); t.tv0 = float(s.tv0); t.tu1 = float(s.tu1);
+        t.tv1 = float(s.tv1); t.tu2 = float(s.tu2); t.tv2 = float(s.tv2);
+        t.texture_offset = -1;
+        t.texture_width = 0;
+        t.texture_height = 0;
+        if (s.texture) {
+            auto ins = atlas.emplace(s.texture.get(), int(out.texels.size() / 3));
+            if (ins.second)
+                for (const Vec3& p : s.texture->pixels) {
+                    out.texels.push_back(float(p.x));
+                    out.texels.push_back(float(p.y));
+                    out.texels.push_back(float(p.z));
+                }
+            t.texture_offset = ins.first->second;
+            t.texture_width = s.texture->width;
+            t.texture_height = s.texture->height;
+        }
+    }
+    // Preorder with escape links: a node is followed by its left subtree, then
+    // its right subtree; `right` holds the index just past its own subtree.
+    out.nodes.clear();
+    out.nodes.reserve(scene.nodes.size());
+    struct Item { int index; bool exit; };
+    std::vector<Item> todo;
+    if (!scene.nodes.empty()) todo.push_back({0, false});
+    while (!todo.empty()) {
+        const Item it = todo.back();
+        todo.pop_back();
+        if (it.exit) {
+            out.nodes[size_t(it.index)].right = int32_t(out.nodes.size());
+            continue;
+        }
+        const BvhNode& s = scene.nodes.at(size_t(it.index));
+        rm_bvh_node n{};
+        n.mn = pack(s.mn);
+        n.mx = pack(s.mx);
+        n.offset = s.offset;
+        n.count = s.count;
+        const int at = int(out.nodes.size());
+        out.nodes.push_back(n);
+        todo.push_back({at, true});
+        if (!s.is_leaf()) {
+            todo.push_back({s.right, false});
+            todo.push_back({s.left, false});
+        }
+    }
+    out.cdf.assign(scene.light_cdf.begin(), scene.light_cdf.end());
+}
+
+rm_frame make_frame(const Camera& cam, const RenderConfig& cfg) {
+    rm_frame f{};
+    f.camera = {float(cam.x), float(cam.y), float(cam.z), float(cam.focal), float(cam.m0), float(cam.m1),
+                float(cam.m2), float(cam.m3), float(cam.m4), float(cam.m5), float(cam.m6), float(cam.m7),
+                float(cam.m8)};
+    f.width = cfg.width;
+    f.height = cfg.height;
+    f.spp = cfg.spp;
+    f.bounces = cfg.max_bounces;
+    f.resolution = float(cfg.resolution);
+    f.clamp_value = float(cfg.clamp);
+    f.adapt_rel = float(cfg.adapt_rel);
+    f.adapt_abs = float(cfg.adapt_abs);
+    f.adapt_min = std::getenv("UOW2_NOADAPT") ? 0 : cfg.adapt_min;
+    f.adapt_step = cfg.adapt_step > 0 ? cfg.adapt_step : 8;
+    f.nomis = std::getenv("UOW2_NOMIS") != nullptr;
+    const bool restir_off = env_is("RAYMOTION_RESTIR", "0");
+    f.restir_candidates = restir_off ? 1 : (cfg.restir_candidates > 0 ? cfg.restir_candidates : 1);
+    f.restir_candidates = std::min(32, std::max(1, f.restir_candidates));
+    f.restir_spatial = restir_off ? 0 : (cfg.restir_spatial > 0 ? cfg.restir_spatial : 0);
+    f.restir_spatial = std::min(8, std::max(0, f.restir_spatial));
+    f.restir_mcap = cfg.restir_mcap > 0 ? cfg.restir_mcap : 128;
+    f.restir_radius = cfg.restir_radius > 0 ? float(cfg.restir_radius) : 16.0f;
+    if (const char* v = std::getenv("RAYMOTION_RESTIR_CANDIDATES")) f.restir_candidates = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("RAYMOTION_RESTIR_SPATIAL")) f.restir_spatial = std::max(0, std::atoi(v));
+    f.denoise = env_is("RAYMOTION_DENOISE", "0") ? 0 : 1;
+    f.denoise_iterations = std::max(0, cfg.denoise_iterations);
+    f.temporal = env_is("RAYMOTION_RESTIR_TEMPORAL", "0") ? 0 : 1;
+    return f;
+}
+
+}  // namespace raymotion

@@ -1,0 +1,29 @@
+// Scene -> reference-layout arrays handed to the C-ABI (internal header).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "raymotion/pathtrace.hpp"
+#include "rm_hip.h"
+
+namespace raymotion {
+
+struct MarshalledScene {
+    std::vector<rm_triangle> tris;
+    std::vector<rm_bvh_node> nodes;
+    std::vector<float> cdf;
+    std::vector<float> texels;
+    rm_scene_desc desc(const Scene& scene) const;
+};
+
+// FP64 triangles -> 176-byte FP32 records with a deduplicated texture atlas
+// (metal.mm:109-128), binary BVH -> preorder array with escape links
+// (metal.mm:129-144), light CDF -> FP32 (metal.mm:145).
+void marshal_scene(const Scene& scene, MarshalledScene& out);
+
+// Reference host-side parameter handling (metal.mm:146-170, 251-252, 269-271):
+// defaults, clamps and environment switches.
+rm_frame make_frame(const Camera& camera, const RenderConfig& config);
+
+}  // namespace raymotion

@@ -1,0 +1,75 @@
+// render_image_hip / render_image_metal: the reference's C++ boundary
+// (include/raymotion/metal.hpp:6-7, src/engine/metal.mm:95-414) on top of the
+// C-ABI. One process-wide context, created on first use (metal.mm:102-103).
+#include <cstdlib>
+#include <mutex>
+
+#include "marshal.hpp"
+#include "raymotion/metal.hpp"
+
+namespace raymotion {
+namespace {
+
+struct Backend {
+    rm_ctx* ctx = nullptr;
+    std::string error;
+    Backend() {
+        int device = 0;
+        if (const char* v = std::getenv("RAYMOTION_HIP_DEVICE")) device = std::atoi(v);
+        if (rm_create(&ctx, device) != RM_OK) error = rm_last_error(nullptr);
+    }
+    ~Backend() {
+        if (ctx) rm_destroy(ctx);
+    }
+};
+
+Backend& backend() {
+    static Backend b;
+    return b;
+}
+
+}  // namespace
+
+bool render_image_hip_hdr(const Scene& scene, const Camera& cam, const RenderConfig& cfg, uint64_t seed,
+                          std::vector<uint8_t>& output, std::vector<float>* hdr, std::string& error) {
+    if (cfg.width <= 0 || cfg.height <= 0 || cfg.spp <= 0 || cfg.max_bounces < 0 ||
+        size_t(cfg.width) * size_t(cfg.height) > 100000000) {
+        error = "Invalid render dimensions or sample count";
+        return false;
+    }
+    Backend& b = backend();
+    if (!b.ctx) {
+        error = b.error;
+        return false;
+    }
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    static MarshalledScene marshalled;
+    marshal_scene(scene, marshalled);
+    const rm_scene_desc desc = marshalled.desc(scene);
+    if (rm_set_scene(b.ctx, &desc) != RM_OK) {
+        error = rm_last_error(b.ctx);
+        return false;
+    }
+    const rm_frame frame = make_frame(cam, cfg);
+    const size_t px = size_t(cfg.width) * size_t(cfg.height);
+    output.resize(px * 3);
+    if (hdr) hdr->resize(px * 3);
+    if (rm_render(b.ctx, &frame, seed, output.data(), hdr ? hdr->data() : nullptr) != RM_OK) {
+        error = rm_last_error(b.ctx);
+        return false;
+    }
+    return true;
+}
+
+bool render_image_hip(const Scene& scene, const Camera& cam, const RenderConfig& cfg, uint64_t seed,
+                      std::vector<uint8_t>& output, std::string& error) {
+    return render_image_hip_hdr(scene, cam, cfg, seed, output, nullptr, error);
+}
+
+bool render_image_metal(const Scene& scene, const Camera& cam, const RenderConfig& cfg, uint64_t seed,
+                        std::vector<uint8_t>& output, std::string& error) {
+    return render_image_hip(scene, cam, cfg, seed, output, error);
+}
+
+}  // namespace raymotion

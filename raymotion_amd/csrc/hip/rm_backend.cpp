@@ -1,0 +1,597 @@
+// Host runtime of the HIP backend: the C-ABI declared in include/rm_hip.h.
+//
+// Replaces the reference's Metal host code (src/engine/metal.mm:23-414): one
+// context per device holding the resident scene, grow-only frame buffers and
+// the temporal history; per frame the same branch decisions (ReSTIR +
+// denoise path or megakernel with 1/8/32 lanes), the same pass graph and the
+// same history ping-pong, issued on one HIP stream and synchronised once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rm_hip.h"
+#include "rm_kernels.h"
+#include "rm_layout.h"
+#include "rm_pack.h"
+
+namespace {
+
+using namespace rmh;
+
+std::mutex g_create_mu;
+std::string g_create_error;
+
+// Grow-only device allocation (the reference's upload/alloc helpers,
+// metal.mm:171-178,256-260).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    bool ensure(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 16);
+        if (bytes <= cap) return true;
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return false;
+        cap = bytes;
+        // Fresh buffers start zeroed so history that was never written reads
+        // as "no data" (kind 0, invalid reservoir, black).
+        return hipMemset(ptr, 0, bytes) == hipSuccess;
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct GBufSet {
+    DevBuf p[5];
+    bool ensure(size_t px) {
+        for (auto& b : p)
+            if (!b.ensure(px * sizeof(float4))) return false;
+        return true;
+    }
+    GBufferSoA soa() const {
+        return {p[0].as<float4>(), p[1].as<float4>(), p[2].as<float4>(), p[3].as<float4>(), p[4].as<float4>()};
+    }
+    void release() { for (auto& b : p) b.release(); }
+};
+
+struct ResSet {
+    DevBuf r[4];
+    bool ensure(size_t px) {
+        for (auto& b : r)
+            if (!b.ensure(px * sizeof(float4))) return false;
+        return true;
+    }
+    ReservoirSoA soa() const { return {r[0].as<float4>(), r[1].as<float4>(), r[2].as<float4>(), r[3].as<float4>()}; }
+    void release() { for (auto& b : r) b.release(); }
+};
+
+}  // namespace
+
+struct rm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    bool profiling = false;
+    hipEvent_t ev[16] = {};
+
+    // Resident scene.
+    bool has_scene = false;
+    DevBuf nodes, geo, nrm, nn, matid, mats, uv, lights, cdf, texels;
+    DevBuf level_nodes;
+    std::vector<int32_t> level_start;  // refit launch ranges into level_nodes, deepest level first
+    int32_t nnodes = 0, nlights = 0, opaque = 1, stack_need = 0, ntri = 0;
+    float light_total = 0.0f;
+    uint64_t content = 0, hist_key = 0;
+    std::vector<int32_t> prev_bvh_tri, prev_topo;
+    int refits = 0;
+    int last_refit = 0;
+
+    // Frame buffers.
+    GBufSet gbuf, hist_gbuf;
+    ResSet rinit, rmerged, hist_res;
+    DevBuf shade, den_a, hist_color;  // float4 per pixel
+    DevBuf rgb8, hdr3, mega_hdr;
+    DevBuf counters;
+    DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
+
+    // Temporal history (metal.mm:36-40).
+    rm_camera hist_cam{};
+    int hist_w = 0, hist_h = 0;
+    uint64_t hist_seed = 0, hist_hash = 0;
+    bool hist_valid = false;
+
+    rm_stats stats{};
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    DevScene dev_scene() const {
+        DevScene s{};
+        s.nodes = nodes.as<float4>();
+        s.geo = geo.as<float4>();
+        s.nrm = nrm.as<float4>();
+        s.nn = nn.as<float4>();
+        s.matid = matid.as<int32_t>();
+        s.mats = mats.as<float4>();
+        s.uv = uv.as<float4>();
+        s.lights = lights.as<float4>();
+        s.cdf = cdf.as<float>();
+        s.texels = texels.as<float>();
+        s.nnodes = nnodes;
+        s.nlights = nlights;
+        s.opaque = opaque;
+        s.light_total = light_total;
+        return s;
+    }
+    void drop_history() { hist_valid = false; }
+};
+
+namespace {
+
+int check(rm_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return RM_OK;
+    c->drop_history();
+    return c->fail(RM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Per-pass counters into rm_stats; index 5 holds the totals.
+void collect(rm_stats& st, const unsigned long long* cnt) {
+    st.rays[5] = st.node_visits[5] = st.tri_tests[5] = 0;
+    for (int p = 0; p < kNumPasses; ++p) {
+        st.rays[p] = cnt[3 * p];
+        st.node_visits[p] = cnt[3 * p + 1];
+        st.tri_tests[p] = cnt[3 * p + 2];
+        st.rays[5] += st.rays[p];
+        st.node_visits[5] += st.node_visits[p];
+        st.tri_tests[5] += st.tri_tests[p];
+    }
+}
+
+template <class T>
+bool upload(rm_ctx* c, DevBuf& b, const std::vector<T>& v) {
+    if (!b.ensure(v.size() * sizeof(T))) return false;
+    if (v.empty()) return true;
+    return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rm_create(rm_ctx** out, int hip_device) {
+    std::lock_guard<std::mutex> lock(g_create_mu);
+    if (!out) {
+        g_create_error = "rm_create: null output pointer";
+        return RM_ERR_INVALID;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        g_create_error = "No HIP GPU found";
+        return RM_ERR_NO_DEVICE;
+    }
+    if (hip_device < 0 || hip_device >= count) {
+        g_create_error = "HIP device index out of range";
+        return RM_ERR_INVALID;
+    }
+    if (hipSetDevice(hip_device) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return RM_ERR_DEVICE;
+    }
+    rm_ctx* c = new rm_ctx();
+    c->device = hip_device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        g_create_error = "Could not create HIP stream";
+        return RM_ERR_DEVICE;
+    }
+    for (auto& e : c->ev) hipEventCreate(&e);
+    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long))) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        g_create_error = "HIP counter allocation failed";
+        return RM_ERR_ALLOC;
+    }
+    g_create_error.clear();
+    *out = c;
+    return RM_OK;
+}
+
+void rm_destroy(rm_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->nodes, &c->geo, &c->nrm, &c->nn, &c->matid, &c->mats, &c->uv, &c->lights, &c->cdf,
+                      &c->texels, &c->level_nodes, &c->shade, &c->den_a, &c->hist_color, &c->rgb8, &c->hdr3,
+                      &c->mega_hdr, &c->counters, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
+                      &c->probe_tuv};
+    for (DevBuf* b : bufs) b->release();
+    c->gbuf.release();
+    c->hist_gbuf.release();
+    c->rinit.release();
+    c->rmerged.release();
+    c->hist_res.release();
+    for (auto& e : c->ev) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* rm_last_error(const rm_ctx* c) {
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int rm_set_profiling(rm_ctx* c, int enable) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->profiling = enable != 0;
+    return RM_OK;
+}
+
+int rm_get_stats(rm_ctx* c, rm_stats* out) {
+    if (!c || !out) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    *out = c->stats;
+    return RM_OK;
+}
+
+int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
+    if (!c || !s) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    hipSetDevice(c->device);
+    const uint64_t content = content_key(*s);
+    if (c->has_scene && content == c->content && size_t(c->ntri) == s->ntri) {
+        c->last_refit = 0;
+        c->stats.refit = 0;
+        return RM_OK;  // identical scene already resident
+    }
+    PackedScene ps;
+    std::string perr;
+    if (!pack_scene(*s, ps, perr)) return c->fail(RM_ERR_INVALID, perr);
+    if (ps.stack_need > kMaxStack)
+        return c->fail(RM_ERR_INVALID, "BVH too deep: the traversal stack would need " + std::to_string(ps.stack_need) +
+                                           " entries (at most " + std::to_string(kMaxStack) + ")");
+    // Refit in place when the topology is unchanged, at most 31 times in a row
+    // (metal.mm:211); otherwise upload the tree collapsed from the given nodes.
+    const bool refit = c->has_scene && c->refits < 31 && same_topology(*s, c->prev_bvh_tri, c->prev_topo);
+    c->has_scene = false;
+    bool ok = upload(c, c->geo, ps.geo) && upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn) &&
+              upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
+              upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf) && upload(c, c->texels, ps.texels);
+    if (ok && !refit) {
+        ok = upload(c, c->nodes, ps.nodes);
+        // Nodes grouped by depth, deepest level first, for the refit launches.
+        const int nn4 = int(ps.depth.size());
+        int maxd = 0;
+        for (int d : ps.depth) maxd = std::max(maxd, d);
+        std::vector<int32_t> order;
+        order.reserve(size_t(nn4));
+        c->level_start.assign(1, 0);
+        for (int d = maxd; d >= 0 && nn4 > 0; --d) {
+            for (int i = 0; i < nn4; ++i)
+                if (ps.depth[size_t(i)] == d) order.push_back(i);
+            c->level_start.push_back(int32_t(order.size()));
+        }
+        ok = ok && upload(c, c->level_nodes, order);
+        c->refits = 0;
+    }
+    if (!ok) {
+        c->drop_history();
+        return c->fail(RM_ERR_ALLOC, "HIP scene buffer allocation failed");
+    }
+    if (refit) {
+        for (size_t l = 0; l + 1 < c->level_start.size(); ++l)
+            launch_refit_level(c->nodes.as<float4>(), c->geo.as<float4>(),
+                               c->level_nodes.as<int32_t>() + c->level_start[l],
+                               c->level_start[l + 1] - c->level_start[l], c->stream);
+        ++c->refits;
+    }
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "scene upload")) return rc;
+    c->nnodes = int32_t(ps.depth.size());
+    c->nlights = int32_t(s->nlight);
+    c->opaque = ps.opaque;
+    c->stack_need = ps.stack_need;
+    c->light_total = ps.light_total;
+    c->ntri = int32_t(s->ntri);
+    c->content = content;
+    c->hist_key = history_key(*s);
+    topology_of(*s, c->prev_bvh_tri, c->prev_topo);
+    c->has_scene = true;
+    c->last_refit = refit ? 1 : 2;
+    c->stats.refit = c->last_refit;
+    c->stats.bvh_nodes = c->nnodes;
+    c->stats.stack_depth = c->stack_need;
+    return RM_OK;
+}
+
+int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr) {
+    if (!c || !f) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    hipSetDevice(c->device);
+    if (f->width <= 0 || f->height <= 0 || f->spp <= 0 || f->bounces < 0 ||
+        size_t(f->width) * size_t(f->height) > 100000000)
+        return c->fail(RM_ERR_INVALID, "Invalid render dimensions or sample count");
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    const size_t px = size_t(f->width) * size_t(f->height);
+    const int npx = int(px);
+
+    FrameParams P{};
+    P.cam = f->camera;
+    P.width = f->width; P.height = f->height; P.spp = f->spp; P.bounces = f->bounces;
+    P.adapt_min = f->adapt_min;
+    P.adapt_step = f->adapt_step > 0 ? f->adapt_step : 8;
+    P.nomis = f->nomis;
+    P.resolution = f->resolution; P.clamp_value = f->clamp_value;
+    P.adapt_rel = f->adapt_rel; P.adapt_abs = f->adapt_abs;
+    P.restir_candidates = f->restir_candidates; P.restir_spatial = f->restir_spatial;
+    P.restir_mcap = f->restir_mcap; P.restir_radius = f->restir_radius;
+    P.seed = seed;
+    const DevScene sc = c->dev_scene();
+
+    // Branch choice (metal.mm:232-254), with the reference's record sizes.
+    const size_t restir_bytes = px * (sizeof(rm_gbuffer) + 2 * sizeof(rm_reservoir));
+    const bool use_spatial = P.restir_spatial > 0 && c->nlights > 0 && P.bounces > 0 && c->opaque &&
+                             restir_bytes <= size_t(512) * 1024 * 1024;
+    bool have_history = use_spatial && c->hist_valid && c->hist_w == f->width && c->hist_h == f->height &&
+                        seed != c->hist_seed && c->hist_key == c->hist_hash;
+    if (!f->temporal) have_history = false;
+    P.has_history = have_history ? 1 : 0;
+    P.prev = c->hist_cam;
+
+    if (!c->rgb8.ensure(px * 3) || (hdr && !c->hdr3.ensure(px * 3 * sizeof(float)))) {
+        c->drop_history();
+        return c->fail(RM_ERR_ALLOC, "HIP image buffer allocation failed");
+    }
+    unsigned long long* cnt = c->counters.as<unsigned long long>();
+    if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
+        return rc;
+    rm_stats st{};
+    st.bvh_nodes = c->nnodes;
+    st.stack_depth = c->stack_need;
+    st.refit = c->last_refit;
+    st.has_history = P.has_history;
+    const bool prof = c->profiling;
+    auto mark = [&](int i) {
+        if (prof) hipEventRecord(c->ev[i], c->stream);
+    };
+    mark(0);
+    int denoise_passes = 0;
+    if (use_spatial) {
+        st.path = 1;
+        if (!c->gbuf.ensure(px) || !c->hist_gbuf.ensure(px) || !c->rinit.ensure(px) || !c->rmerged.ensure(px) ||
+            !c->hist_res.ensure(px) || !c->shade.ensure(px * sizeof(float4)) || !c->den_a.ensure(px * sizeof(float4)) ||
+            !c->hist_color.ensure(px * sizeof(float4))) {
+            c->drop_history();
+            return c->fail(RM_ERR_ALLOC, "HIP ReSTIR buffer allocation failed");
+        }
+        const GBufferSoA gb = c->gbuf.soa(), hg = c->hist_gbuf.soa();
+        const ReservoirSoA ri = c->rinit.soa(), rmg = c->rmerged.soa(), hr = c->hist_res.soa();
+        launch_generate(c->stack_need, sc, P, gb, ri, cnt, c->stream);
+        mark(1);
+        launch_spatial(c->stack_need, sc, P, gb, ri, rmg, hg, hr, cnt, c->stream);
+        mark(2);
+        launch_shade(c->stack_need, sc, P, gb, rmg, c->shade.as<float4>(), cnt, c->stream);
+        mark(3);
+        const float4* final_img = c->shade.as<float4>();
+        bool final_in_hist = false;
+        if (f->denoise) {
+            // Temporal pass into A, then a-trous passes ping-ponging between A
+            // and the history color buffer (its old content has been consumed).
+            float4* a = c->den_a.as<float4>();
+            float4* b = c->hist_color.as<float4>();
+            launch_temporal(P, gb, hg, c->shade.as<float4>(), c->hist_color.as<float4>(), a, c->stream);
+            mark(4);
+            const int iters = std::max(0, f->denoise_iterations);
+            for (int s = 0; s < iters; ++s) {
+                launch_atrous(P, gb, a, b, s, c->stream);
+                std::swap(a, b);
+            }
+            denoise_passes = iters;
+            mark(5);
+            final_img = a;
+            final_in_hist = a == c->hist_color.as<float4>();
+        } else {
+            mark(4);
+            mark(5);
+        }
+        launch_tonemap(npx, final_img, c->rgb8.as<uint8_t>(), hdr ? c->hdr3.as<float>() : nullptr, c->stream);
+        mark(6);
+        if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+        if (rgb8)
+            if (int rc = check(c, hipMemcpyAsync(rgb8, c->rgb8.ptr, px * 3, hipMemcpyDeviceToHost, c->stream), "readback"))
+                return rc;
+        if (hdr)
+            if (int rc = check(c, hipMemcpyAsync(hdr, c->hdr3.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                                 c->stream), "readback"))
+                return rc;
+        unsigned long long host_cnt[kNumCounters];
+        if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
+            return rc;
+        if (int rc = check(c, hipStreamSynchronize(c->stream), "frame")) return rc;
+        if (host_cnt[kCntOverflow]) {
+            c->drop_history();
+            return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+        }
+        // Publish this frame as the next frame's history (metal.mm:374-385).
+        std::swap(c->gbuf, c->hist_gbuf);
+        std::swap(c->rmerged, c->hist_res);
+        if (f->denoise && !final_in_hist) std::swap(c->den_a, c->hist_color);
+        c->hist_cam = f->camera;
+        c->hist_w = f->width;
+        c->hist_h = f->height;
+        c->hist_seed = seed;
+        c->hist_hash = c->hist_key;
+        c->hist_valid = true;
+        collect(st, host_cnt);
+    } else {
+        // Megakernel with 1, 8 or 32 lanes per pixel (metal.mm:390-394).
+        int lanes = 1;
+        if (f->spp >= 8 && (P.adapt_min <= 0 || P.adapt_step % 8 == 0)) lanes = 8;
+        if (f->spp >= 32 && (P.adapt_min <= 0 || P.adapt_step % 32 == 0)) lanes = 32;
+        st.path = 0;
+        st.lanes = lanes;
+        if (hdr && !c->mega_hdr.ensure(px * sizeof(float4))) {
+            c->drop_history();
+            return c->fail(RM_ERR_ALLOC, "HIP image buffer allocation failed");
+        }
+        launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(), hdr ? c->mega_hdr.as<float4>() : nullptr,
+                            cnt, c->stream);
+        mark(1);
+        if (hdr) launch_tonemap(npx, c->mega_hdr.as<float4>(), c->rgb8.as<uint8_t>(), c->hdr3.as<float>(), c->stream);
+        if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+        if (rgb8)
+            if (int rc = check(c, hipMemcpyAsync(rgb8, c->rgb8.ptr, px * 3, hipMemcpyDeviceToHost, c->stream), "readback"))
+                return rc;
+        if (hdr)
+            if (int rc = check(c, hipMemcpyAsync(hdr, c->hdr3.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                                 c->stream), "readback"))
+                return rc;
+        unsigned long long host_cnt[kNumCounters];
+        if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
+            return rc;
+        if (int rc = check(c, hipStreamSynchronize(c->stream), "frame")) return rc;
+        if (host_cnt[kCntOverflow]) {
+            c->drop_history();
+            return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+        }
+        collect(st, host_cnt);
+    }
+    if (prof) {
+        auto ms = [&](int a, int b) {
+            float v = 0.0f;
+            hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
+            return v;
+        };
+        if (st.path == 1) {
+            st.ms[1] = ms(0, 1);
+            st.ms[2] = ms(1, 2);
+            st.ms[3] = ms(2, 3);
+            st.ms[4] = ms(3, 4);
+            st.ms[5] = denoise_passes > 0 ? ms(4, 5) : 0.0f;
+            st.ms[6] = ms(5, 6);
+            st.ms[7] = ms(0, 6);
+        } else {
+            st.ms[0] = ms(0, 1);
+            st.ms[7] = ms(0, 1);
+        }
+    }
+    c->stats = st;
+    c->err.clear();
+    return RM_OK;
+}
+
+int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tmax, size_t n, int any, int32_t* tri,
+             float* t, float* u, float* v) {
+    if (!c || (n > 0 && (!origins || !dirs || !tri))) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    hipSetDevice(c->device);
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    if (n == 0) return RM_OK;
+    if (n > (size_t(1) << 30)) return c->fail(RM_ERR_INVALID, "too many rays in one call");
+    if (!c->probe_o.ensure(n * 12) || !c->probe_d.ensure(n * 12) || (tmax && !c->probe_t.ensure(n * 4)) ||
+        !c->probe_tri.ensure(n * 4) || !c->probe_tuv.ensure(n * 16))
+        return c->fail(RM_ERR_ALLOC, "HIP ray buffer allocation failed");
+    hipStream_t s = c->stream;
+    unsigned long long* cnt = c->counters.as<unsigned long long>();
+    hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), s);
+    hipMemcpyAsync(c->probe_o.ptr, origins, n * 12, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(c->probe_d.ptr, dirs, n * 12, hipMemcpyHostToDevice, s);
+    if (tmax) hipMemcpyAsync(c->probe_t.ptr, tmax, n * 4, hipMemcpyHostToDevice, s);
+    FrameParams P{};
+    const DevScene sc = c->dev_scene();
+    if (c->profiling) hipEventRecord(c->ev[0], s);
+    launch_trace(c->stack_need, sc, P, int(n), c->probe_o.as<float>(), c->probe_d.as<float>(),
+                 tmax ? c->probe_t.as<float>() : nullptr, any, c->probe_tri.as<int32_t>(), c->probe_tuv.as<float4>(),
+                 cnt, s);
+    if (c->profiling) hipEventRecord(c->ev[1], s);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    std::vector<float> tuv(n * 4);
+    unsigned long long host_cnt[kNumCounters];
+    hipMemcpyAsync(tri, c->probe_tri.ptr, n * 4, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(tuv.data(), c->probe_tuv.ptr, n * 16, hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, s);
+    if (int rc = check(c, hipStreamSynchronize(s), "trace")) return rc;
+    if (host_cnt[kCntOverflow]) return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+    for (size_t i = 0; i < n; ++i) {
+        if (t) t[i] = tuv[4 * i];
+        if (u) u[i] = tuv[4 * i + 1];
+        if (v) v[i] = tuv[4 * i + 2];
+    }
+    rm_stats st{};
+    collect(st, host_cnt);
+    st.bvh_nodes = c->nnodes;
+    st.stack_depth = c->stack_need;
+    if (c->profiling) hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]);
+    c->stats = st;
+    return RM_OK;
+}
+
+int rm_read_restir(rm_ctx* c, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    hipSetDevice(c->device);
+    if (!c->hist_valid) return c->fail(RM_ERR_INVALID, "no ReSTIR frame rendered yet");
+    const size_t px = size_t(c->hist_w) * size_t(c->hist_h);
+    std::vector<float4> a(px), b(px), d(px), e(px), g(px);
+    auto get = [&](const DevBuf& buf, std::vector<float4>& dst) {
+        return hipMemcpy(dst.data(), buf.ptr, px * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    // After a frame the G-buffer and merged reservoirs live in the history slots.
+    if (gbuf) {
+        const GBufSet& s = c->hist_gbuf;
+        if (!get(s.p[0], a) || !get(s.p[1], b) || !get(s.p[2], d) || !get(s.p[3], e) || !get(s.p[4], g))
+            return c->fail(RM_ERR_DEVICE, "readback failed");
+        for (size_t i = 0; i < px; ++i) {
+            rm_gbuffer& o = gbuf[i];
+            o.pos = {a[i].x, a[i].y, a[i].z};
+            std::memcpy(&o.kind, &a[i].w, 4);
+            o.normal = {b[i].x, b[i].y, b[i].z};
+            o.rough = b[i].w;
+            o.albedo = {d[i].x, d[i].y, d[i].z};
+            o.metallic = d[i].w;
+            o.wo = {e[i].x, e[i].y, e[i].z};
+            o.ior = e[i].w;
+            o.emission = {g[i].x, g[i].y, g[i].z};
+        }
+    }
+    auto res = [&](const ResSet& s, rm_reservoir* out) {
+        if (!get(s.r[0], a) || !get(s.r[1], b) || !get(s.r[2], d) || !get(s.r[3], e)) return false;
+        for (size_t i = 0; i < px; ++i) {
+            rm_reservoir& o = out[i];
+            o.light_pos = {a[i].x, a[i].y, a[i].z};
+            o.p_hat = a[i].w;
+            o.light_normal = {b[i].x, b[i].y, b[i].z};
+            o.alpha = b[i].w;
+            o.emission = {d[i].x, d[i].y, d[i].z};
+            o.sum_ws = d[i].w;
+            o.sum_wu = e[i].x;
+            std::memcpy(&o.m, &e[i].y, 4);
+            std::memcpy(&o.valid, &e[i].z, 4);
+        }
+        return true;
+    };
+    if (rinit && !res(c->rinit, rinit)) return c->fail(RM_ERR_DEVICE, "readback failed");
+    if (rmerged && !res(c->hist_res, rmerged)) return c->fail(RM_ERR_DEVICE, "readback failed");
+    if (shade) {
+        if (!get(c->shade, a)) return c->fail(RM_ERR_DEVICE, "readback failed");
+        for (size_t i = 0; i < px; ++i) {
+            shade[3 * i] = a[i].x;
+            shade[3 * i + 1] = a[i].y;
+            shade[3 * i + 2] = a[i].z;
+        }
+    }
+    return RM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,776 @@
+// Device functions of the path tracer: RNG, 4-wide BVH traversal, surface
+// lookup, BSDFs, light sampling, dielectrics and the bounce loop.
+//
+// Each function computes what the same-named function of the reference
+// computes (cited per function) with the same IEEE float expressions in the
+// same evaluation order; the library is compiled without FP contraction and
+// with correctly rounded division and square root, so the kernels produce the
+// same per-pixel results as a host evaluation of those expressions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/rm_sincos.h"
+#include "rm_layout.h"
+
+namespace rmh {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross3(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 vmin(V3 a, V3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+__device__ __forceinline__ V3 vmax(V3 a, V3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+__device__ __forceinline__ V3 xyz(float4 f) { return {f.x, f.y, f.z}; }
+__device__ __forceinline__ float lum3(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+__device__ __forceinline__ int ibits(float f) { return __float_as_int(f); }
+
+// PCG RXS-M-XS 32 seeded by splitmix64 (pathtrace.metal:93-101).
+struct Rng {
+    uint32_t state;
+    __device__ __forceinline__ void seed(uint64_t s) {
+        uint64_t z = s + 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        state = uint32_t(z ^ (z >> 31));
+    }
+    __device__ __forceinline__ float next() {
+        state = state * 747796405u + 2891336453u;
+        const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+        return float(((word >> 22u) ^ word) >> 8u) * 0x1p-24f;
+    }
+};
+
+__device__ __forceinline__ uint64_t pixel_seed(uint64_t seed, uint32_t x, uint32_t y) {
+    return seed ^ (uint64_t(y + 1) * 0x9E3779B97F4A7C15ULL) ^ (uint64_t(x + 1) * 0xBF58476D1CE4E5B9ULL);
+}
+// restir_spatial's salt: the low 64 bits of the 17-digit literal at pathtrace.metal:937.
+constexpr uint64_t kSpatialSalt = 0x1B54A35A6F31B458ULL;
+
+struct Hit {
+    float t, u, v;
+    int tri;
+};
+
+// Surface data at a hit (HitInfo, pathtrace.metal:11-23).
+struct Surface {
+    float t, u, v;
+    int tri;
+    V3 p, n_interp, n_geom;
+    float ar, ag, ab, er, eg, eb, ior, rough, metallic, alpha;
+    int shader;
+};
+
+struct Tally {
+    uint32_t rays, nodes, tris, overflow;
+};
+
+struct Bsdf {
+    float r, g, b, pdf;
+};
+
+struct DielectricSample {
+    bool valid;
+    float n_x, n_y, n_z, eta_scale;
+};
+
+// Per-lane tracer state. S is the capacity of the lane's LDS traversal stack;
+// stk points at this lane's column of a [S][kBlock] shared array.
+template <int S>
+struct Tracer {
+    const DevScene& sc;
+    const FrameParams& P;
+    uint32_t* stk;
+    Rng rng;
+    Tally tl;
+
+    __device__ Tracer(const DevScene& s, const FrameParams& p, uint32_t* stack) : sc(s), P(p), stk(stack) {
+        rng.state = 0;
+        tl = {0, 0, 0, 0};
+    }
+    __device__ __forceinline__ float urand() { return rng.next(); }
+
+    // ------------------------------------------------------------ traversal
+    // Closest hit (or, with any, the first hit) along o + t d for t in
+    // (1e-8, tmax). Children of each 4-wide node are tested together; hit leaves
+    // are intersected nearest-first and hit inner children are descended
+    // nearest-first with the others kept on the LDS stack. A box is skipped
+    // when its entry lies beyond the current hit by more than 2^-12 (relative),
+    // and equal distances resolve to the smaller triangle index, so the result
+    // is the closest hit regardless of visit order (same rule as the oracle).
+    __device__ Hit intersect(V3 ro, V3 rd, float tmax, bool any) {
+        Hit h;
+        h.t = tmax;
+        h.tri = -1;
+        h.u = 0.0f;
+        h.v = 0.0f;
+        ++tl.rays;
+        if (sc.nnodes == 0) return h;
+        const float ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x);
+        const float iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y);
+        const float iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
+        const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
+        int sp = 0;
+        int node = 0;
+        for (;;) {
+            const float4* n = sc.nodes + size_t(node) * kNodeWords;
+            const float4 lox = n[0], hix = n[1], loy = n[2], hiy = n[3], loz = n[4], hiz = n[5];
+            const float4 rf = n[6], kd = n[7];
+            ++tl.nodes;
+            const float tc = h.t + h.t * (1.0f / 4096.0f);
+            float tn[4];
+            int ref[4], kind[4];
+            {
+                const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+                const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+                const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+                const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
+                const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float ax = __builtin_fmaf(lx[c], ix, ox), bx = __builtin_fmaf(hx[c], ix, ox);
+                    const float ay = __builtin_fmaf(ly[c], iy, oy), by = __builtin_fmaf(hy[c], iy, oy);
+                    const float az = __builtin_fmaf(lz[c], iz, oz), bz = __builtin_fmaf(hz[c], iz, oz);
+                    const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+                    const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
+                    const bool ok = k4[c] >= 0 && nr <= fr;
+                    tn[c] = ok ? nr : __builtin_inff();
+                    ref[c] = r4[c];
+                    kind[c] = ok ? k4[c] : -1;
+                }
+            }
+            // Sort the four slots by entry distance (misses last).
+#define RM_CX(a, b)                                                   \
+    if (tn[b] < tn[a]) {                                              \
+        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
+        const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
+        const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
+    }
+            RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
+#undef RM_CX
+            // Leaves first, nearest first.
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (kind[c] > 0 && tn[c] <= h.t + h.t * (1.0f / 4096.0f)) {
+                    const int first = ref[c], count = kind[c];
+                    for (int k = 0; k < count; ++k) {
+                        const float4* g = sc.geo + size_t(first + k) * kTriGeoWords;
+                        const float4 g0 = g[0], g1 = g[1], g2 = g[2];
+                        ++tl.tris;
+                        const V3 v0{g0.x, g0.y, g0.z}, v1{g0.w, g1.x, g1.y}, v2{g1.z, g1.w, g2.x};
+                        const int id = ibits(g2.y);
+                        const V3 e1 = v1 - v0, e2 = v2 - v0;
+                        const V3 p = cross3(rd, e2);
+                        const float det = dot3(e1, p);
+                        if (fabsf(det) < 1e-12f) continue;
+                        const V3 tv = ro - v0;
+                        const float u = dot3(tv, p) / det;
+                        if (u < 0.0f || u > 1.0f) continue;
+                        const V3 q = cross3(tv, e1);
+                        const float v = dot3(rd, q) / det;
+                        if (v < 0.0f || u + v > 1.0f) continue;
+                        const float d = dot3(e2, q) / det;
+                        if (d > 1e-8f && (d < h.t || (d == h.t && h.tri >= 0 && id < h.tri))) {
+                            h.t = d;
+                            h.tri = id;
+                            h.u = u;
+                            h.v = v;
+                            if (any) return h;
+                        }
+                    }
+                }
+            }
+            // Inner children: continue with the nearest, stack the rest.
+            int next = -1;
+#pragma unroll
+            for (int c = 3; c >= 0; --c) {
+                if (kind[c] == 0) {
+                    if (next >= 0) {
+                        if (sp < S) {
+                            stk[sp * kBlock] = uint32_t(next);
+                            ++sp;
+                        } else {
+                            tl.overflow = 1;
+                        }
+                    }
+                    next = ref[c];
+                }
+            }
+            if (next < 0) {
+                if (sp == 0) break;
+                --sp;
+                next = int(stk[sp * kBlock]);
+            }
+            node = next;
+        }
+        return h;
+    }
+
+    // cast_ray (pathtrace.metal:165-204): hit + interpolated normal + material.
+    __device__ Surface cast_ray(V3 ro, V3 rd, float dist) {
+        const Hit h = intersect(ro, rd, dist, false);
+        Surface s;
+        s.t = h.t; s.u = h.u; s.v = h.v; s.tri = h.tri;
+        s.p = ro + rd * h.t;
+        s.ar = s.ag = s.ab = s.er = s.eg = s.eb = s.ior = s.rough = s.metallic = 0.0f;
+        s.alpha = 1.0f;
+        s.shader = 0;
+        s.n_interp = {0.0f, 0.0f, 0.0f};
+        s.n_geom = {0.0f, 0.0f, 0.0f};
+        if (h.tri >= 0) {
+            const float4* nr = sc.nrm + size_t(h.tri) * kTriNrmWords;
+            const float4 a = nr[0], b = nr[1], c = nr[2];
+            const float w = 1.0f - (h.u + h.v);
+            const float nx = w * a.x + (h.u * b.x + h.v * c.x);
+            const float ny = w * a.y + (h.u * b.y + h.v * c.y);
+            const float nz = w * a.z + (h.u * b.z + h.v * c.z);
+            const float inv = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+            s.n_interp = {nx * inv, ny * inv, nz * inv};
+            s.n_geom = {a.w, b.w, c.w};
+            const float4* m = sc.mats + size_t(sc.matid[h.tri]) * kMatWords;
+            const float4 m0 = m[0], m1 = m[1], m2 = m[2];
+            s.ar = m0.x; s.ag = m0.y; s.ab = m0.z; s.alpha = m0.w;
+            s.er = m1.x; s.eg = m1.y; s.eb = m1.z; s.ior = m1.w;
+            s.rough = m2.x; s.metallic = m2.y; s.shader = ibits(m2.z);
+            const int texoff = ibits(m2.w);
+            if (texoff >= 0) {
+                const float4 m3 = m[3];
+                const int tw = ibits(m3.x), th = ibits(m3.y);
+                const float4 uv0 = sc.uv[size_t(h.tri) * 2], uv1 = sc.uv[size_t(h.tri) * 2 + 1];
+                float fu = w * uv0.x + h.u * uv0.z + h.v * uv1.x;
+                float fv = w * uv0.y + h.u * uv0.w + h.v * uv1.y;
+                fu = fminf(fu - floorf(fu), 0.99999994f);
+                fv = fminf(fv - floorf(fv), 0.99999994f);
+                const int x = min(int(fu * float(tw)), tw - 1);
+                const int y = th - 1 - min(int(fv * float(th)), th - 1);
+                const float* t = sc.texels + 3 * size_t(texoff + y * tw + x);
+                s.ar *= t[0]; s.ag *= t[1]; s.ab *= t[2];
+            }
+            if (s.shader == 1) {
+                // Hue from azimuth through hsv2rgb(h, 0.8, 1) (pathtrace.metal:27-64,193-201).
+                const float kpi = 3.14159265358979323846f;
+                const float az = atanf(s.p.x / s.p.z) * 180.0f / kpi + ((s.p.z > 0.0f) ? 0.0f : 180.0f);
+                const float hh = (az - floorf(az / 360.0f) * 360.0f) / 360.0f;
+                const float cc = 0.8f * 1.0f;
+                const float q = hh / 60.0f;
+                const float xx = cc * (1.0f - fabsf((q - floorf(q / 2.0f) * 2.0f) - 1.0f));
+                const float mm = 1.0f - cc;
+                float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+                if (hh < 60.0f) { cr = cc; cg = xx; }
+                else if (hh < 120.0f) { cr = xx; cg = cc; }
+                else if (hh < 180.0f) { cg = cc; cb = xx; }
+                else if (hh < 240.0f) { cg = xx; cb = cc; }
+                else if (hh < 300.0f) { cr = xx; cb = cc; }
+                else if (hh < 360.0f) { cr = cc; cb = xx; }
+                s.ar = (cr + mm) * 255.0f; s.ag = (cg + mm) * 255.0f; s.ab = (cb + mm) * 255.0f;
+            }
+        }
+        return s;
+    }
+
+    // cast_alpha_ray (pathtrace.metal:536-550): stochastic coverage.
+    __device__ Surface cast_alpha_ray(V3 ro, V3 rd, float dist) {
+        if (sc.opaque) return cast_ray(ro, rd, dist);
+        float traveled = 0.0f;
+        while (traveled < dist) {
+            Surface s = cast_ray(ro + rd * traveled, rd, dist - traveled);
+            if (s.tri < 0) return s;
+            const float opacity = s.alpha;
+            if (opacity >= 1.0f || (opacity > 0.0f && urand() < opacity)) {
+                s.t += traveled;
+                return s;
+            }
+            traveled += s.t + 0.0001f;
+        }
+        Surface miss;
+        miss.t = 1e30f; miss.tri = -1; miss.u = miss.v = 0.0f;
+        miss.ar = miss.ag = miss.ab = miss.er = miss.eg = miss.eb = miss.ior = miss.rough = miss.metallic = 0.0f;
+        miss.alpha = 1.0f; miss.shader = 0;
+        miss.p = miss.n_interp = miss.n_geom = {0.0f, 0.0f, 0.0f};
+        return miss;
+    }
+
+    __device__ bool occluded(V3 o, V3 d, float dist) {
+        if (sc.opaque) return intersect(o, d, dist, true).tri >= 0;
+        return cast_alpha_ray(o, d, dist).tri >= 0;
+    }
+
+    // ------------------------------------------------------------ sampling
+    static __device__ __forceinline__ void make_tangent(float nx, float ny, float nz, float& tx, float& ty, float& tz,
+                                                        float& bx, float& by, float& bz) {
+        if (fabsf(nx) > 0.1f) { tx = nz; ty = 0.0f; tz = -nx; }
+        else { tx = 0.0f; ty = -nz; tz = ny; }
+        const float l = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
+        tx *= l; ty *= l; tz *= l;
+        bx = ny * tz - nz * ty;
+        by = nz * tx - nx * tz;
+        bz = nx * ty - ny * tx;
+    }
+
+    // sample_cosine_hemisphere (pathtrace.metal:205-220)
+    __device__ void sample_cosine(float nx, float ny, float nz, float& ox, float& oy, float& oz) {
+        const float ru = urand();
+        const float phi = urand() * 6.2831855f;
+        float sp, c;
+        rm_sincosf(phi, &sp, &c);
+        const float rr = sqrtf(ru);
+        const float lx = rr * c, ly = rr * sp, lz = sqrtf(fmaxf(0.0f, 1.0f - ru));
+        float tx, ty, tz, bx, by, bz;
+        make_tangent(nx, ny, nz, tx, ty, tz, bx, by, bz);
+        ox = tx * lx + (bx * ly + nx * lz);
+        oy = ty * lx + (by * ly + ny * lz);
+        oz = tz * lx + (bz * ly + nz * lz);
+    }
+
+    // sample_ggx_half (pathtrace.metal:221-242); its first draw is unused.
+    __device__ void sample_ggx_half(float alpha2, float nx, float ny, float nz, float& hx, float& hy, float& hz) {
+        (void)urand();
+        const float rv = urand();
+        const float phi = urand() * 6.2831855f;
+        float sp, c;
+        rm_sincosf(phi, &sp, &c);
+        const float cs = sqrtf((1.0f - rv) / (1.0f + (alpha2 - 1.0f) * rv));
+        const float sn = sqrtf(fmaxf(0.0f, 1.0f - cs * cs));
+        const float lx = sn * c, ly = sn * sp, lz = cs;
+        float tx, ty, tz, bx, by, bz;
+        make_tangent(nx, ny, nz, tx, ty, tz, bx, by, bz);
+        hx = tx * lx + (bx * ly + nx * lz);
+        hy = ty * lx + (by * ly + ny * lz);
+        hz = tz * lx + (bz * ly + nz * lz);
+        const float d = 1.0f / sqrtf(hx * hx + hy * hy + hz * hz);
+        hx *= d; hy *= d; hz *= d;
+    }
+
+    // evaluate_opaque_bsdf (pathtrace.metal:243-295): Lambert + GGX/Smith/Schlick.
+    static __device__ Bsdf eval_bsdf(float wi_x, float wi_y, float wi_z, float wo_x, float wo_y, float wo_z, float n_x,
+                                     float n_y, float n_z, float alpha2, float metallic, float spec_prob, float ar,
+                                     float ag, float ab) {
+        const float kpi = 3.14159265358979323846f;
+        Bsdf out{0.0f, 0.0f, 0.0f, 0.0f};
+        const float cos_i = wi_x * n_x + (wi_y * n_y + wi_z * n_z);
+        const float cos_o = wo_x * n_x + (wo_y * n_y + wo_z * n_z);
+        if (!(cos_i > 1e-8f && cos_o > 1e-8f)) return out;
+        float hx = wi_x + wo_x, hy = wi_y + wo_y, hz = wi_z + wo_z;
+        const float hl = sqrtf(hx * hx + hy * hy + hz * hz);
+        if (!(hl > 1e-8f)) return out;
+        const float inv = 1.0f / hl;
+        hx *= inv; hy *= inv; hz *= inv;
+        float cos_h = hx * n_x + (hy * n_y + hz * n_z);
+        float vo_h = wo_x * hx + (wo_y * hy + wo_z * hz);
+        if (cos_h < 0.0f) { cos_h = -cos_h; vo_h = -vo_h; }
+        if (!(vo_h > 1e-8f)) return out;
+        const float dd = cos_h * cos_h * (alpha2 - 1.0f) + 1.0f;
+        const float D = alpha2 / (kpi * dd * dd);
+        const float g1 = 2.0f * cos_o / (cos_o + sqrtf(cos_o * cos_o + alpha2 * (1.0f - cos_o * cos_o)));
+        const float g2 = 2.0f * cos_i / (cos_i + sqrtf(cos_i * cos_i + alpha2 * (1.0f - cos_i * cos_i)));
+        const float G = g1 * g2;
+        float om = 1.0f - vo_h;
+        om *= om;
+        om *= om;
+        om *= 1.0f - vo_h;
+        const float di = 1.0f - metallic;
+        const float f0r = 0.04f * di + ar * metallic, f0g = 0.04f * di + ag * metallic, f0b = 0.04f * di + ab * metallic;
+        const float fr = f0r + (1.0f - f0r) * om, fg = f0g + (1.0f - f0g) * om, fb = f0b + (1.0f - f0b) * om;
+        const float kdr = ar * di * (1.0f - fr), kdg = ag * di * (1.0f - fg), kdb = ab * di * (1.0f - fb);
+        const float base = D * G / (4.0f * cos_i * cos_o);
+        out.r = kdr / kpi + base * fr;
+        out.g = kdg / kpi + base * fg;
+        out.b = kdb / kpi + base * fb;
+        const float pdf_d = cos_i / kpi;
+        const float pdf_s = D * cos_h / (4.0f * vo_h);
+        out.pdf = (1.0f - spec_prob) * pdf_d + spec_prob * pdf_s;
+        return out;
+    }
+
+    __device__ int pick_light(float pick) {  // CDF binary search (pathtrace.metal:323-330)
+        int lo = 0, hi = sc.nlights - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (sc.cdf[mid] >= pick) hi = mid;
+            else lo = mid + 1;
+        }
+        return lo;
+    }
+
+    // direct_lighting (pathtrace.metal:299-469): legacy NEE for one candidate,
+    // otherwise RIS over Mc candidates with one shadow ray.
+    __device__ void direct_lighting(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
+                                    float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
+                                    float ag, float ab, float& or_, float& og, float& ob, float& out_sumW,
+                                    uint32_t& out_M) {
+        or_ = og = ob = 0.0f;
+        out_sumW = 0.0f;
+        out_M = 0;
+        if (sc.nlights == 0 || !(sc.light_total > 1e-12f)) return;
+        int Mc = P.restir_candidates;
+        if (Mc < 1) Mc = 1;
+        if (Mc > 32) Mc = 32;
+        float r2 = rough;
+        if (r2 < 0.0f) r2 = 0.0f;
+        if (r2 > 1.0f) r2 = 1.0f;
+        if (r2 < 0.02f) r2 = 0.02f;
+        const float alpha2 = (r2 * r2) * (r2 * r2);
+        const bool nomis = P.nomis != 0;
+        const float total = sc.light_total;
+        if (Mc <= 1) {
+            const float pick = urand() * total;
+            const float4* L = sc.lights + size_t(pick_light(pick)) * kLightWords;
+            const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
+            float lu = urand(), lv = urand();
+            if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
+            const float lw = 1.0f - lu - lv;
+            float ldx = lw * l0.x + (lu * l0.w + lv * l1.z) - p_x;
+            float ldy = lw * l0.y + (lu * l1.x + lv * l1.w) - p_y;
+            float ldz = lw * l0.z + (lu * l1.y + lv * l2.x) - p_z;
+            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
+            if (!(dist > 0.003f)) { out_M = 1; return; }
+            const float inv = 1.0f / dist;
+            ldx *= inv; ldy *= inv; ldz *= inv;
+            const float lcos = -(l2.y * ldx + (l2.z * ldy + l2.w * ldz));
+            const float scos = n_x * ldx + (n_y * ldy + n_z * ldz);
+            if (!(lcos > 1e-8f && scos > 1e-8f)) { out_M = 1; return; }
+            float lt_lum = lum3(l3.x, l3.y, l3.z);
+            if (lt_lum < 1e-6f) lt_lum = 1e-6f;
+            const float pdf = dist * dist * lt_lum / (lcos * total);
+            const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
+            const bool occ = occluded(so, {ldx, ldy, ldz}, dist - 0.003f);
+            const Bsdf e = eval_bsdf(ldx, ldy, ldz, wo_x, wo_y, wo_z, n_x, n_y, n_z, alpha2, metallic, spec_prob, ar,
+                                     ag, ab);
+            const float Fr = l3.x * e.r * scos * l3.w, Fg = l3.y * e.g * scos * l3.w, Fb = l3.z * e.b * scos * l3.w;
+            const float ph = lum3(Fr, Fg, Fb);
+            out_M = 1;
+            if (occ) {
+                out_sumW = (ph > 0.0f && pdf > 1e-12f) ? ph / pdf : 0.0f;
+                return;
+            }
+            if (!(pdf > 1e-12f)) return;
+            out_sumW = (ph > 0.0f) ? ph / pdf : 0.0f;
+            float wnee = 1.0f;
+            if (!nomis && e.pdf > 1e-20f) wnee = pdf / (pdf + e.pdf);
+            const float k = scos * wnee * l3.w / pdf;
+            or_ = l3.x * e.r * k;
+            og = l3.y * e.g * k;
+            ob = l3.z * e.b * k;
+            return;
+        }
+        float sumW = 0.0f;
+        float sel_lx = 0.0f, sel_ly = 0.0f, sel_lz = 0.0f, sel_Fr = 0.0f, sel_Fg = 0.0f, sel_Fb = 0.0f;
+        float sel_phat = 0.0f, sel_epdf = 0.0f;
+        bool have = false;
+        for (int j = 0; j < Mc; ++j) {
+            const float pick = urand() * total;
+            const float4* L = sc.lights + size_t(pick_light(pick)) * kLightWords;
+            const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
+            float lu = urand(), lv = urand();
+            if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
+            const float lw = 1.0f - lu - lv;
+            const float qx = lw * l0.x + (lu * l0.w + lv * l1.z);
+            const float qy = lw * l0.y + (lu * l1.x + lv * l1.w);
+            const float qz = lw * l0.z + (lu * l1.y + lv * l2.x);
+            const float ldx = qx - p_x, ldy = qy - p_y, ldz = qz - p_z;
+            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
+            float w = 0.0f, ph = 0.0f, epdf = 0.0f, Fr = 0.0f, Fg = 0.0f, Fb = 0.0f;
+            if (dist > 0.003f) {
+                const float inv = 1.0f / dist;
+                const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
+                const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
+                const float scos = n_x * dxn + (n_y * dyn + n_z * dzn);
+                if (lcos > 1e-8f && scos > 1e-8f) {
+                    float lt_lum = lum3(l3.x, l3.y, l3.z);
+                    if (lt_lum < 1e-6f) lt_lum = 1e-6f;
+                    const float q = dist * dist * lt_lum / (lcos * total);
+                    if (q > 1e-12f) {
+                        const Bsdf e = eval_bsdf(dxn, dyn, dzn, wo_x, wo_y, wo_z, n_x, n_y, n_z, alpha2, metallic,
+                                                 spec_prob, ar, ag, ab);
+                        epdf = e.pdf;
+                        Fr = l3.x * e.r * scos * l3.w;
+                        Fg = l3.y * e.g * scos * l3.w;
+                        Fb = l3.z * e.b * scos * l3.w;
+                        ph = lum3(Fr, Fg, Fb);
+                        if (ph > 0.0f) w = ph / q;
+                    }
+                }
+            }
+            sumW += w;
+            const float r = urand();
+            if (w > 0.0f && r * sumW < w) {
+                have = true;
+                sel_lx = qx; sel_ly = qy; sel_lz = qz;
+                sel_Fr = Fr; sel_Fg = Fg; sel_Fb = Fb;
+                sel_phat = ph; sel_epdf = epdf;
+            }
+        }
+        out_sumW = sumW;
+        out_M = uint32_t(Mc);
+        if (!have || !(sumW > 0.0f) || !(sel_phat > 0.0f)) return;
+        const float pdf_ris = float(Mc) * sel_phat / sumW;
+        float wnee = 1.0f;
+        if (!nomis && sel_epdf > 1e-20f) wnee = pdf_ris / (pdf_ris + sel_epdf);
+        float vdx = sel_lx - p_x, vdy = sel_ly - p_y, vdz = sel_lz - p_z;
+        const float vdist = sqrtf(vdx * vdx + vdy * vdy + vdz * vdz);
+        if (!(vdist > 0.003f)) return;
+        const float vinv = 1.0f / vdist;
+        vdx *= vinv; vdy *= vinv; vdz *= vinv;
+        const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
+        if (occluded(so, {vdx, vdy, vdz}, vdist - 0.003f)) return;
+        const float W = sumW / (float(Mc) * sel_phat);
+        or_ = sel_Fr * W * wnee;
+        og = sel_Fg * W * wnee;
+        ob = sel_Fb * W * wnee;
+    }
+
+    static __device__ float fresnel(float ci, float eta_in, float eta_out) {  // pathtrace.metal:80-91
+        if (ci < 0.0f) ci = 0.0f;
+        if (ci > 1.0f) ci = 1.0f;
+        const float eta = eta_in / eta_out;
+        const float s2 = eta * eta * (1.0f - ci * ci);
+        if (s2 >= 1.0f) return 1.0f;
+        const float ct = sqrtf(1.0f - s2);
+        const float rp = (eta_out * ci - eta_in * ct) / (eta_out * ci + eta_in * ct);
+        const float rs = (eta_in * ci - eta_out * ct) / (eta_in * ci + eta_out * ct);
+        return 0.5f * (rp * rp + rs * rs);
+    }
+
+    // sample_dielectric (pathtrace.metal:470-534)
+    __device__ DielectricSample sample_dielectric(float wo_x, float wo_y, float wo_z, float n_x, float n_y, float n_z,
+                                                  float cdx, float cdy, float cdz, float ior, bool front, float rough,
+                                                  float alpha2) {
+        DielectricSample out{true, 0.0f, 0.0f, 0.0f, 1.0f};
+        float eta_i = 1.0f, eta_t = ior;
+        if (!front) { eta_i = ior; eta_t = 1.0f; }
+        float hx, hy, hz;
+        if (rough <= 0.0201f) {
+            hx = n_x; hy = n_y; hz = n_z;
+        } else {
+            sample_ggx_half(alpha2, n_x, n_y, n_z, hx, hy, hz);
+            const float md = wo_x * hx + (wo_y * hy + wo_z * hz);
+            if (!(md > 1e-8f)) out.valid = false;
+        }
+        float cos_m = wo_x * hx + (wo_y * hy + wo_z * hz);
+        if (cos_m < 0.0f) { hx = -hx; hy = -hy; hz = -hz; cos_m = -cos_m; }
+        if (cos_m > 1.0f) cos_m = 1.0f;
+        const float fres = fresnel(cos_m, eta_i, eta_t);
+        const float eta = eta_i / eta_t;
+        const float s2 = eta * eta * (1.0f - cos_m * cos_m);
+        if (s2 >= 1.0f || urand() < fres) {
+            out.n_x = 2.0f * cos_m * hx - wo_x;
+            out.n_y = 2.0f * cos_m * hy - wo_y;
+            out.n_z = 2.0f * cos_m * hz - wo_z;
+            out.eta_scale = 1.0f;
+        } else {
+            const float cos_t = sqrtf(1.0f - s2);
+            out.n_x = eta * cdx + hx * (eta * cos_m - cos_t);
+            out.n_y = eta * cdy + hy * (eta * cos_m - cos_t);
+            out.n_z = eta * cdz + hz * (eta * cos_m - cos_t);
+            out.eta_scale = eta * eta;
+        }
+        {
+            const float d = 1.0f / sqrtf(out.n_x * out.n_x + out.n_y * out.n_y + out.n_z * out.n_z);
+            out.n_x *= d; out.n_y *= d; out.n_z *= d;
+        }
+        if (rough > 0.0201f) {
+            const float cos_i = fabsf(out.n_x * n_x + (out.n_y * n_y + out.n_z * n_z));
+            const float cos_o = fabsf(wo_x * n_x + (wo_y * n_y + wo_z * n_z));
+            const float cos_h = fabsf(hx * n_x + (hy * n_y + hz * n_z));
+            const float vo_h = fabsf(wo_x * hx + (wo_y * hy + wo_z * hz));
+            if (cos_i > 1e-8f && cos_o > 1e-8f && cos_h > 1e-8f && vo_h > 1e-8f) {
+                const float g1 = 2.0f * cos_o / (cos_o + sqrtf(cos_o * cos_o + alpha2 * (1.0f - cos_o * cos_o)));
+                const float g2 = 2.0f * cos_i / (cos_i + sqrtf(cos_i * cos_i + alpha2 * (1.0f - cos_i * cos_i)));
+                float w = g1 * g2 * vo_h / (cos_o * cos_h);
+                if (w > 10.0f) w = 10.0f;
+                out.eta_scale *= w;
+            } else {
+                out.valid = false;
+            }
+        }
+        return out;
+    }
+
+    // MIS weight of emission reached by a BSDF-sampled ray, with the
+    // ReSTIR-aware light pdf (pathtrace.metal:595-636 and 1191-1209).
+    __device__ float emission_mis(const Surface& hit, float ox, float oy, float oz, float last_pdf, float prev_sumW,
+                                  uint32_t prev_M, float pb_r, float pb_g, float pb_b, float pb_cos) {
+        float lx = hit.p.x - ox, ly = hit.p.y - oy, lz = hit.p.z - oz;
+        const float dist2 = lx * lx + ly * ly + lz * lz;
+        const float dist = sqrtf(dist2);
+        float pl = 1e30f;
+        const float4 nn = sc.nn[hit.tri];
+        if (dist > 1e-20f && last_pdf > 1e-20f && prev_M > 0 && prev_sumW > 1e-20f) {
+            const float ph = lum3(hit.er * pb_r * pb_cos * hit.alpha, hit.eg * pb_g * pb_cos * hit.alpha,
+                                  hit.eb * pb_b * pb_cos * hit.alpha);
+            if (ph > 1e-20f) {
+                pl = float(prev_M) * ph / prev_sumW;
+            } else {
+                const float inv = 1.0f / dist;
+                const float lxn = lx * inv, lyn = ly * inv, lzn = lz * inv;
+                const float cos_l = -(nn.x * lxn + (nn.y * lyn + nn.z * lzn));
+                if (cos_l > 1e-8f) {
+                    float lum = lum3(hit.er, hit.eg, hit.eb);
+                    if (lum < 1e-6f) lum = 1e-6f;
+                    pl = dist2 * lum / (cos_l * sc.light_total);
+                }
+            }
+        } else if (dist > 1e-20f && last_pdf > 1e-20f) {
+            const float inv = 1.0f / dist;
+            lx *= inv; ly *= inv; lz *= inv;
+            const float cos_l = -(nn.x * lx + (nn.y * ly + nn.z * lz));
+            if (cos_l > 1e-8f) {
+                float lum = lum3(hit.er, hit.eg, hit.eb);
+                if (lum < 1e-6f) lum = 1e-6f;
+                pl = dist2 * lum / (cos_l * sc.light_total);
+            }
+        }
+        return P.nomis ? 0.0f : ((last_pdf + pl > 1e-20f) ? last_pdf / (last_pdf + pl) : 0.0f);
+    }
+
+    // pathtrace (pathtrace.metal:552-728)
+    __device__ V3 pathtrace(V3 ro, V3 rd, int max_bounces) {
+        float ox = ro.x, oy = ro.y, oz = ro.z, dx = rd.x, dy = rd.y, dz = rd.z;
+        float accr = 0.0f, accg = 0.0f, accb = 0.0f, thr_r = 1.0f, thr_g = 1.0f, thr_b = 1.0f;
+        int bounce = 0, prev_delta = 1;
+        float last_pdf = 0.0f, prev_sumW = 0.0f;
+        uint32_t prev_M = 0;
+        float prev_br = 0.0f, prev_bg = 0.0f, prev_bb = 0.0f, prev_cos = 0.0f;
+        for (int b = 0; b < max_bounces; ++b) {
+            const Surface hit = cast_alpha_ray({ox, oy, oz}, {dx, dy, dz}, 1e30f);
+            if (hit.tri < 0) break;
+            float pnx = hit.n_interp.x, pny = hit.n_interp.y, pnz = hit.n_interp.z;
+            const float geom = hit.n_geom.x * pnx + (hit.n_geom.y * pny + hit.n_geom.z * pnz);
+            if (geom < 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
+            const float ndot = dx * pnx + (dy * pny + dz * pnz);
+            bool front = true;
+            if (ndot > 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; front = false; }
+            const float wox = -dx, woy = -dy, woz = -dz;
+            float rough = hit.rough;
+            if (rough < 0.0f) rough = 0.0f;
+            if (rough > 1.0f) rough = 1.0f;
+            if (rough < 0.02f) rough = 0.02f;
+            const float alpha = rough * rough;
+            const float alpha2 = alpha * alpha;
+            float metallic = hit.metallic;
+            if (metallic < 0.0f) metallic = 0.0f;
+            if (metallic > 1.0f) metallic = 1.0f;
+            if (bounce == 0 || prev_delta == 1) {
+                accr += hit.er * thr_r; accg += hit.eg * thr_g; accb += hit.eb * thr_b;
+            } else {
+                const float w = emission_mis(hit, ox, oy, oz, last_pdf, prev_sumW, prev_M, prev_br, prev_bg, prev_bb,
+                                             prev_cos);
+                accr += hit.er * thr_r * w; accg += hit.eg * thr_g * w; accb += hit.eb * thr_b * w;
+            }
+            if (hit.ior > 1.0001f) {
+                const DielectricSample ds =
+                    sample_dielectric(wox, woy, woz, pnx, pny, pnz, dx, dy, dz, hit.ior, front, rough, alpha2);
+                if (!ds.valid) break;
+                thr_r *= ds.eta_scale; thr_g *= ds.eta_scale; thr_b *= ds.eta_scale;
+                float s = 1.0f;
+                if (ds.n_x * pnx + (ds.n_y * pny + ds.n_z * pnz) < 0.0f) s = -1.0f;
+                ox = hit.p.x + pnx * 0.001f * s;
+                oy = hit.p.y + pny * 0.001f * s;
+                oz = hit.p.z + pnz * 0.001f * s;
+                dx = ds.n_x; dy = ds.n_y; dz = ds.n_z;
+                prev_delta = 1;
+                prev_sumW = 0.0f; prev_M = 0;
+            } else {
+                float spec_prob = 0.05f + 0.95f * metallic;
+                if (spec_prob > 0.98f) spec_prob = 0.98f;
+                if (metallic >= 0.999f) spec_prob = 1.0f;
+                float cur_sumW = 0.0f;
+                uint32_t cur_M = 0;
+                {
+                    float dlr, dlg, dlb;
+                    direct_lighting(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rough, metallic,
+                                    spec_prob, hit.ar, hit.ag, hit.ab, dlr, dlg, dlb, cur_sumW, cur_M);
+                    accr += dlr * thr_r; accg += dlg * thr_g; accb += dlb * thr_b;
+                }
+                float nx2, ny2, nz2;
+                if (urand() < spec_prob) {
+                    float hx, hy, hz;
+                    sample_ggx_half(alpha2, pnx, pny, pnz, hx, hy, hz);
+                    const float md = wox * hx + (woy * hy + woz * hz);
+                    if (!(md > 1e-8f)) break;
+                    nx2 = 2.0f * md * hx - wox; ny2 = 2.0f * md * hy - woy; nz2 = 2.0f * md * hz - woz;
+                    prev_delta = (rough <= 0.0201f) ? 1 : 0;
+                } else {
+                    sample_cosine(pnx, pny, pnz, nx2, ny2, nz2);
+                    prev_delta = 0;
+                }
+                const float ncos = nx2 * pnx + (ny2 * pny + nz2 * pnz);
+                if (!(ncos > 1e-8f)) break;
+                {
+                    const float d = 1.0f / sqrtf(nx2 * nx2 + ny2 * ny2 + nz2 * nz2);
+                    nx2 *= d; ny2 *= d; nz2 *= d;
+                }
+                const Bsdf e = eval_bsdf(nx2, ny2, nz2, wox, woy, woz, pnx, pny, pnz, alpha2, metallic, spec_prob,
+                                         hit.ar, hit.ag, hit.ab);
+                if (!(e.pdf > 1e-12f)) break;
+                last_pdf = e.pdf;
+                const float cos_i = nx2 * pnx + (ny2 * pny + nz2 * pnz);
+                thr_r *= e.r * cos_i / e.pdf;
+                thr_g *= e.g * cos_i / e.pdf;
+                thr_b *= e.b * cos_i / e.pdf;
+                prev_sumW = cur_sumW; prev_M = cur_M;
+                prev_br = e.r; prev_bg = e.g; prev_bb = e.b; prev_cos = cos_i;
+                ox = hit.p.x + pnx * 0.001f;
+                oy = hit.p.y + pny * 0.001f;
+                oz = hit.p.z + pnz * 0.001f;
+                dx = nx2; dy = ny2; dz = nz2;
+            }
+            if (thr_r < 0.0f) thr_r = 0.0f;
+            if (thr_g < 0.0f) thr_g = 0.0f;
+            if (thr_b < 0.0f) thr_b = 0.0f;
+            float mx = fmaxf(thr_r, thr_g);
+            mx = fmaxf(mx, thr_b);
+            if (!(mx >= 1e-12f)) break;
+            if (bounce >= 1) {
+                float rr = mx;
+                if (rr < 0.05f) rr = 0.05f;
+                if (rr > 0.9f) rr = 0.9f;
+                if (!(urand() <= rr)) break;
+                const float ir = 1.0f / rr;
+                thr_r *= ir; thr_g *= ir; thr_b *= ir;
+            }
+            bounce++;
+        }
+        return {accr, accg, accb};
+    }
+
+    // Camera ray (sample_sensor, pathtrace.metal:729-736). generate selects the
+    // (sx^2 + sy^2) + f^2 grouping of restir_generate (pathtrace.metal:834).
+    __device__ V3 primary_dir(float sx, float sy, bool generate) const {
+        const rm_camera& c = P.cam;
+        const float f2 = c.focal * c.focal;
+        const float d = 1.0f / sqrtf(generate ? (sx * sx + sy * sy) + f2 : sx * sx + (sy * sy + f2));
+        const float tx = sx * d, ty = sy * d, tz = c.focal * d;
+        return {tx * c.m0 + (ty * c.m1 + tz * c.m2), tx * c.m3 + (ty * c.m4 + tz * c.m5),
+                tx * c.m6 + (ty * c.m7 + tz * c.m8)};
+    }
+    __device__ void jitter(uint32_t x, uint32_t y, float& sx, float& sy) {  // pathtrace.metal:754-755
+        sx = float(x) + 1e-12f + (P.resolution - 2e-12f) * urand() - float(P.width) * 0.5f;
+        sy = float(P.height) * 0.5f - (float(y) + 1e-12f + (P.resolution - 2e-12f) * urand());
+    }
+    __device__ V3 sample_sensor(float sx, float sy) {
+        return pathtrace({P.cam.x, P.cam.y, P.cam.z}, primary_dir(sx, sy, false), P.bounces);
+    }
+
+    // Adds this lane's counts to the frame totals of its pass.
+    __device__ void flush(unsigned long long* counters, int pass) {
+        if (!counters) return;
+        atomicAdd(counters + 3 * pass, (unsigned long long)tl.rays);
+        atomicAdd(counters + 3 * pass + 1, (unsigned long long)tl.nodes);
+        atomicAdd(counters + 3 * pass + 2, (unsigned long long)tl.tris);
+        if (tl.overflow) atomicAdd(counters + kCntOverflow, 1ull);
+    }
+};
+
+__device__ __forceinline__ uint8_t tonemap8(float c) {
+    float v = floorf(c / (c + 1.0f) * 255.0f);
+    v = fminf(fmaxf(v, 0.0f), 255.0f);
+    return uint8_t(v);
+}
+
+}  // namespace rmh

@@ -1,0 +1,31 @@
+// Host-side launchers of the gfx950 kernels (rm_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rm_layout.h"
+
+namespace rmh {
+
+// `stack` is the traversal stack depth the resident tree needs; it selects the
+// kernel instance with a large enough LDS stack (32 or 64 entries per lane).
+constexpr int kMaxStack = 64;
+
+void launch_render_paths(int stack, int lanes, const DevScene& sc, const FrameParams& P, uint8_t* out, float4* hdr,
+                         unsigned long long* cnt, hipStream_t st);
+void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
+                     unsigned long long* cnt, hipStream_t st);
+void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
+                    const ReservoirSoA& ro, const GBufferSoA& hg, const ReservoirSoA& hr, unsigned long long* cnt,
+                    hipStream_t st);
+void launch_shade(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& rm,
+                  float4* hdr, unsigned long long* cnt, hipStream_t st);
+void launch_temporal(const FrameParams& P, const GBufferSoA& gb, const GBufferSoA& hg, const float4* in,
+                     const float4* hist, float4* out, hipStream_t st);
+void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in, float4* out, int step, hipStream_t st);
+void launch_tonemap(int npx, const float4* in, uint8_t* rgb, float* hdr3, hipStream_t st);
+void launch_trace(int stack, const DevScene& sc, const FrameParams& P, int n, const float* o, const float* d,
+                  const float* tmax, int any, int32_t* tri, float4* tuv, unsigned long long* cnt, hipStream_t st);
+// Bottom-up refit of the nodes listed in `order` (one tree level, deepest first).
+void launch_refit_level(float4* nodes, const float4* geo, const int32_t* level_nodes, int count, hipStream_t st);
+
+}  // namespace rmh

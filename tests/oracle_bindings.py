@@ -1,0 +1,231 @@
+"""ctypes bindings of the CPU oracle (oracle/liborc.so) and of the reference's
+own CPU engine built from its sources (oracle/_ref/libref.so). Test
+infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from raymotion_amd._capi import GBUFFER_DTYPE, RESERVOIR_DTYPE, Frame, SceneDesc, ptr
+
+ROOT = Path(__file__).resolve().parent.parent
+ORC_PATH = ROOT / "oracle" / "liborc.so"
+REF_PATH = ROOT / "oracle" / "_ref" / "libref.so"
+THREADS = min(16, os.cpu_count() or 1)
+
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+
+
+class OrcDebug(ctypes.Structure):
+    _fields_ = [("gbuf", _P), ("rinit", _P), ("rmerged", _P), ("shade", _P), ("path", _P), ("lanes", _P),
+                ("has_history", _P), ("stats", _P)]
+
+
+_orc = None
+_ref = None
+
+
+def orc() -> ctypes.CDLL:
+    global _orc
+    if _orc is None:
+        if not ORC_PATH.exists():
+            raise RuntimeError(f"{ORC_PATH} missing: run __graft_entry__.build()")
+        lib = ctypes.CDLL(str(ORC_PATH))
+        sigs = [
+            ("orc_history_create", _P, []),
+            ("orc_history_destroy", None, [_P]),
+            ("orc_render_frame", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame), ctypes.c_uint64, _P,
+                                                ctypes.c_int, _P, _P, ctypes.POINTER(OrcDebug)]),
+            ("orc_render_reference_f64", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame),
+                                                        ctypes.c_uint64, ctypes.c_int, _P]),
+            ("orc_trace", ctypes.c_int, [ctypes.POINTER(SceneDesc), _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
+                                         _P, _P, _P]),
+            ("orc_trace_brute", ctypes.c_int, [ctypes.POINTER(SceneDesc), _P, _P, _SZ, ctypes.c_int, _P, _P]),
+            ("orc_rng_draws", None, [ctypes.c_uint64, ctypes.c_int, _P, _P]),
+            ("orc_sincos", None, [_P, _SZ, _P, _P]),
+        ]
+        for name, res, args in sigs:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _orc = lib
+    return _orc
+
+
+def ref() -> ctypes.CDLL | None:
+    """The reference engine library, or None where it was not built."""
+    global _ref
+    if _ref is None and REF_PATH.exists():
+        lib = ctypes.CDLL(str(REF_PATH))
+        sigs = [
+            ("ref_scene_create", _P, []), ("ref_scene_destroy", None, [_P]), ("ref_scene_clear", None, [_P]),
+            ("ref_add_triangles", None, [_P, _SZ, _P, _P, _P, _P, _P]),
+            ("ref_build_bvh", None, [_P, ctypes.c_int]), ("ref_refit_bvh", None, [_P]),
+            ("ref_counts", None, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+            ("ref_nodes", None, [_P, _P, _P, _P]),
+            ("ref_precompute", None, [_P, _P, _P, _P, _P, _P]),
+            ("ref_camera", None, [_P, _P, ctypes.c_double, _P]),
+        ]
+        for name, res, args in sigs:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ref = lib
+    return _ref
+
+
+class History:
+    def __init__(self):
+        self.h = orc().orc_history_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            orc().orc_history_destroy(self.h)
+            self.h = None
+
+
+def render_frame(exported, frame: Frame, seed: int, history: History | None = None, debug: bool = False,
+                 threads: int = THREADS):
+    """One oracle frame: returns (rgb8, hdr, info) where info holds the
+    intermediates when debug is set."""
+    d = exported.desc()
+    px = frame.width * frame.height
+    rgb = np.zeros(px * 3, np.uint8)
+    hdr = np.zeros(px * 3, np.float32)
+    info = {}
+    dbg = OrcDebug()
+    path = ctypes.c_int32(-1)
+    lanes = ctypes.c_int32(0)
+    hh = ctypes.c_int32(0)
+    stats = np.zeros(3, np.uint64)
+    dbg.path = ctypes.addressof(path)
+    dbg.lanes = ctypes.addressof(lanes)
+    dbg.has_history = ctypes.addressof(hh)
+    dbg.stats = ptr(stats)
+    if debug:
+        info["gbuf"] = np.zeros(px, GBUFFER_DTYPE)
+        info["rinit"] = np.zeros(px, RESERVOIR_DTYPE)
+        info["rmerged"] = np.zeros(px, RESERVOIR_DTYPE)
+        info["shade"] = np.zeros(px * 3, np.float32)
+        dbg.gbuf = ptr(info["gbuf"])
+        dbg.rinit = ptr(info["rinit"])
+        dbg.rmerged = ptr(info["rmerged"])
+        dbg.shade = ptr(info["shade"])
+    rc = orc().orc_render_frame(ctypes.byref(d), ctypes.byref(frame), seed, history.h if history else None, threads,
+                                ptr(rgb), ptr(hdr), ctypes.byref(dbg))
+    if rc:
+        raise RuntimeError(f"orc_render_frame failed ({rc})")
+    info.update(path=path.value, lanes=lanes.value, has_history=hh.value, rays=int(stats[0]),
+                node_visits=int(stats[1]), tri_tests=int(stats[2]))
+    return rgb, hdr, info
+
+
+def render_reference_f64(exported, frame: Frame, seed: int, threads: int = THREADS) -> np.ndarray:
+    d = exported.desc()
+    out = np.zeros(frame.width * frame.height * 3, np.float64)
+    orc().orc_render_reference_f64(ctypes.byref(d), ctypes.byref(frame), seed, threads, ptr(out))
+    return out
+
+
+def trace(exported, origins, dirs, tmax=None, any_hit=False, threads: int = THREADS):
+    d = exported.desc()
+    o = np.ascontiguousarray(origins, np.float32)
+    dd = np.ascontiguousarray(dirs, np.float32)
+    n = len(o)
+    tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+    tri = np.empty(n, np.int32)
+    t = np.empty(n, np.float32); u = np.empty(n, np.float32); v = np.empty(n, np.float32)
+    stats = np.zeros(2, np.uint64)
+    orc().orc_trace(ctypes.byref(d), ptr(o), ptr(dd), ptr(tm), n, int(any_hit), threads, ptr(tri), ptr(t), ptr(u),
+                    ptr(v), ptr(stats))
+    return tri, t, u, v, stats
+
+
+def trace_brute(exported, origins, dirs, threads: int = THREADS):
+    d = exported.desc()
+    o = np.ascontiguousarray(origins, np.float32)
+    dd = np.ascontiguousarray(dirs, np.float32)
+    n = len(o)
+    tri = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    orc().orc_trace_brute(ctypes.byref(d), ptr(o), ptr(dd), n, threads, ptr(tri), ptr(t))
+    return tri, t
+
+
+def rng_draws(seed: int, n: int):
+    out = np.empty(n, np.float32)
+    states = np.empty(n + 1, np.uint32)
+    orc().orc_rng_draws(seed, n, ptr(out), ptr(states))
+    return out, states
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    orc().orc_sincos(ptr(x), len(x), ptr(s), ptr(c))
+    return s, c
+
+
+class RefScene:
+    """The reference engine's Scene (oracle/_ref/libref.so)."""
+
+    def __init__(self):
+        self.lib = ref()
+        self.h = self.lib.ref_scene_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ref_scene_destroy(self.h)
+            self.h = None
+
+    def add_triangles(self, pos, mat, normals=None, uv=None):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 9)
+        n = len(pos)
+        mat = np.ascontiguousarray(np.broadcast_to(mat, (n, 11)), np.float64)
+        nrm = None if normals is None else np.ascontiguousarray(normals, np.float64).reshape(n, 9)
+        uvs = None if uv is None else np.ascontiguousarray(uv, np.float64).reshape(n, 6)
+        idx = np.empty(n, np.int32)
+        self.lib.ref_add_triangles(self.h, n, ptr(pos), ptr(nrm), ptr(uvs), ptr(mat), ptr(idx))
+        return idx
+
+    def clear(self):
+        self.lib.ref_scene_clear(self.h)
+
+    def build_bvh(self, leaf=8):
+        self.lib.ref_build_bvh(self.h, leaf)
+
+    def refit_bvh(self):
+        self.lib.ref_refit_bvh(self.h)
+
+    def counts(self):
+        a, b, c = _SZ(), _SZ(), _SZ()
+        self.lib.ref_counts(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def nodes(self):
+        ntri, nnode, _ = self.counts()
+        boxes = np.empty((nnode, 6)); links = np.empty((nnode, 4), np.int32)
+        bvh_tri = np.empty(ntri if nnode else 0, np.int32)
+        if nnode:
+            self.lib.ref_nodes(self.h, ptr(boxes), ptr(links), ptr(bvh_tri))
+        return boxes, links, bvh_tri
+
+    def precompute(self):
+        ntri, _, nlight = self.counts()
+        n = np.empty((ntri, 3)); nn = np.empty((ntri, 3)); denom = np.empty(ntri); cdf = np.empty(nlight)
+        total = ctypes.c_double()
+        self.lib.ref_precompute(self.h, ptr(n), ptr(nn), ptr(denom), ptr(cdf), ctypes.byref(total))
+        return n, nn, denom, cdf, total.value
+
+
+def ref_camera(pos, rot, fov):
+    out = np.empty(13)
+    p = np.ascontiguousarray(pos, np.float64)
+    r = np.ascontiguousarray(rot, np.float64)
+    ref().ref_camera(ptr(p), ptr(r), float(fov), ptr(out))
+    return out
