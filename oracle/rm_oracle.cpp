@@ -246,7 +246,7 @@ template <class R> struct Tracer {
         hx *= d; hy *= d; hz *= d;
     }
 
-    Bsdf evaluate_opaque_bsdf(R wi_x, R wi_y, R wi_z, R wo_x, R wo_y, R wo_z, R n_x, R n_y, R n_z,
+    static Bsdf evaluate_opaque_bsdf(R wi_x, R wi_y, R wi_z, R wo_x, R wo_y, R wo_z, R n_x, R n_y, R n_z,
                               R alpha2, R metallic, R spec_prob, R ar, R ag, R ab) {
         const R kpi = R(3.141592653589793);
         Bsdf out;
@@ -1604,6 +1604,35 @@ void orc_rng_draws(uint64_t s, int n, float* out, uint32_t* states) {
 
 void orc_sincos(const float* x, size_t n, float* s, float* c) {
     for (size_t i = 0; i < n; ++i) rm_sincosf(x[i], &s[i], &c[i]);
+}
+
+// Closed-form test hooks (FP32 oracle functions).
+void orc_eval_bsdf(size_t n, const float* wi, const float* wo, const float* nrm, float alpha2, float metallic,
+                   float spec_prob, const float* albedo, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const auto b = orc::Tracer<float>::evaluate_opaque_bsdf(
+            wi[3 * i], wi[3 * i + 1], wi[3 * i + 2], wo[3 * i], wo[3 * i + 1], wo[3 * i + 2], nrm[3 * i],
+            nrm[3 * i + 1], nrm[3 * i + 2], alpha2, metallic, spec_prob, albedo[0], albedo[1], albedo[2]);
+        out[4 * i] = b.r; out[4 * i + 1] = b.g; out[4 * i + 2] = b.b; out[4 * i + 3] = b.pdf;
+    }
+}
+
+float orc_fresnel(float ci, float eta_in, float eta_out) {
+    return orc::Tracer<float>::dielectric_fresnel(ci, eta_in, eta_out);
+}
+
+// n directions from one stream: kind 0 cosine hemisphere, 1 GGX half vector.
+void orc_sample(uint64_t seed, int kind, size_t n, float alpha2, const float* nrm, float* out) {
+    orc::Tracer<float> t;
+    t.sc = nullptr;
+    t.stats = nullptr;
+    t.rng_.seed(seed);
+    for (size_t i = 0; i < n; ++i) {
+        float x, y, z;
+        if (kind == 0) t.sample_cosine_hemisphere(nrm[0], nrm[1], nrm[2], x, y, z);
+        else t.sample_ggx_half(alpha2, nrm[0], nrm[1], nrm[2], x, y, z);
+        out[3 * i] = x; out[3 * i + 1] = y; out[3 * i + 2] = z;
+    }
 }
 
 }  // extern "C"
