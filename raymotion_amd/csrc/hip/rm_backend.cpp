@@ -102,6 +102,7 @@ struct rm_ctx {
     DevBuf shade, den_a, hist_color;  // float4 per pixel
     DevBuf rgb8, hdr3, mega_hdr;
     DevBuf counters;
+    DevBuf queue;  // work counter of the persistent path engine
     DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
 
     // Temporal history (metal.mm:36-40).
@@ -197,7 +198,7 @@ int rm_create(rm_ctx** out, int hip_device) {
         return RM_ERR_DEVICE;
     }
     for (auto& e : c->ev) hipEventCreate(&e);
-    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long))) {
+    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(sizeof(unsigned))) {
         hipStreamDestroy(c->stream);
         delete c;
         g_create_error = "HIP counter allocation failed";
@@ -214,7 +215,7 @@ void rm_destroy(rm_ctx* c) {
     hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->nodes, &c->geo, &c->nrm, &c->nn, &c->matid, &c->mats, &c->uv, &c->lights, &c->cdf,
                       &c->texels, &c->level_nodes, &c->shade, &c->den_a, &c->hist_color, &c->rgb8, &c->hdr3,
-                      &c->mega_hdr, &c->counters, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
+                      &c->mega_hdr, &c->counters, &c->queue, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
                       &c->probe_tuv};
     for (DevBuf* b : bufs) b->release();
     c->gbuf.release();
@@ -356,6 +357,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     unsigned long long* cnt = c->counters.as<unsigned long long>();
     if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
         return rc;
+    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, sizeof(unsigned), c->stream), "work queue")) return rc;
     rm_stats st{};
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
@@ -381,7 +383,8 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         mark(1);
         launch_spatial(c->stack_need, sc, P, gb, ri, rmg, hg, hr, cnt, c->stream);
         mark(2);
-        launch_shade(c->stack_need, sc, P, gb, rmg, c->shade.as<float4>(), cnt, c->stream);
+        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr, c->queue.as<unsigned>(), cnt,
+                    c->stream);
         mark(3);
         const float4* final_img = c->shade.as<float4>();
         bool final_in_hist = false;
@@ -445,8 +448,12 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
             c->drop_history();
             return c->fail(RM_ERR_ALLOC, "HIP image buffer allocation failed");
         }
-        launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(), hdr ? c->mega_hdr.as<float4>() : nullptr,
-                            cnt, c->stream);
+        if (lanes == 1)
+            launch_wave(c->stack_need, false, sc, P, GBufferSoA{}, ReservoirSoA{}, hdr ? c->mega_hdr.as<float4>() : nullptr,
+                        c->rgb8.as<uint8_t>(), c->queue.as<unsigned>(), cnt, c->stream);
+        else
+            launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(),
+                                hdr ? c->mega_hdr.as<float4>() : nullptr, cnt, c->stream);
         mark(1);
         if (hdr) launch_tonemap(npx, c->mega_hdr.as<float4>(), c->rgb8.as<uint8_t>(), c->hdr3.as<float>(), c->stream);
         if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;

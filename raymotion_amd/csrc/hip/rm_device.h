@@ -103,7 +103,7 @@ struct Tracer {
     // when its entry lies beyond the current hit by more than 2^-12 (relative),
     // and equal distances resolve to the smaller triangle index, so the result
     // is the closest hit regardless of visit order (same rule as the oracle).
-    __device__ Hit intersect(V3 ro, V3 rd, float tmax, bool any) {
+    __device__ __forceinline__ Hit intersect(V3 ro, V3 rd, float tmax, bool any) {
         Hit h;
         h.t = tmax;
         h.tri = -1;
@@ -212,8 +212,15 @@ struct Tracer {
     }
 
     // cast_ray (pathtrace.metal:165-204): hit + interpolated normal + material.
-    __device__ Surface cast_ray(V3 ro, V3 rd, float dist) {
-        const Hit h = intersect(ro, rd, dist, false);
+    __device__ Surface cast_ray(V3 ro, V3 rd, float dist) { return surface_at(intersect(ro, rd, dist, false), ro, rd); }
+
+    // Material alpha of a triangle (coverage test of cast_alpha_ray).
+    __device__ __forceinline__ float tri_alpha(int tri) const {
+        return sc.mats[size_t(sc.matid[tri]) * kMatWords].w;
+    }
+
+    // The surface data cast_ray derives from a hit of the ray ro + t rd.
+    __device__ Surface surface_at(const Hit& h, V3 ro, V3 rd) const {
         Surface s;
         s.t = h.t; s.u = h.u; s.v = h.v; s.tri = h.tri;
         s.p = ro + rd * h.t;
@@ -347,7 +354,7 @@ struct Tracer {
     }
 
     // evaluate_opaque_bsdf (pathtrace.metal:243-295): Lambert + GGX/Smith/Schlick.
-    static __device__ Bsdf eval_bsdf(float wi_x, float wi_y, float wi_z, float wo_x, float wo_y, float wo_z, float n_x,
+    static __device__ __forceinline__ Bsdf eval_bsdf(float wi_x, float wi_y, float wi_z, float wo_x, float wo_y, float wo_z, float n_x,
                                      float n_y, float n_z, float alpha2, float metallic, float spec_prob, float ar,
                                      float ag, float ab) {
         const float kpi = 3.14159265358979323846f;
@@ -397,16 +404,27 @@ struct Tracer {
         return lo;
     }
 
-    // direct_lighting (pathtrace.metal:299-469): legacy NEE for one candidate,
-    // otherwise RIS over Mc candidates with one shadow ray.
-    __device__ void direct_lighting(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
-                                    float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
-                                    float ag, float ab, float& or_, float& og, float& ob, float& out_sumW,
-                                    uint32_t& out_M) {
-        or_ = og = ob = 0.0f;
-        out_sumW = 0.0f;
-        out_M = 0;
-        if (sc.nlights == 0 || !(sc.light_total > 1e-12f)) return;
+    // direct_lighting (pathtrace.metal:299-469) up to its shadow ray: legacy
+    // NEE for one candidate, otherwise RIS over Mc candidates. Consumes every
+    // RNG draw the reference takes before the shadow ray; the caller traces
+    // the ray (if any) and keeps (r, g, b) only when it is unoccluded.
+    struct NeeSample {
+        bool shadow;
+        V3 o, d;
+        float dist;
+        float r, g, b, sumW;
+        uint32_t M;
+    };
+    __device__ __forceinline__ NeeSample nee_prepare(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
+                                     float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
+                                     float ag, float ab) {
+        NeeSample ns;
+        ns.shadow = false;
+        ns.o = ns.d = V3{0.0f, 0.0f, 0.0f};
+        ns.dist = 0.0f;
+        ns.r = ns.g = ns.b = ns.sumW = 0.0f;
+        ns.M = 0;
+        if (sc.nlights == 0 || !(sc.light_total > 1e-12f)) return ns;
         int Mc = P.restir_candidates;
         if (Mc < 1) Mc = 1;
         if (Mc > 32) Mc = 32;
@@ -417,6 +435,7 @@ struct Tracer {
         const float alpha2 = (r2 * r2) * (r2 * r2);
         const bool nomis = P.nomis != 0;
         const float total = sc.light_total;
+        const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
         if (Mc <= 1) {
             const float pick = urand() * total;
             const float4* L = sc.lights + size_t(pick_light(pick)) * kLightWords;
@@ -428,35 +447,35 @@ struct Tracer {
             float ldy = lw * l0.y + (lu * l1.x + lv * l1.w) - p_y;
             float ldz = lw * l0.z + (lu * l1.y + lv * l2.x) - p_z;
             const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
-            if (!(dist > 0.003f)) { out_M = 1; return; }
+            ns.M = 1;
+            if (!(dist > 0.003f)) return ns;
             const float inv = 1.0f / dist;
             ldx *= inv; ldy *= inv; ldz *= inv;
             const float lcos = -(l2.y * ldx + (l2.z * ldy + l2.w * ldz));
             const float scos = n_x * ldx + (n_y * ldy + n_z * ldz);
-            if (!(lcos > 1e-8f && scos > 1e-8f)) { out_M = 1; return; }
+            if (!(lcos > 1e-8f && scos > 1e-8f)) return ns;
             float lt_lum = lum3(l3.x, l3.y, l3.z);
             if (lt_lum < 1e-6f) lt_lum = 1e-6f;
             const float pdf = dist * dist * lt_lum / (lcos * total);
-            const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
-            const bool occ = occluded(so, {ldx, ldy, ldz}, dist - 0.003f);
             const Bsdf e = eval_bsdf(ldx, ldy, ldz, wo_x, wo_y, wo_z, n_x, n_y, n_z, alpha2, metallic, spec_prob, ar,
                                      ag, ab);
             const float Fr = l3.x * e.r * scos * l3.w, Fg = l3.y * e.g * scos * l3.w, Fb = l3.z * e.b * scos * l3.w;
             const float ph = lum3(Fr, Fg, Fb);
-            out_M = 1;
-            if (occ) {
-                out_sumW = (ph > 0.0f && pdf > 1e-12f) ? ph / pdf : 0.0f;
-                return;
+            // The same sum whether or not the shadow ray is blocked (metal:364,370,375).
+            ns.sumW = (ph > 0.0f && pdf > 1e-12f) ? ph / pdf : 0.0f;
+            ns.shadow = true;
+            ns.o = so;
+            ns.d = V3{ldx, ldy, ldz};
+            ns.dist = dist - 0.003f;
+            if (pdf > 1e-12f) {
+                float wnee = 1.0f;
+                if (!nomis && e.pdf > 1e-20f) wnee = pdf / (pdf + e.pdf);
+                const float k = scos * wnee * l3.w / pdf;
+                ns.r = l3.x * e.r * k;
+                ns.g = l3.y * e.g * k;
+                ns.b = l3.z * e.b * k;
             }
-            if (!(pdf > 1e-12f)) return;
-            out_sumW = (ph > 0.0f) ? ph / pdf : 0.0f;
-            float wnee = 1.0f;
-            if (!nomis && e.pdf > 1e-20f) wnee = pdf / (pdf + e.pdf);
-            const float k = scos * wnee * l3.w / pdf;
-            or_ = l3.x * e.r * k;
-            og = l3.y * e.g * k;
-            ob = l3.z * e.b * k;
-            return;
+            return ns;
         }
         float sumW = 0.0f;
         float sel_lx = 0.0f, sel_ly = 0.0f, sel_lz = 0.0f, sel_Fr = 0.0f, sel_Fg = 0.0f, sel_Fb = 0.0f;
@@ -505,23 +524,41 @@ struct Tracer {
                 sel_phat = ph; sel_epdf = epdf;
             }
         }
-        out_sumW = sumW;
-        out_M = uint32_t(Mc);
-        if (!have || !(sumW > 0.0f) || !(sel_phat > 0.0f)) return;
+        ns.sumW = sumW;
+        ns.M = uint32_t(Mc);
+        if (!have || !(sumW > 0.0f) || !(sel_phat > 0.0f)) return ns;
         const float pdf_ris = float(Mc) * sel_phat / sumW;
         float wnee = 1.0f;
         if (!nomis && sel_epdf > 1e-20f) wnee = pdf_ris / (pdf_ris + sel_epdf);
         float vdx = sel_lx - p_x, vdy = sel_ly - p_y, vdz = sel_lz - p_z;
         const float vdist = sqrtf(vdx * vdx + vdy * vdy + vdz * vdz);
-        if (!(vdist > 0.003f)) return;
+        if (!(vdist > 0.003f)) return ns;
         const float vinv = 1.0f / vdist;
         vdx *= vinv; vdy *= vinv; vdz *= vinv;
-        const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
-        if (occluded(so, {vdx, vdy, vdz}, vdist - 0.003f)) return;
         const float W = sumW / (float(Mc) * sel_phat);
-        or_ = sel_Fr * W * wnee;
-        og = sel_Fg * W * wnee;
-        ob = sel_Fb * W * wnee;
+        ns.shadow = true;
+        ns.o = so;
+        ns.d = V3{vdx, vdy, vdz};
+        ns.dist = vdist - 0.003f;
+        ns.r = sel_Fr * W * wnee;
+        ns.g = sel_Fg * W * wnee;
+        ns.b = sel_Fb * W * wnee;
+        return ns;
+    }
+
+    // direct_lighting with its shadow ray traced in place.
+    __device__ void direct_lighting(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
+                                    float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
+                                    float ag, float ab, float& or_, float& og, float& ob, float& out_sumW,
+                                    uint32_t& out_M) {
+        const NeeSample ns = nee_prepare(p_x, p_y, p_z, n_x, n_y, n_z, wo_x, wo_y, wo_z, rough, metallic, spec_prob,
+                                         ar, ag, ab);
+        const bool lit = ns.shadow && !occluded(ns.o, ns.d, ns.dist);
+        or_ = lit ? ns.r : 0.0f;
+        og = lit ? ns.g : 0.0f;
+        ob = lit ? ns.b : 0.0f;
+        out_sumW = ns.sumW;
+        out_M = ns.M;
     }
 
     static __device__ float fresnel(float ci, float eta_in, float eta_out) {  // pathtrace.metal:80-91

@@ -13,8 +13,9 @@ __device__ __forceinline__ V3 clamp3(V3 c, float cl) {
     return cl > 0.0f ? vmin(c, V3{cl, cl, cl}) : c;
 }
 
-// render_paths (pathtrace.metal:739-769) for LANES == 1 and
-// render_paths_simd<LANES> (pathtrace.metal:773-815) otherwise.
+// render_paths_simd<LANES> (pathtrace.metal:773-815): LANES lanes per pixel,
+// each with its own stream, combined by an xor butterfly per block of samples.
+// (One lane per pixel runs in the persistent engine, rm_wave.hip.)
 template <int S, int LANES>
 __global__ __launch_bounds__(kBlock) void k_render_paths(DevScene sc, FrameParams P, uint8_t* __restrict__ out,
                                                           float4* __restrict__ hdr, unsigned long long* cnt) {
@@ -354,240 +355,6 @@ __global__ __launch_bounds__(kBlock) void k_spatial(DevScene sc, FrameParams P, 
     ro_.r3[id] = make_float4(cur_sumWu, __uint_as_float(cur_M), __uint_as_float(1u), 0.0f);
 }
 
-// restir_shade (pathtrace.metal:1056-1258)
-template <int S>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameParams P, GBufferSoA gb, ReservoirSoA rm,
-                                                   float4* __restrict__ hdr, unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
-    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
-    if (id >= uint32_t(P.width) * uint32_t(P.height)) return;
-    const uint32_t x = id % uint32_t(P.width), y = id / uint32_t(P.width);
-    const float4 g0 = gb.p0[id];
-    const int kind = ibits(g0.w);
-    if (kind == 0) {
-        hdr[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
-    Tracer<S> pt(sc, P, stack + threadIdx.x);
-    pt.rng.seed(P.seed ^ (uint64_t(id + 1) * 0xBF58476D1CE4E5B9ULL) ^ 0x94D049BB133111EBULL);
-    const float cl = P.clamp_value;
-    if (kind == 3) {
-        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-        int n = 0;
-        for (int s = 0; s < P.spp; ++s) {
-            float sx, sy;
-            pt.jitter(x, y, sx, sy);
-            const V3 c = clamp3(pt.sample_sensor(sx, sy), cl);
-            sr += c.x; sg += c.y; sb += c.z;
-            ++n;
-        }
-        const float fn = float(max(n, 1));
-        const V3 c = clamp3(V3{sr / fn, sg / fn, sb / fn}, cl);
-        hdr[id] = make_float4(c.x, c.y, c.z, 0.0f);
-        pt.flush(cnt, kCntShade);
-        return;
-    }
-    const float4 g1 = gb.p1[id], g2 = gb.p2[id], g3 = gb.p3[id], g4 = gb.p4[id];
-    const V3 gpos = xyz(g0), gnor = xyz(g1), galb = xyz(g2), gwo = xyz(g3), gemi = xyz(g4);
-    float rough = g1.w;
-    if (rough < 0.0f) rough = 0.0f;
-    if (rough > 1.0f) rough = 1.0f;
-    float metallic = g2.w;
-    if (metallic < 0.0f) metallic = 0.0f;
-    if (metallic > 1.0f) metallic = 1.0f;
-    float r2 = rough;
-    if (r2 < 0.02f) r2 = 0.02f;
-    const float alpha2 = (r2 * r2) * (r2 * r2);
-    float spec_prob = 0.05f + 0.95f * metallic;
-    if (spec_prob > 0.98f) spec_prob = 0.98f;
-    if (metallic >= 0.999f) spec_prob = 1.0f;
-    const bool nomis = P.nomis != 0;
-    V3 prim{0.0f, 0.0f, 0.0f};
-    float m_sumWu = 0.0f;
-    uint32_t m_M = 0;
-    {
-        const float4 m0 = rm.r0[id], m1 = rm.r1[id], m2 = rm.r2[id], m3 = rm.r3[id];
-        const uint32_t valid = __float_as_uint(m3.z), M = __float_as_uint(m3.y);
-        if (valid != 0u && M > 0u && m3.x > 0.0f && m2.w > 0.0f && m0.w > 0.0f) {
-            const V3 lp = xyz(m0), ln = xyz(m1), le = xyz(m2);
-            const float lalpha = m1.w;
-            m_sumWu = m3.x;
-            m_M = M;
-            const float m_sumWs = m2.w;
-            const float ldx = lp.x - gpos.x, ldy = lp.y - gpos.y, ldz = lp.z - gpos.z;
-            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
-            if (dist > 0.003f) {
-                const float inv = 1.0f / dist;
-                const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
-                const float lcos = -(ln.x * dxn + (ln.y * dyn + ln.z * dzn));
-                const float scos = gnor.x * dxn + (gnor.y * dyn + gnor.z * dzn);
-                if (lcos > 1e-8f && scos > 1e-8f) {
-                    const Bsdf e = Tracer<S>::eval_bsdf(dxn, dyn, dzn, gwo.x, gwo.y, gwo.z, gnor.x, gnor.y, gnor.z,
-                                                        alpha2, metallic, spec_prob, galb.x, galb.y, galb.z);
-                    const float Fr = le.x * e.r * scos * lalpha, Fg = le.y * e.g * scos * lalpha,
-                                Fb = le.z * e.b * scos * lalpha;
-                    const float ph = lum3(Fr, Fg, Fb);
-                    if (ph > 0.0f) {
-                        const float pdf_ris = float(m_M) * ph / m_sumWu;
-                        float wnee = 1.0f;
-                        if (!nomis && e.pdf > 1e-20f) wnee = pdf_ris / (pdf_ris + e.pdf);
-                        const float W = m_sumWs / (float(m_M) * ph);
-                        prim = {Fr * W * wnee, Fg * W * wnee, Fb * W * wnee};
-                    }
-                }
-            }
-        }
-    }
-    if (P.bounces <= 1 || P.spp <= 0) {
-        const V3 c = clamp3(gemi + prim, cl);
-        hdr[id] = make_float4(c.x, c.y, c.z, 0.0f);
-        pt.flush(cnt, kCntShade);
-        return;
-    }
-    const V3 direct = clamp3(gemi + prim, cl);
-    float sr = 0.0f, sg = 0.0f, sb = 0.0f;
-    int n = 0;
-    for (int s = 0; s < P.spp; ++s) {
-        float nx2, ny2, nz2;
-        int pdelta = 0;
-        bool ok = true;
-        if (pt.urand() < spec_prob) {
-            float hx, hy, hz;
-            pt.sample_ggx_half(alpha2, gnor.x, gnor.y, gnor.z, hx, hy, hz);
-            const float md = gwo.x * hx + (gwo.y * hy + gwo.z * hz);
-            if (!(md > 1e-8f)) {
-                ok = false;
-            } else {
-                nx2 = 2.0f * md * hx - gwo.x; ny2 = 2.0f * md * hy - gwo.y; nz2 = 2.0f * md * hz - gwo.z;
-                pdelta = (rough <= 0.0201f) ? 1 : 0;
-            }
-        } else {
-            pt.sample_cosine(gnor.x, gnor.y, gnor.z, nx2, ny2, nz2);
-            pdelta = 0;
-        }
-        Bsdf e0{0.0f, 0.0f, 0.0f, 0.0f};
-        if (ok) {
-            const float ncos = nx2 * gnor.x + (ny2 * gnor.y + nz2 * gnor.z);
-            if (!(ncos > 1e-8f)) {
-                ok = false;
-            } else {
-                const float d = 1.0f / sqrtf(nx2 * nx2 + ny2 * ny2 + nz2 * nz2);
-                nx2 *= d; ny2 *= d; nz2 *= d;
-                e0 = Tracer<S>::eval_bsdf(nx2, ny2, nz2, gwo.x, gwo.y, gwo.z, gnor.x, gnor.y, gnor.z, alpha2, metallic,
-                                          spec_prob, galb.x, galb.y, galb.z);
-                if (!(e0.pdf > 1e-12f)) ok = false;
-            }
-        }
-        if (!ok) {
-            sr += direct.x; sg += direct.y; sb += direct.z;
-            ++n;
-            continue;
-        }
-        const float cos0 = nx2 * gnor.x + (ny2 * gnor.y + nz2 * gnor.z);
-        float tr = e0.r * cos0 / e0.pdf, tg = e0.g * cos0 / e0.pdf, tb = e0.b * cos0 / e0.pdf;
-        float ox = gpos.x + gnor.x * 0.001f, oy = gpos.y + gnor.y * 0.001f, oz = gpos.z + gnor.z * 0.001f;
-        float dx = nx2, dy = ny2, dz = nz2;
-        float last_pdf = e0.pdf;
-        float pb_r = e0.r, pb_g = e0.g, pb_b = e0.b, pb_cos = cos0;
-        float psumW = m_sumWu;
-        uint32_t pM = m_M;
-        int pdel = pdelta;
-        float ar_ = 0.0f, ag_ = 0.0f, ab_ = 0.0f;
-        for (int b = 1; b < P.bounces; ++b) {
-            const Surface hit = pt.cast_alpha_ray({ox, oy, oz}, {dx, dy, dz}, 1e30f);
-            if (hit.tri < 0) break;
-            float pnx = hit.n_interp.x, pny = hit.n_interp.y, pnz = hit.n_interp.z;
-            const float gg = hit.n_geom.x * pnx + (hit.n_geom.y * pny + hit.n_geom.z * pnz);
-            if (gg < 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
-            const float nd = dx * pnx + (dy * pny + dz * pnz);
-            bool front = true;
-            if (nd > 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; front = false; }
-            const float wox = -dx, woy = -dy, woz = -dz;
-            float rh = hit.rough;
-            if (rh < 0.0f) rh = 0.0f;
-            if (rh > 1.0f) rh = 1.0f;
-            if (rh < 0.02f) rh = 0.02f;
-            const float al = rh * rh;
-            const float a2 = al * al;
-            float met = hit.metallic;
-            if (met < 0.0f) met = 0.0f;
-            if (met > 1.0f) met = 1.0f;
-            if (pdel == 1) {
-                ar_ += hit.er * tr; ag_ += hit.eg * tg; ab_ += hit.eb * tb;
-            } else {
-                const float w = pt.emission_mis(hit, ox, oy, oz, last_pdf, psumW, pM, pb_r, pb_g, pb_b, pb_cos);
-                ar_ += hit.er * tr * w; ag_ += hit.eg * tg * w; ab_ += hit.eb * tb * w;
-            }
-            if (hit.ior > 1.0001f) {
-                const DielectricSample ds =
-                    pt.sample_dielectric(wox, woy, woz, pnx, pny, pnz, dx, dy, dz, hit.ior, front, rh, a2);
-                if (!ds.valid) break;
-                tr *= ds.eta_scale; tg *= ds.eta_scale; tb *= ds.eta_scale;
-                float sgn = 1.0f;
-                if (ds.n_x * pnx + (ds.n_y * pny + ds.n_z * pnz) < 0.0f) sgn = -1.0f;
-                ox = hit.p.x + pnx * 0.001f * sgn; oy = hit.p.y + pny * 0.001f * sgn; oz = hit.p.z + pnz * 0.001f * sgn;
-                dx = ds.n_x; dy = ds.n_y; dz = ds.n_z;
-                pdel = 1;
-                psumW = 0.0f; pM = 0;
-            } else {
-                float sp2 = 0.05f + 0.95f * met;
-                if (sp2 > 0.98f) sp2 = 0.98f;
-                if (met >= 0.999f) sp2 = 1.0f;
-                float dlr = 0.0f, dlg = 0.0f, dlb = 0.0f, csum = 0.0f;
-                uint32_t cM = 0;
-                pt.direct_lighting(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rh, met, sp2, hit.ar,
-                                   hit.ag, hit.ab, dlr, dlg, dlb, csum, cM);
-                ar_ += dlr * tr; ag_ += dlg * tg; ab_ += dlb * tb;
-                float qx2, qy2, qz2;
-                if (pt.urand() < sp2) {
-                    float hx, hy, hz;
-                    pt.sample_ggx_half(a2, pnx, pny, pnz, hx, hy, hz);
-                    const float md = wox * hx + (woy * hy + woz * hz);
-                    if (!(md > 1e-8f)) break;
-                    qx2 = 2.0f * md * hx - wox; qy2 = 2.0f * md * hy - woy; qz2 = 2.0f * md * hz - woz;
-                    pdel = (rh <= 0.0201f) ? 1 : 0;
-                } else {
-                    pt.sample_cosine(pnx, pny, pnz, qx2, qy2, qz2);
-                    pdel = 0;
-                }
-                const float nc = qx2 * pnx + (qy2 * pny + qz2 * pnz);
-                if (!(nc > 1e-8f)) break;
-                {
-                    const float d = 1.0f / sqrtf(qx2 * qx2 + qy2 * qy2 + qz2 * qz2);
-                    qx2 *= d; qy2 *= d; qz2 *= d;
-                }
-                const Bsdf ee = Tracer<S>::eval_bsdf(qx2, qy2, qz2, wox, woy, woz, pnx, pny, pnz, a2, met, sp2, hit.ar,
-                                                     hit.ag, hit.ab);
-                if (!(ee.pdf > 1e-12f)) break;
-                last_pdf = ee.pdf;
-                const float ci = qx2 * pnx + (qy2 * pny + qz2 * pnz);
-                tr *= ee.r * ci / ee.pdf; tg *= ee.g * ci / ee.pdf; tb *= ee.b * ci / ee.pdf;
-                psumW = csum; pM = cM;
-                pb_r = ee.r; pb_g = ee.g; pb_b = ee.b; pb_cos = ci;
-                ox = hit.p.x + pnx * 0.001f; oy = hit.p.y + pny * 0.001f; oz = hit.p.z + pnz * 0.001f;
-                dx = qx2; dy = qy2; dz = qz2;
-            }
-            float mx = fmaxf(tr, tg);
-            mx = fmaxf(mx, tb);
-            if (!(mx >= 1e-12f)) break;
-            if (b >= 2) {
-                float rr = mx;
-                if (rr < 0.05f) rr = 0.05f;
-                if (rr > 0.9f) rr = 0.9f;
-                if (!(pt.urand() <= rr)) break;
-                const float ir = 1.0f / rr;
-                tr *= ir; tg *= ir; tb *= ir;
-            }
-        }
-        const V3 full = clamp3(gemi + prim + V3{ar_, ag_, ab_}, cl);
-        sr += full.x; sg += full.y; sb += full.z;
-        ++n;
-    }
-    const float fn = float(max(n, 1));
-    hdr[id] = make_float4(sr / fn, sg / fn, sb / fn, 0.0f);
-    pt.flush(cnt, kCntShade);
-}
-
 __device__ __forceinline__ float luma(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
 // denoise_temporal (pathtrace.metal:1263-1314)
@@ -733,8 +500,7 @@ void launch_render_paths(int stack, int lanes, const DevScene& sc, const FramePa
     const dim3 grid(blocks_for(threads)), block(kBlock);
     RM_STACK_DISPATCH(stack, {
         if (lanes == 32) hipLaunchKernelGGL((k_render_paths<S, 32>), grid, block, 0, st, sc, P, out, hdr, cnt);
-        else if (lanes == 8) hipLaunchKernelGGL((k_render_paths<S, 8>), grid, block, 0, st, sc, P, out, hdr, cnt);
-        else hipLaunchKernelGGL((k_render_paths<S, 1>), grid, block, 0, st, sc, P, out, hdr, cnt);
+        else hipLaunchKernelGGL((k_render_paths<S, 8>), grid, block, 0, st, sc, P, out, hdr, cnt);
     });
 }
 
@@ -749,12 +515,6 @@ void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const G
                     hipStream_t st) {
     const dim3 grid(blocks_for(size_t(P.width) * size_t(P.height))), block(kBlock);
     RM_STACK_DISPATCH(stack, hipLaunchKernelGGL((k_spatial<S>), grid, block, 0, st, sc, P, gb, ri, ro, hg, hr, cnt));
-}
-
-void launch_shade(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& rm,
-                  float4* hdr, unsigned long long* cnt, hipStream_t st) {
-    const dim3 grid(blocks_for(size_t(P.width) * size_t(P.height))), block(kBlock);
-    RM_STACK_DISPATCH(stack, hipLaunchKernelGGL((k_shade<S>), grid, block, 0, st, sc, P, gb, rm, hdr, cnt));
 }
 
 void launch_temporal(const FrameParams& P, const GBufferSoA& gb, const GBufferSoA& hg, const float4* in,

@@ -1,0 +1,534 @@
+// Persistent path engine for the two long-running passes: render_paths with
+// one lane per pixel (pathtrace.metal:739-769) and restir_shade
+// (pathtrace.metal:1056-1258).
+//
+// The reference runs each pixel's whole sample loop inside one thread, so the
+// threads of a wavefront sit at different places of their paths and trace
+// their rays one at a time in divergent code. Here every lane keeps its
+// pixel's state in registers and the kernel is a loop whose body traces
+// exactly one ray per active lane (extension, shadow, or the next step of a
+// stochastic-coverage walk), then advances each lane's state machine to its
+// next ray. All active lanes of a wave therefore run the traversal together,
+// and a lane whose pixel is finished takes the next pixel from a global
+// counter (one atomic per wave), so waves stay full until the image is done.
+// The per-pixel arithmetic and random-number order are those of the reference
+// functions (shared with the other kernels through Tracer), which keeps every
+// pixel's result bit-identical to a straight evaluation.
+#include "rm_device.h"
+#include "rm_kernels.h"
+
+namespace rmh {
+namespace {
+
+enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2 };
+enum Step : int { kTrace = 0, kPathEnd = 1, kPixelDone = 2 };
+
+struct Lane {
+    int phase;
+    int pixel, kind;  // kind 1: ReSTIR indirect samples, 3: camera paths
+    int s, n;
+    float sr, sg, sb;     // sum of sample radiance
+    float sl, sl2;        // adaptive-sampling luminance sums (megakernel)
+    float dr, dg, db;     // emission + ReSTIR direct term of the primary (kind 1)
+    float m_sumWu;
+    uint32_t m_M;
+    // path
+    float tr, tg, tb, ar, ag, ab;
+    V3 po, d;             // path vertex origin and direction of the current segment
+    float last_pdf, prev_sumW;
+    uint32_t prev_M;
+    float pbr, pbg, pbb, pbcos;
+    int prev_delta, hit, max_hits;
+    bool clamp_neg;
+    // ray being traced (stochastic-coverage walks advance along it)
+    V3 ro, rd, base;
+    float tmax, traveled, seq_dist;
+    // vertex waiting for its shadow ray
+    V3 p, pn;
+    float h_ar, h_ag, h_ab, h_rough, h_alpha2, h_met, h_spec;
+    float nr, ng, nb, nsumW;
+    uint32_t nM;
+};
+
+template <int S, bool SHADE>
+struct Engine {
+    Tracer<S>& pt;
+    const FrameParams& P;
+    const GBufferSoA& gb;
+    const ReservoirSoA& rm;
+    float4* hdr;
+    uint8_t* rgb;
+
+    __device__ void set_ray(Lane& L, int phase, V3 o, V3 dir, float dist) {
+        L.phase = phase;
+        L.ro = o;
+        L.base = o;
+        L.rd = dir;
+        L.tmax = dist;
+        L.traveled = 0.0f;
+        L.seq_dist = dist;
+    }
+
+    // Camera path (pathtrace from sample_sensor): megakernel and kind 3.
+    __device__ int begin_camera_sample(Lane& L) {
+        const uint32_t x = uint32_t(L.pixel) % uint32_t(P.width), y = uint32_t(L.pixel) / uint32_t(P.width);
+        float sx, sy;
+        pt.jitter(x, y, sx, sy);
+        L.d = pt.primary_dir(sx, sy, false);
+        L.po = V3{P.cam.x, P.cam.y, P.cam.z};
+        L.tr = L.tg = L.tb = 1.0f;
+        L.ar = L.ag = L.ab = 0.0f;
+        L.prev_delta = 1;
+        L.last_pdf = 0.0f;
+        L.prev_sumW = 0.0f;
+        L.prev_M = 0;
+        L.pbr = L.pbg = L.pbb = L.pbcos = 0.0f;
+        L.hit = 0;
+        L.max_hits = P.bounces;
+        L.clamp_neg = true;
+        if (L.max_hits <= 0) return kPathEnd;
+        set_ray(L, kExt, L.po, L.d, 1e30f);
+        return kTrace;
+    }
+
+    // One ReSTIR indirect sample: BSDF sample at the primary hit, then a path
+    // from bounce 1 (restir_shade, pathtrace.metal:1137-1174).
+    __device__ int begin_restir_sample(Lane& L) {
+        const float4 g0 = gb.p0[L.pixel], g1 = gb.p1[L.pixel], g2 = gb.p2[L.pixel], g3 = gb.p3[L.pixel];
+        const V3 gpos = xyz(g0), gnor = xyz(g1), galb = xyz(g2), gwo = xyz(g3);
+        float rough = g1.w;
+        if (rough < 0.0f) rough = 0.0f;
+        if (rough > 1.0f) rough = 1.0f;
+        float metallic = g2.w;
+        if (metallic < 0.0f) metallic = 0.0f;
+        if (metallic > 1.0f) metallic = 1.0f;
+        float r2 = rough;
+        if (r2 < 0.02f) r2 = 0.02f;
+        const float alpha2 = (r2 * r2) * (r2 * r2);
+        float spec_prob = 0.05f + 0.95f * metallic;
+        if (spec_prob > 0.98f) spec_prob = 0.98f;
+        if (metallic >= 0.999f) spec_prob = 1.0f;
+        float nx2, ny2, nz2;
+        int pdelta = 0;
+        if (pt.urand() < spec_prob) {
+            float hx, hy, hz;
+            pt.sample_ggx_half(alpha2, gnor.x, gnor.y, gnor.z, hx, hy, hz);
+            const float md = gwo.x * hx + (gwo.y * hy + gwo.z * hz);
+            if (!(md > 1e-8f)) return kPathEnd;
+            nx2 = 2.0f * md * hx - gwo.x; ny2 = 2.0f * md * hy - gwo.y; nz2 = 2.0f * md * hz - gwo.z;
+            pdelta = (rough <= 0.0201f) ? 1 : 0;
+        } else {
+            pt.sample_cosine(gnor.x, gnor.y, gnor.z, nx2, ny2, nz2);
+            pdelta = 0;
+        }
+        const float ncos = nx2 * gnor.x + (ny2 * gnor.y + nz2 * gnor.z);
+        if (!(ncos > 1e-8f)) return kPathEnd;
+        {
+            const float dn = 1.0f / sqrtf(nx2 * nx2 + ny2 * ny2 + nz2 * nz2);
+            nx2 *= dn; ny2 *= dn; nz2 *= dn;
+        }
+        const Bsdf e0 = Tracer<S>::eval_bsdf(nx2, ny2, nz2, gwo.x, gwo.y, gwo.z, gnor.x, gnor.y, gnor.z, alpha2,
+                                             metallic, spec_prob, galb.x, galb.y, galb.z);
+        if (!(e0.pdf > 1e-12f)) return kPathEnd;
+        const float cos0 = nx2 * gnor.x + (ny2 * gnor.y + nz2 * gnor.z);
+        L.tr = e0.r * cos0 / e0.pdf;
+        L.tg = e0.g * cos0 / e0.pdf;
+        L.tb = e0.b * cos0 / e0.pdf;
+        L.ar = L.ag = L.ab = 0.0f;
+        L.po = V3{gpos.x + gnor.x * 0.001f, gpos.y + gnor.y * 0.001f, gpos.z + gnor.z * 0.001f};
+        L.d = V3{nx2, ny2, nz2};
+        L.last_pdf = e0.pdf;
+        L.pbr = e0.r; L.pbg = e0.g; L.pbb = e0.b; L.pbcos = cos0;
+        L.prev_sumW = L.m_sumWu;
+        L.prev_M = L.m_M;
+        L.prev_delta = pdelta;
+        L.hit = 0;
+        L.max_hits = P.bounces - 1;
+        L.clamp_neg = false;
+        if (L.max_hits <= 0) return kPathEnd;
+        set_ray(L, kExt, L.po, L.d, 1e30f);
+        return kTrace;
+    }
+
+    __device__ V3 clamped(float r, float g, float b) const {
+        const float c = P.clamp_value;
+        return c > 0.0f ? V3{fminf(r, c), fminf(g, c), fminf(b, c)} : V3{r, g, b};
+    }
+
+    __device__ void finish_pixel(Lane& L) {
+        if (!SHADE) {
+            const float mr = L.sr / float(L.n), mg = L.sg / float(L.n), mb = L.sb / float(L.n);
+            rgb[3 * L.pixel] = tonemap8(mr);
+            rgb[3 * L.pixel + 1] = tonemap8(mg);
+            rgb[3 * L.pixel + 2] = tonemap8(mb);
+            if (hdr) hdr[L.pixel] = make_float4(mr, mg, mb, 0.0f);
+        } else {
+            const float fn = float(max(L.n, 1));
+            V3 c{L.sr / fn, L.sg / fn, L.sb / fn};
+            if (L.kind == 3) c = clamped(c.x, c.y, c.z);
+            hdr[L.pixel] = make_float4(c.x, c.y, c.z, 0.0f);
+        }
+        L.phase = kIdle;
+    }
+
+    // Accounts the path that just ended and starts the next sample; loops over
+    // samples that finish without a ray. Returns kTrace or kPixelDone.
+    __device__ int next_sample(Lane& L) {
+        for (;;) {
+            bool stop = false;
+            if (L.kind == 1) {
+                const V3 full = clamped(L.dr + L.ar, L.dg + L.ag, L.db + L.ab);
+                L.sr += full.x; L.sg += full.y; L.sb += full.z;
+            } else {
+                const V3 c = clamped(L.ar, L.ag, L.ab);
+                L.sr += c.x; L.sg += c.y; L.sb += c.z;
+                if (!SHADE && P.adapt_min > 0) {
+                    const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+                    L.sl += lum;
+                    L.sl2 += lum * lum;
+                    const int n = L.n + 1;
+                    if (n >= P.adapt_min && n > 1 && n % P.adapt_step == 0) {
+                        const float mean = L.sl / float(n);
+                        const float var = fmaxf(0.0f, (L.sl2 - L.sl * mean) / float(n - 1));
+                        const float th = P.adapt_rel * mean + P.adapt_abs;
+                        stop = var < th * th;
+                    }
+                }
+            }
+            ++L.n;
+            ++L.s;
+            if (stop || L.s >= P.spp) {
+                finish_pixel(L);
+                return kPixelDone;
+            }
+            const int r = L.kind == 1 ? begin_restir_sample(L) : begin_camera_sample(L);
+            if (r == kTrace) return kTrace;
+            L.ar = L.ag = L.ab = 0.0f;  // a sample without a path adds the direct term only
+        }
+    }
+
+    // Starts pixel `px`. Returns kTrace, or kPixelDone when it needs no ray.
+    __device__ int start_pixel(Lane& L, int px) {
+        L.pixel = px;
+        L.s = L.n = 0;
+        L.sr = L.sg = L.sb = L.sl = L.sl2 = 0.0f;
+        const uint32_t x = uint32_t(px) % uint32_t(P.width), y = uint32_t(px) / uint32_t(P.width);
+        if (!SHADE) {
+            L.kind = 3;
+            pt.rng.seed(pixel_seed(P.seed, x, y));
+        } else {
+            const float4 g0 = gb.p0[px];
+            L.kind = ibits(g0.w);
+            if (L.kind == 0) {
+                hdr[px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                L.phase = kIdle;
+                return kPixelDone;
+            }
+            pt.rng.seed(P.seed ^ (uint64_t(px + 1) * 0xBF58476D1CE4E5B9ULL) ^ 0x94D049BB133111EBULL);
+            if (L.kind == 1) {
+                if (!prepare_direct(L, px)) return kPixelDone;
+            }
+        }
+        const int r = L.kind == 1 ? begin_restir_sample(L) : begin_camera_sample(L);
+        if (r == kTrace) return kTrace;
+        L.ar = L.ag = L.ab = 0.0f;
+        return next_sample(L);
+    }
+
+    // Primary direct lighting from the merged reservoir (no shadow ray: the
+    // visibility was reused), pathtrace.metal:1086-1134. Returns false when the
+    // pixel is complete without indirect samples.
+    __device__ bool prepare_direct(Lane& L, int px) {
+        const float4 g0 = gb.p0[px], g1 = gb.p1[px], g2 = gb.p2[px], g3 = gb.p3[px], g4 = gb.p4[px];
+        const V3 gpos = xyz(g0), gnor = xyz(g1), galb = xyz(g2), gwo = xyz(g3), gemi = xyz(g4);
+        float rough = g1.w;
+        if (rough < 0.0f) rough = 0.0f;
+        if (rough > 1.0f) rough = 1.0f;
+        float metallic = g2.w;
+        if (metallic < 0.0f) metallic = 0.0f;
+        if (metallic > 1.0f) metallic = 1.0f;
+        float r2 = rough;
+        if (r2 < 0.02f) r2 = 0.02f;
+        const float alpha2 = (r2 * r2) * (r2 * r2);
+        float spec_prob = 0.05f + 0.95f * metallic;
+        if (spec_prob > 0.98f) spec_prob = 0.98f;
+        if (metallic >= 0.999f) spec_prob = 1.0f;
+        V3 prim{0.0f, 0.0f, 0.0f};
+        L.m_sumWu = 0.0f;
+        L.m_M = 0;
+        const float4 m0 = rm.r0[px], m1 = rm.r1[px], m2 = rm.r2[px], m3 = rm.r3[px];
+        const uint32_t valid = __float_as_uint(m3.z), M = __float_as_uint(m3.y);
+        if (valid != 0u && M > 0u && m3.x > 0.0f && m2.w > 0.0f && m0.w > 0.0f) {
+            const V3 lp = xyz(m0), ln = xyz(m1), le = xyz(m2);
+            const float lalpha = m1.w;
+            L.m_sumWu = m3.x;
+            L.m_M = M;
+            const float m_sumWs = m2.w;
+            const float ldx = lp.x - gpos.x, ldy = lp.y - gpos.y, ldz = lp.z - gpos.z;
+            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
+            if (dist > 0.003f) {
+                const float inv = 1.0f / dist;
+                const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
+                const float lcos = -(ln.x * dxn + (ln.y * dyn + ln.z * dzn));
+                const float scos = gnor.x * dxn + (gnor.y * dyn + gnor.z * dzn);
+                if (lcos > 1e-8f && scos > 1e-8f) {
+                    const Bsdf e = Tracer<S>::eval_bsdf(dxn, dyn, dzn, gwo.x, gwo.y, gwo.z, gnor.x, gnor.y, gnor.z,
+                                                        alpha2, metallic, spec_prob, galb.x, galb.y, galb.z);
+                    const float Fr = le.x * e.r * scos * lalpha, Fg = le.y * e.g * scos * lalpha,
+                                Fb = le.z * e.b * scos * lalpha;
+                    const float ph = lum3(Fr, Fg, Fb);
+                    if (ph > 0.0f) {
+                        const float pdf_ris = float(L.m_M) * ph / L.m_sumWu;
+                        float wnee = 1.0f;
+                        if (!P.nomis && e.pdf > 1e-20f) wnee = pdf_ris / (pdf_ris + e.pdf);
+                        const float W = m_sumWs / (float(L.m_M) * ph);
+                        prim = V3{Fr * W * wnee, Fg * W * wnee, Fb * W * wnee};
+                    }
+                }
+            }
+        }
+        L.dr = gemi.x + prim.x;
+        L.dg = gemi.y + prim.y;
+        L.db = gemi.z + prim.z;
+        if (P.bounces <= 1 || P.spp <= 0) {
+            const V3 c = clamped(L.dr, L.dg, L.db);
+            hdr[px] = make_float4(c.x, c.y, c.z, 0.0f);
+            L.phase = kIdle;
+            return false;
+        }
+        return true;
+    }
+
+    // End of one bounce (pathtrace.metal:708-725, 1244-1250).
+    __device__ int post_bounce(Lane& L) {
+        if (L.clamp_neg) {
+            if (L.tr < 0.0f) L.tr = 0.0f;
+            if (L.tg < 0.0f) L.tg = 0.0f;
+            if (L.tb < 0.0f) L.tb = 0.0f;
+        }
+        float mx = fmaxf(L.tr, L.tg);
+        mx = fmaxf(mx, L.tb);
+        if (!(mx >= 1e-12f)) return kPathEnd;
+        if (L.hit >= 1) {
+            float rr = mx;
+            if (rr < 0.05f) rr = 0.05f;
+            if (rr > 0.9f) rr = 0.9f;
+            if (!(pt.urand() <= rr)) return kPathEnd;
+            const float ir = 1.0f / rr;
+            L.tr *= ir; L.tg *= ir; L.tb *= ir;
+        }
+        ++L.hit;
+        if (L.hit >= L.max_hits) return kPathEnd;
+        set_ray(L, kExt, L.po, L.d, 1e30f);
+        return kTrace;
+    }
+
+    // After the shadow ray of an opaque vertex: add the light sample, sample
+    // the BSDF for the next direction (pathtrace.metal:666-707).
+    __device__ int after_nee(Lane& L, bool lit) {
+        const float dlr = lit ? L.nr : 0.0f, dlg = lit ? L.ng : 0.0f, dlb = lit ? L.nb : 0.0f;
+        L.ar += dlr * L.tr;
+        L.ag += dlg * L.tg;
+        L.ab += dlb * L.tb;
+        const float wox = -L.d.x, woy = -L.d.y, woz = -L.d.z;
+        const V3 pn = L.pn;
+        float nx2, ny2, nz2;
+        if (pt.urand() < L.h_spec) {
+            float hx, hy, hz;
+            pt.sample_ggx_half(L.h_alpha2, pn.x, pn.y, pn.z, hx, hy, hz);
+            const float md = wox * hx + (woy * hy + woz * hz);
+            if (!(md > 1e-8f)) return kPathEnd;
+            nx2 = 2.0f * md * hx - wox; ny2 = 2.0f * md * hy - woy; nz2 = 2.0f * md * hz - woz;
+            L.prev_delta = (L.h_rough <= 0.0201f) ? 1 : 0;
+        } else {
+            pt.sample_cosine(pn.x, pn.y, pn.z, nx2, ny2, nz2);
+            L.prev_delta = 0;
+        }
+        const float ncos = nx2 * pn.x + (ny2 * pn.y + nz2 * pn.z);
+        if (!(ncos > 1e-8f)) return kPathEnd;
+        {
+            const float dn = 1.0f / sqrtf(nx2 * nx2 + ny2 * ny2 + nz2 * nz2);
+            nx2 *= dn; ny2 *= dn; nz2 *= dn;
+        }
+        const Bsdf e = Tracer<S>::eval_bsdf(nx2, ny2, nz2, wox, woy, woz, pn.x, pn.y, pn.z, L.h_alpha2, L.h_met,
+                                            L.h_spec, L.h_ar, L.h_ag, L.h_ab);
+        if (!(e.pdf > 1e-12f)) return kPathEnd;
+        L.last_pdf = e.pdf;
+        const float cos_i = nx2 * pn.x + (ny2 * pn.y + nz2 * pn.z);
+        L.tr *= e.r * cos_i / e.pdf;
+        L.tg *= e.g * cos_i / e.pdf;
+        L.tb *= e.b * cos_i / e.pdf;
+        L.prev_sumW = L.nsumW;
+        L.prev_M = L.nM;
+        L.pbr = e.r; L.pbg = e.g; L.pbb = e.b; L.pbcos = cos_i;
+        L.po = V3{L.p.x + pn.x * 0.001f, L.p.y + pn.y * 0.001f, L.p.z + pn.z * 0.001f};
+        L.d = V3{nx2, ny2, nz2};
+        return post_bounce(L);
+    }
+
+    // A path vertex (the body of the bounce loops, pathtrace.metal:567-707).
+    __device__ int shade_hit(Lane& L, const Surface& hit) {
+        float pnx = hit.n_interp.x, pny = hit.n_interp.y, pnz = hit.n_interp.z;
+        const float geom = hit.n_geom.x * pnx + (hit.n_geom.y * pny + hit.n_geom.z * pnz);
+        if (geom < 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
+        const float ndot = L.d.x * pnx + (L.d.y * pny + L.d.z * pnz);
+        bool front = true;
+        if (ndot > 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; front = false; }
+        const float wox = -L.d.x, woy = -L.d.y, woz = -L.d.z;
+        float rough = hit.rough;
+        if (rough < 0.0f) rough = 0.0f;
+        if (rough > 1.0f) rough = 1.0f;
+        if (rough < 0.02f) rough = 0.02f;
+        const float alpha = rough * rough;
+        const float alpha2 = alpha * alpha;
+        float metallic = hit.metallic;
+        if (metallic < 0.0f) metallic = 0.0f;
+        if (metallic > 1.0f) metallic = 1.0f;
+        if (L.prev_delta == 1) {
+            L.ar += hit.er * L.tr; L.ag += hit.eg * L.tg; L.ab += hit.eb * L.tb;
+        } else {
+            const float w = pt.emission_mis(hit, L.po.x, L.po.y, L.po.z, L.last_pdf, L.prev_sumW, L.prev_M, L.pbr,
+                                            L.pbg, L.pbb, L.pbcos);
+            L.ar += hit.er * L.tr * w; L.ag += hit.eg * L.tg * w; L.ab += hit.eb * L.tb * w;
+        }
+        if (hit.ior > 1.0001f) {
+            const DielectricSample ds = pt.sample_dielectric(wox, woy, woz, pnx, pny, pnz, L.d.x, L.d.y, L.d.z,
+                                                             hit.ior, front, rough, alpha2);
+            if (!ds.valid) return kPathEnd;
+            L.tr *= ds.eta_scale; L.tg *= ds.eta_scale; L.tb *= ds.eta_scale;
+            float sgn = 1.0f;
+            if (ds.n_x * pnx + (ds.n_y * pny + ds.n_z * pnz) < 0.0f) sgn = -1.0f;
+            L.po = V3{hit.p.x + pnx * 0.001f * sgn, hit.p.y + pny * 0.001f * sgn, hit.p.z + pnz * 0.001f * sgn};
+            L.d = V3{ds.n_x, ds.n_y, ds.n_z};
+            L.prev_delta = 1;
+            L.prev_sumW = 0.0f;
+            L.prev_M = 0;
+            return post_bounce(L);
+        }
+        float spec_prob = 0.05f + 0.95f * metallic;
+        if (spec_prob > 0.98f) spec_prob = 0.98f;
+        if (metallic >= 0.999f) spec_prob = 1.0f;
+        const typename Tracer<S>::NeeSample ns = pt.nee_prepare(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy,
+                                                                woz, rough, metallic, spec_prob, hit.ar, hit.ag,
+                                                                hit.ab);
+        L.p = hit.p;
+        L.pn = V3{pnx, pny, pnz};
+        L.h_ar = hit.ar; L.h_ag = hit.ag; L.h_ab = hit.ab;
+        L.h_rough = rough; L.h_alpha2 = alpha2; L.h_met = metallic; L.h_spec = spec_prob;
+        L.nr = ns.r; L.ng = ns.g; L.nb = ns.b;
+        L.nsumW = ns.sumW;
+        L.nM = ns.M;
+        if (ns.shadow) {
+            set_ray(L, kShadow, ns.o, ns.d, ns.dist);
+            return kTrace;
+        }
+        return after_nee(L, false);
+    }
+
+    // Consumes the result of the ray the lane traced.
+    __device__ int on_trace(Lane& L, Hit h) {
+        if (!pt.sc.opaque && h.tri >= 0) {
+            // Stochastic coverage (cast_alpha_ray, pathtrace.metal:536-550).
+            const float opacity = pt.tri_alpha(h.tri);
+            if (!(opacity >= 1.0f || (opacity > 0.0f && pt.urand() < opacity))) {
+                L.traveled += h.t + 0.0001f;
+                if (L.traveled < L.seq_dist) {
+                    L.ro = L.base + L.rd * L.traveled;
+                    L.tmax = L.seq_dist - L.traveled;
+                    return kTrace;
+                }
+                h.tri = -1;
+            }
+        }
+        if (L.phase == kShadow) return after_nee(L, h.tri < 0);
+        if (h.tri < 0) return kPathEnd;
+        return shade_hit(L, pt.surface_at(h, L.ro, L.rd));
+    }
+};
+
+template <int S, bool SHADE>
+__global__ __launch_bounds__(kBlock) void k_wave(DevScene sc, FrameParams P, GBufferSoA gb, ReservoirSoA rm,
+                                                  float4* __restrict__ hdr, uint8_t* __restrict__ rgb,
+                                                  unsigned int* __restrict__ queue, unsigned long long* cnt) {
+    __shared__ uint32_t stack[S * kBlock];
+    Tracer<S> pt(sc, P, stack + threadIdx.x);
+    Engine<S, SHADE> eng{pt, P, gb, rm, hdr, rgb};
+    const int npx = P.width * P.height;
+    const unsigned lane = __lane_id();
+    Lane L;
+    L.phase = kIdle;
+    bool exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(L.phase == kIdle);
+            if (need) {
+                const int count = __popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned base = 0;
+                if (int(lane) == leader) base = atomicAdd(queue, unsigned(count));
+                base = __shfl(base, leader);
+                if (L.phase == kIdle) {
+                    const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(need >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo(unsigned(need), 0u));
+                    const unsigned px = base + rank;
+                    if (px < unsigned(npx)) {
+                        L.phase = kExt;  // provisional; start_pixel sets the real phase
+                        if (eng.start_pixel(L, int(px)) != kTrace) L.phase = kIdle;
+                    }
+                }
+                if (base + unsigned(count) >= unsigned(npx)) exhausted = true;
+            }
+        }
+        if (__ballot(L.phase != kIdle) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (L.phase != kIdle) {
+            const Hit h = pt.intersect(L.ro, L.rd, L.tmax, L.phase == kShadow && sc.opaque);
+            int r = eng.on_trace(L, h);
+            if (r == kPathEnd) r = eng.next_sample(L);
+            if (r == kPixelDone) L.phase = kIdle;
+        }
+    }
+    pt.flush(cnt, SHADE ? kCntShade : kCntMega);
+}
+
+}  // namespace
+
+int wave_blocks(int npx, int stack) {
+    static int per_cu[2][2] = {{0, 0}, {0, 0}};
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 1;
+    }
+    const int si = stack <= 32 ? 0 : 1;
+    int& occ = per_cu[si][0];
+    if (occ == 0) {
+        int a = 0;
+        if (si == 0) hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wave<32, true>, kBlock, 0);
+        else hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wave<64, true>, kBlock, 0);
+        occ = a > 0 ? a : 1;
+    }
+    const int need = (npx + kBlock - 1) / kBlock;
+    return need < cus * occ ? need : cus * occ;
+}
+
+void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
+                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, unsigned int* queue, unsigned long long* cnt,
+                 hipStream_t st) {
+    const int npx = P.width * P.height;
+    const dim3 grid(unsigned(wave_blocks(npx, stack))), block(kBlock);
+    if (stack <= 32) {
+        if (shade) hipLaunchKernelGGL((k_wave<32, true>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+        else hipLaunchKernelGGL((k_wave<32, false>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+    } else {
+        if (shade) hipLaunchKernelGGL((k_wave<64, true>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+        else hipLaunchKernelGGL((k_wave<64, false>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+    }
+}
+
+}  // namespace rmh
