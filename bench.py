@@ -106,7 +106,7 @@ def main() -> int:
 
     for k in range(args.warmup):
         backend.render(frame_settings(k), 12345 + k, want_hdr=False)
-    wall, rays, pass_ms, nodes, tris = [], [], [], [], []
+    wall, rays, pass_ms, nodes, tris, prays = [], [], [], [], [], []
     for k in range(args.warmup, args.warmup + args.steps):
         f = frame_settings(k)
         t0 = time.perf_counter()
@@ -117,6 +117,7 @@ def main() -> int:
         pass_ms.append([st.ms[i] for i in range(8)])
         nodes.append([st.node_visits[i] for i in range(6)])
         tris.append([st.tri_tests[i] for i in range(6)])
+        prays.append([st.rays[i] for i in range(6)])
         if st.path != 1:
             raise RuntimeError("expected the ReSTIR + denoise path")
     backend.close()
@@ -127,6 +128,7 @@ def main() -> int:
     pm = np.mean(np.array(pass_ms), axis=0)
     nd = np.mean(np.array(nodes, np.float64), axis=0)
     tr = np.mean(np.array(tris, np.float64), axis=0)
+    ry = np.mean(np.array(prays, np.float64), axis=0)
     # Dominant kernel among the tracing passes (index = rm_stats pass index).
     tracing = {1: "restir_generate", 2: "restir_spatial", 3: "restir_shade"}
     dom = max(tracing, key=lambda i: pm[i])
@@ -152,6 +154,9 @@ def main() -> int:
                    "restir": {"candidates": 32, "spatial": 3, "radius_px": 30, "temporal": True},
                    "atrous_iterations": 5, "rays_per_frame": int(rays_per_frame),
                    "pass_ms": {PASS_NAMES[i]: round(float(pm[i]), 3) for i in range(1, 7)},
+                   "pass_rays": {tracing[i]: int(ry[i]) for i in tracing},
+                   "pass_nodes_per_ray": {tracing[i]: round(float(nd[i] / max(ry[i], 1.0)), 2) for i in tracing},
+                   "pass_tris_per_ray": {tracing[i]: round(float(tr[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "gpu_frame_ms": round(float(pm[7]), 3)},
         "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

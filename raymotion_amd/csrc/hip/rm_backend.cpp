@@ -33,7 +33,7 @@ struct DevBuf {
     bool ensure(size_t bytes) {
         bytes = std::max<size_t>(bytes, 16);
         if (bytes <= cap) return true;
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
         if (hipMalloc(&ptr, bytes) != hipSuccess) return false;
@@ -43,7 +43,7 @@ struct DevBuf {
         return hipMemset(ptr, 0, bytes) == hipSuccess;
     }
     void release() {
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
     }
@@ -102,7 +102,7 @@ struct rm_ctx {
     DevBuf shade, den_a, hist_color;  // float4 per pixel
     DevBuf rgb8, hdr3, mega_hdr;
     DevBuf counters;
-    DevBuf queue;  // work counter of the persistent path engine
+    DevBuf queue;  // work counters of the persistent passes: [0] generate, [1] shade / megakernel
     DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
 
     // Temporal history (metal.mm:36-40).
@@ -197,9 +197,14 @@ int rm_create(rm_ctx** out, int hip_device) {
         g_create_error = "Could not create HIP stream";
         return RM_ERR_DEVICE;
     }
-    for (auto& e : c->ev) hipEventCreate(&e);
-    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(sizeof(unsigned))) {
-        hipStreamDestroy(c->stream);
+    for (auto& e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+            c->profiling = false;
+        }
+    }
+    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(2 * sizeof(unsigned))) {
+        (void)hipStreamDestroy(c->stream);
         delete c;
         g_create_error = "HIP counter allocation failed";
         return RM_ERR_ALLOC;
@@ -211,8 +216,8 @@ int rm_create(rm_ctx** out, int hip_device) {
 
 void rm_destroy(rm_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->nodes, &c->geo, &c->nrm, &c->nn, &c->matid, &c->mats, &c->uv, &c->lights, &c->cdf,
                       &c->texels, &c->level_nodes, &c->shade, &c->den_a, &c->hist_color, &c->rgb8, &c->hdr3,
                       &c->mega_hdr, &c->counters, &c->queue, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
@@ -223,8 +228,9 @@ void rm_destroy(rm_ctx* c) {
     c->rinit.release();
     c->rmerged.release();
     c->hist_res.release();
-    for (auto& e : c->ev) hipEventDestroy(e);
-    hipStreamDestroy(c->stream);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -236,6 +242,8 @@ const char* rm_last_error(const rm_ctx* c) {
 int rm_set_profiling(rm_ctx* c, int enable) {
     if (!c) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
+    for (auto& e : c->ev)
+        if (enable && !e) return c->fail(RM_ERR_DEVICE, "HIP events unavailable for profiling");
     c->profiling = enable != 0;
     return RM_OK;
 }
@@ -250,7 +258,7 @@ int rm_get_stats(rm_ctx* c, rm_stats* out) {
 int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     if (!c || !s) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
-    hipSetDevice(c->device);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     const uint64_t content = content_key(*s);
     if (c->has_scene && content == c->content && size_t(c->ntri) == s->ntri) {
         c->last_refit = 0;
@@ -319,7 +327,7 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
 int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr) {
     if (!c || !f) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
-    hipSetDevice(c->device);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     if (f->width <= 0 || f->height <= 0 || f->spp <= 0 || f->bounces < 0 ||
         size_t(f->width) * size_t(f->height) > 100000000)
         return c->fail(RM_ERR_INVALID, "Invalid render dimensions or sample count");
@@ -357,7 +365,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     unsigned long long* cnt = c->counters.as<unsigned long long>();
     if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
         return rc;
-    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, sizeof(unsigned), c->stream), "work queue")) return rc;
+    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, 2 * sizeof(unsigned), c->stream), "work queue")) return rc;
     rm_stats st{};
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
@@ -365,7 +373,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     st.has_history = P.has_history;
     const bool prof = c->profiling;
     auto mark = [&](int i) {
-        if (prof) hipEventRecord(c->ev[i], c->stream);
+        if (prof) (void)hipEventRecord(c->ev[i], c->stream);
     };
     mark(0);
     int denoise_passes = 0;
@@ -379,11 +387,11 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         }
         const GBufferSoA gb = c->gbuf.soa(), hg = c->hist_gbuf.soa();
         const ReservoirSoA ri = c->rinit.soa(), rmg = c->rmerged.soa(), hr = c->hist_res.soa();
-        launch_generate(c->stack_need, sc, P, gb, ri, cnt, c->stream);
+        launch_generate(c->stack_need, sc, P, gb, ri, c->queue.as<unsigned>(), cnt, c->stream);
         mark(1);
         launch_spatial(c->stack_need, sc, P, gb, ri, rmg, hg, hr, cnt, c->stream);
         mark(2);
-        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr, c->queue.as<unsigned>(), cnt,
+        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr, c->queue.as<unsigned>() + 1, cnt,
                     c->stream);
         mark(3);
         const float4* final_img = c->shade.as<float4>();
@@ -450,7 +458,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         }
         if (lanes == 1)
             launch_wave(c->stack_need, false, sc, P, GBufferSoA{}, ReservoirSoA{}, hdr ? c->mega_hdr.as<float4>() : nullptr,
-                        c->rgb8.as<uint8_t>(), c->queue.as<unsigned>(), cnt, c->stream);
+                        c->rgb8.as<uint8_t>(), c->queue.as<unsigned>() + 1, cnt, c->stream);
         else
             launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(),
                                 hdr ? c->mega_hdr.as<float4>() : nullptr, cnt, c->stream);
@@ -477,7 +485,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     if (prof) {
         auto ms = [&](int a, int b) {
             float v = 0.0f;
-            hipEventElapsedTime(&v, c->ev[a], c->ev[b]);
+            if (hipEventElapsedTime(&v, c->ev[a], c->ev[b]) != hipSuccess) v = 0.0f;
             return v;
         };
         if (st.path == 1) {
@@ -502,7 +510,7 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
              float* t, float* u, float* v) {
     if (!c || (n > 0 && (!origins || !dirs || !tri))) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
-    hipSetDevice(c->device);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
     if (n == 0) return RM_OK;
     if (n > (size_t(1) << 30)) return c->fail(RM_ERR_INVALID, "too many rays in one call");
@@ -511,23 +519,29 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
         return c->fail(RM_ERR_ALLOC, "HIP ray buffer allocation failed");
     hipStream_t s = c->stream;
     unsigned long long* cnt = c->counters.as<unsigned long long>();
-    hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), s);
-    hipMemcpyAsync(c->probe_o.ptr, origins, n * 12, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync(c->probe_d.ptr, dirs, n * 12, hipMemcpyHostToDevice, s);
-    if (tmax) hipMemcpyAsync(c->probe_t.ptr, tmax, n * 4, hipMemcpyHostToDevice, s);
+    if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), s), "counters")) return rc;
+    if (int rc = check(c, hipMemcpyAsync(c->probe_o.ptr, origins, n * 12, hipMemcpyHostToDevice, s), "ray upload"))
+        return rc;
+    if (int rc = check(c, hipMemcpyAsync(c->probe_d.ptr, dirs, n * 12, hipMemcpyHostToDevice, s), "ray upload"))
+        return rc;
+    if (tmax)
+        if (int rc = check(c, hipMemcpyAsync(c->probe_t.ptr, tmax, n * 4, hipMemcpyHostToDevice, s), "ray upload"))
+            return rc;
     FrameParams P{};
     const DevScene sc = c->dev_scene();
-    if (c->profiling) hipEventRecord(c->ev[0], s);
+    if (c->profiling) (void)hipEventRecord(c->ev[0], s);
     launch_trace(c->stack_need, sc, P, int(n), c->probe_o.as<float>(), c->probe_d.as<float>(),
                  tmax ? c->probe_t.as<float>() : nullptr, any, c->probe_tri.as<int32_t>(), c->probe_tuv.as<float4>(),
                  cnt, s);
-    if (c->profiling) hipEventRecord(c->ev[1], s);
+    if (c->profiling) (void)hipEventRecord(c->ev[1], s);
     if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
     std::vector<float> tuv(n * 4);
     unsigned long long host_cnt[kNumCounters];
-    hipMemcpyAsync(tri, c->probe_tri.ptr, n * 4, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(tuv.data(), c->probe_tuv.ptr, n * 16, hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, s);
+    if (int rc = check(c, hipMemcpyAsync(tri, c->probe_tri.ptr, n * 4, hipMemcpyDeviceToHost, s), "readback")) return rc;
+    if (int rc = check(c, hipMemcpyAsync(tuv.data(), c->probe_tuv.ptr, n * 16, hipMemcpyDeviceToHost, s), "readback"))
+        return rc;
+    if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, s), "counters"))
+        return rc;
     if (int rc = check(c, hipStreamSynchronize(s), "trace")) return rc;
     if (host_cnt[kCntOverflow]) return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
     for (size_t i = 0; i < n; ++i) {
@@ -539,7 +553,7 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
     collect(st, host_cnt);
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
-    if (c->profiling) hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]);
+    if (c->profiling && hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]) != hipSuccess) st.ms[7] = 0.0f;
     c->stats = st;
     return RM_OK;
 }
@@ -547,7 +561,7 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
 int rm_read_restir(rm_ctx* c, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade) {
     if (!c) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
-    hipSetDevice(c->device);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     if (!c->hist_valid) return c->fail(RM_ERR_INVALID, "no ReSTIR frame rendered yet");
     const size_t px = size_t(c->hist_w) * size_t(c->hist_h);
     std::vector<float4> a(px), b(px), d(px), e(px), g(px);

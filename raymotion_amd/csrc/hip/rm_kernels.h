@@ -13,8 +13,9 @@ constexpr int kMaxStack = 64;
 // render_paths_simd with 8 or 32 lanes per pixel.
 void launch_render_paths(int stack, int lanes, const DevScene& sc, const FrameParams& P, uint8_t* out, float4* hdr,
                          unsigned long long* cnt, hipStream_t st);
+// Persistent restir_generate; `queue` must hold 0 when the kernel starts.
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
-                     unsigned long long* cnt, hipStream_t st);
+                     unsigned int* queue, unsigned long long* cnt, hipStream_t st);
 void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                     const ReservoirSoA& ro, const GBufferSoA& hg, const ReservoirSoA& hr, unsigned long long* cnt,
                     hipStream_t st);

@@ -27,57 +27,36 @@ __global__ __launch_bounds__(kBlock) void k_render_paths(DevScene sc, FrameParam
     const uint32_t x = pixel % uint32_t(P.width), y = pixel / uint32_t(P.width);
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     uint64_t s = pixel_seed(P.seed, x, y);
-    if (LANES > 1) s ^= uint64_t(lane) * 0x94D049BB133111EBULL;
+    s ^= uint64_t(lane) * 0x94D049BB133111EBULL;
     pt.rng.seed(s);
     float sr = 0.0f, sg = 0.0f, sb = 0.0f, sum_l = 0.0f, sum_l2 = 0.0f;
     int n = 0;
-    if (LANES == 1) {
-        for (int k = 0; k < P.spp; ++k) {
+    for (int base = 0; base < P.spp; base += LANES) {
+        V3 c{0.0f, 0.0f, 0.0f};
+        if (base + int(lane) < P.spp) {
             float sx, sy;
             pt.jitter(x, y, sx, sy);
-            const V3 c = clamp3(pt.sample_sensor(sx, sy), P.clamp_value);
-            sr += c.x; sg += c.y; sb += c.z;
-            ++n;
-            if (P.adapt_min > 0) {
-                const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
-                sum_l += lum;
-                sum_l2 += lum * lum;
-                if (n >= P.adapt_min && n > 1 && n % P.adapt_step == 0) {
-                    const float mean = sum_l / float(n);
-                    const float var = fmaxf(0.0f, (sum_l2 - sum_l * mean) / float(n - 1));
-                    const float th = P.adapt_rel * mean + P.adapt_abs;
-                    if (var < th * th) break;
-                }
-            }
+            c = clamp3(pt.sample_sensor(sx, sy), P.clamp_value);
         }
-    } else {
-        for (int base = 0; base < P.spp; base += LANES) {
-            V3 c{0.0f, 0.0f, 0.0f};
-            if (base + int(lane) < P.spp) {
-                float sx, sy;
-                pt.jitter(x, y, sx, sy);
-                c = clamp3(pt.sample_sensor(sx, sy), P.clamp_value);
-            }
-            float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
-            float lum2 = lum * lum;
+        float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+        float lum2 = lum * lum;
 #pragma unroll
-            for (int mask = 1; mask < LANES; mask *= 2) {
-                c.x += __shfl_xor(c.x, mask, LANES);
-                c.y += __shfl_xor(c.y, mask, LANES);
-                c.z += __shfl_xor(c.z, mask, LANES);
-                lum += __shfl_xor(lum, mask, LANES);
-                lum2 += __shfl_xor(lum2, mask, LANES);
-            }
-            sr += c.x; sg += c.y; sb += c.z;
-            sum_l += lum;
-            sum_l2 += lum2;
-            n = min(base + LANES, P.spp);
-            if (P.adapt_min > 0 && n >= P.adapt_min && n > 1 && n % P.adapt_step == 0) {
-                const float mean = sum_l / float(n);
-                const float var = fmaxf(0.0f, (sum_l2 - sum_l * mean) / float(n - 1));
-                const float th = P.adapt_rel * mean + P.adapt_abs;
-                if (var < th * th) break;
-            }
+        for (int mask = 1; mask < LANES; mask *= 2) {
+            c.x += __shfl_xor(c.x, mask, LANES);
+            c.y += __shfl_xor(c.y, mask, LANES);
+            c.z += __shfl_xor(c.z, mask, LANES);
+            lum += __shfl_xor(lum, mask, LANES);
+            lum2 += __shfl_xor(lum2, mask, LANES);
+        }
+        sr += c.x; sg += c.y; sb += c.z;
+        sum_l += lum;
+        sum_l2 += lum2;
+        n = min(base + LANES, P.spp);
+        if (P.adapt_min > 0 && n >= P.adapt_min && n > 1 && n % P.adapt_step == 0) {
+            const float mean = sum_l / float(n);
+            const float var = fmaxf(0.0f, (sum_l2 - sum_l * mean) / float(n - 1));
+            const float th = P.adapt_rel * mean + P.adapt_abs;
+            if (var < th * th) break;
         }
     }
     pt.flush(cnt, kCntMega);
@@ -88,119 +67,6 @@ __global__ __launch_bounds__(kBlock) void k_render_paths(DevScene sc, FrameParam
         out[3 * pixel + 2] = tonemap8(mb);
         if (hdr) hdr[pixel] = make_float4(mr, mg, mb, 0.0f);
     }
-}
-
-// restir_generate (pathtrace.metal:817-917): primary hit into the G-buffer and
-// an initial reservoir from Mc light candidates, each shadow-tested.
-template <int S>
-__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, FrameParams P, GBufferSoA gb, ReservoirSoA ri,
-                                                      unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
-    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
-    if (id >= uint32_t(P.width) * uint32_t(P.height)) return;
-    const uint32_t x = id % uint32_t(P.width), y = id / uint32_t(P.width);
-    Tracer<S> pt(sc, P, stack + threadIdx.x);
-    pt.rng.seed(pixel_seed(P.seed, x, y));
-    const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    ri.r0[id] = zero; ri.r1[id] = zero; ri.r2[id] = zero; ri.r3[id] = zero;  // valid = 0
-    float sx, sy;
-    pt.jitter(x, y, sx, sy);
-    const V3 rd = pt.primary_dir(sx, sy, true);
-    const V3 ro{P.cam.x, P.cam.y, P.cam.z};
-    const Surface hit = pt.cast_alpha_ray(ro, rd, 1e30f);
-    if (hit.tri < 0) {
-        gb.p0[id] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
-        gb.p1[id] = zero; gb.p2[id] = zero; gb.p3[id] = zero; gb.p4[id] = zero;
-        pt.flush(cnt, kCntGenerate);
-        return;
-    }
-    float pnx = hit.n_interp.x, pny = hit.n_interp.y, pnz = hit.n_interp.z;
-    const float geom = hit.n_geom.x * pnx + (hit.n_geom.y * pny + hit.n_geom.z * pnz);
-    if (geom < 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
-    const float ndot = rd.x * pnx + (rd.y * pny + rd.z * pnz);
-    if (ndot > 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
-    const float wox = -rd.x, woy = -rd.y, woz = -rd.z;
-    float rough = hit.rough;
-    if (rough < 0.0f) rough = 0.0f;
-    if (rough > 1.0f) rough = 1.0f;
-    float metallic = hit.metallic;
-    if (metallic < 0.0f) metallic = 0.0f;
-    if (metallic > 1.0f) metallic = 1.0f;
-    const bool dielectric = hit.ior > 1.0001f;
-    gb.p0[id] = make_float4(hit.p.x, hit.p.y, hit.p.z, __int_as_float(dielectric ? 3 : 1));
-    gb.p1[id] = make_float4(pnx, pny, pnz, rough);
-    gb.p2[id] = make_float4(hit.ar, hit.ag, hit.ab, metallic);
-    gb.p3[id] = make_float4(wox, woy, woz, dielectric ? hit.ior : 1.0f);
-    gb.p4[id] = make_float4(hit.er, hit.eg, hit.eb, 0.0f);
-    if (dielectric || sc.nlights == 0 || !(sc.light_total > 1e-12f)) {
-        pt.flush(cnt, kCntGenerate);
-        return;
-    }
-    int Mc = P.restir_candidates;
-    if (Mc < 1) Mc = 1;
-    if (Mc > 32) Mc = 32;
-    float r2 = rough;
-    if (r2 < 0.02f) r2 = 0.02f;
-    const float alpha2 = (r2 * r2) * (r2 * r2);
-    float spec_prob = 0.05f + 0.95f * metallic;
-    if (spec_prob > 0.98f) spec_prob = 0.98f;
-    if (metallic >= 0.999f) spec_prob = 1.0f;
-    const float px = hit.p.x, py = hit.p.y, pz = hit.p.z;
-    float sumWs = 0.0f, sumWu = 0.0f;
-    float4 sel0 = zero, sel1 = zero, sel2 = zero;
-    float sel_ph = 0.0f;
-    bool have = false;
-    const V3 so{px + pnx * 0.002f, py + pny * 0.002f, pz + pnz * 0.002f};
-    for (int j = 0; j < Mc; ++j) {
-        const float pick = pt.urand() * sc.light_total;
-        const float4* L = sc.lights + size_t(pt.pick_light(pick)) * kLightWords;
-        const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
-        float lu = pt.urand(), lv = pt.urand();
-        if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
-        const float lw = 1.0f - lu - lv;
-        const float qx = lw * l0.x + (lu * l0.w + lv * l1.z);
-        const float qy = lw * l0.y + (lu * l1.x + lv * l1.w);
-        const float qz = lw * l0.z + (lu * l1.y + lv * l2.x);
-        const float ldx = qx - px, ldy = qy - py, ldz = qz - pz;
-        const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
-        float wu = 0.0f, ws = 0.0f, ph = 0.0f;
-        if (dist > 0.003f) {
-            const float inv = 1.0f / dist;
-            const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
-            const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
-            const float scos = pnx * dxn + (pny * dyn + pnz * dzn);
-            if (lcos > 1e-8f && scos > 1e-8f) {
-                float ll = lum3(l3.x, l3.y, l3.z);
-                if (ll < 1e-6f) ll = 1e-6f;
-                const float q = dist * dist * ll / (lcos * sc.light_total);
-                if (q > 1e-12f) {
-                    const Bsdf e = Tracer<S>::eval_bsdf(dxn, dyn, dzn, wox, woy, woz, pnx, pny, pnz, alpha2, metallic,
-                                                        spec_prob, hit.ar, hit.ag, hit.ab);
-                    ph = lum3(l3.x * e.r * scos * l3.w, l3.y * e.g * scos * l3.w, l3.z * e.b * scos * l3.w);
-                    if (ph > 0.0f) {
-                        wu = ph / q;
-                        if (!pt.occluded(so, {dxn, dyn, dzn}, dist - 0.003f)) ws = wu;
-                    }
-                }
-            }
-        }
-        sumWu += wu;
-        sumWs += ws;
-        const float r = pt.urand();
-        if (ws > 0.0f && r * sumWs < ws) {
-            have = true;
-            sel0 = make_float4(qx, qy, qz, 0.0f);
-            sel1 = make_float4(l2.y, l2.z, l2.w, l3.w);
-            sel2 = make_float4(l3.x, l3.y, l3.z, 0.0f);
-            sel_ph = ph;
-        }
-    }
-    pt.flush(cnt, kCntGenerate);
-    if (!have || !(sumWs > 0.0f) || !(sumWu > 0.0f) || !(sel_ph > 0.0f)) return;
-    ri.r0[id] = make_float4(sel0.x, sel0.y, sel0.z, sel_ph);
-    ri.r1[id] = sel1;
-    ri.r2[id] = make_float4(sel2.x, sel2.y, sel2.z, sumWs);
-    ri.r3[id] = make_float4(sumWu, __uint_as_float(uint32_t(Mc)), __uint_as_float(1u), 0.0f);
 }
 
 __device__ __forceinline__ bool reproject(const FrameParams& P, V3 p, uint32_t& hidx) {  // pathtrace.metal:957-966
@@ -502,12 +368,6 @@ void launch_render_paths(int stack, int lanes, const DevScene& sc, const FramePa
         if (lanes == 32) hipLaunchKernelGGL((k_render_paths<S, 32>), grid, block, 0, st, sc, P, out, hdr, cnt);
         else hipLaunchKernelGGL((k_render_paths<S, 8>), grid, block, 0, st, sc, P, out, hdr, cnt);
     });
-}
-
-void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
-                     unsigned long long* cnt, hipStream_t st) {
-    const dim3 grid(blocks_for(size_t(P.width) * size_t(P.height))), block(kBlock);
-    RM_STACK_DISPATCH(stack, hipLaunchKernelGGL((k_generate<S>), grid, block, 0, st, sc, P, gb, ri, cnt));
 }
 
 void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,

@@ -1,6 +1,6 @@
-// Persistent path engine for the two long-running passes: render_paths with
-// one lane per pixel (pathtrace.metal:739-769) and restir_shade
-// (pathtrace.metal:1056-1258).
+// Persistent engines for the long-running passes: render_paths with one lane
+// per pixel (pathtrace.metal:739-769), restir_shade (pathtrace.metal:1056-1258)
+// and restir_generate (pathtrace.metal:817-917).
 //
 // The reference runs each pixel's whole sample loop inside one thread, so the
 // threads of a wavefront sit at different places of their paths and trace
@@ -23,7 +23,7 @@ namespace {
 enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2 };
 enum Step : int { kTrace = 0, kPathEnd = 1, kPixelDone = 2 };
 
-struct Lane {
+struct PathLane {
     int phase;
     int pixel, kind;  // kind 1: ReSTIR indirect samples, 3: camera paths
     int s, n;
@@ -50,14 +50,30 @@ struct Lane {
     uint32_t nM;
 };
 
+// Buffers a persistent pass reads or writes.
+struct PassIO {
+    GBufferSoA gb;
+    ReservoirSoA res;
+    float4* hdr;
+    uint8_t* rgb;
+};
+
 template <int S, bool SHADE>
-struct Engine {
+struct PathEngine {
+    using Lane = PathLane;
+    static constexpr int kPass = SHADE ? kCntShade : kCntMega;
+    // Resident 256-lane blocks per CU the register allocation targets (3 waves
+    // per SIMD: a little spilling, measured faster than 2 waves without). A
+    // 64-entry stack takes 64 KiB of LDS per block, which allows 2.
+    static constexpr int kMinBlocks = S <= 32 ? 3 : 2;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
     const ReservoirSoA& rm;
     float4* hdr;
     uint8_t* rgb;
+    __device__ PathEngine(Tracer<S>& t, const FrameParams& p, const PassIO& io)
+        : pt(t), P(p), gb(io.gb), rm(io.res), hdr(io.hdr), rgb(io.rgb) {}
 
     __device__ void set_ray(Lane& L, int phase, V3 o, V3 dir, float dist) {
         L.phase = phase;
@@ -444,18 +460,235 @@ struct Engine {
         if (h.tri < 0) return kPathEnd;
         return shade_hit(L, pt.surface_at(h, L.ro, L.rd));
     }
+
+    // Engine interface of k_persistent: start a pixel / consume a trace.
+    // Both return kTrace (a ray is set up) or kPixelDone.
+    __device__ int start(Lane& L, int px) { return start_pixel(L, px); }
+    __device__ int step(Lane& L, const Hit& h) {
+        const int r = on_trace(L, h);
+        return r == kPathEnd ? next_sample(L) : r;
+    }
+    __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
 };
 
-template <int S, bool SHADE>
-__global__ __launch_bounds__(kBlock) void k_wave(DevScene sc, FrameParams P, GBufferSoA gb, ReservoirSoA rm,
-                                                  float4* __restrict__ hdr, uint8_t* __restrict__ rgb,
-                                                  unsigned int* __restrict__ queue, unsigned long long* cnt) {
+
+// restir_generate (pathtrace.metal:817-917) as a state machine: the primary
+// ray (with its stochastic-coverage walk), then up to Mc light candidates,
+// each with a shadow ray when its target function is positive.
+struct GenLane {
+    int phase;
+    V3 ro, rd, base;
+    float tmax, traveled, seq_dist;
+    int pixel, j, Mc;
+    V3 p, pn, wo, alb;
+    float alpha2, met, spec;
+    float sumWs, sumWu, sel_ph;
+    bool have;
+    // A candidate is kept as its light and folded barycentrics; the light
+    // point is recomputed from them with the same arithmetic when needed.
+    int sel_light;
+    float sel_lu, sel_lv;
+    int c_light;  // candidate waiting for its shadow ray
+    float c_lu, c_lv, c_wu, c_ph;
+};
+
+template <int S>
+struct GenEngine {
+    using Lane = GenLane;
+    static constexpr int kPass = kCntGenerate;
+    static constexpr int kMinBlocks = S <= 32 ? 4 : 2;  // shadow rays: latency hiding wins over the spills
+    Tracer<S>& pt;
+    const FrameParams& P;
+    const GBufferSoA& gb;
+    const ReservoirSoA& ri;
+    __device__ GenEngine(Tracer<S>& t, const FrameParams& p, const PassIO& io) : pt(t), P(p), gb(io.gb), ri(io.res) {}
+
+    __device__ void set_ray(Lane& L, int phase, V3 o, V3 dir, float dist) {
+        L.phase = phase;
+        L.ro = o;
+        L.base = o;
+        L.rd = dir;
+        L.tmax = dist;
+        L.traveled = 0.0f;
+        L.seq_dist = dist;
+    }
+
+    __device__ int start(Lane& L, int px) {
+        const uint32_t x = uint32_t(px) % uint32_t(P.width), y = uint32_t(px) / uint32_t(P.width);
+        L.pixel = px;
+        pt.rng.seed(pixel_seed(P.seed, x, y));
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ri.r0[px] = zero; ri.r1[px] = zero; ri.r2[px] = zero; ri.r3[px] = zero;  // valid = 0
+        float sx, sy;
+        pt.jitter(x, y, sx, sy);
+        set_ray(L, kExt, V3{P.cam.x, P.cam.y, P.cam.z}, pt.primary_dir(sx, sy, true), 1e30f);
+        return kTrace;
+    }
+
+    __device__ void finish_candidate(Lane& L, float wu, float ws, float ph, int li, float lu, float lv) {
+        L.sumWu += wu;
+        L.sumWs += ws;
+        const float r = pt.urand();
+        if (ws > 0.0f && r * L.sumWs < ws) {
+            L.have = true;
+            L.sel_light = li;
+            L.sel_lu = lu;
+            L.sel_lv = lv;
+            L.sel_ph = ph;
+        }
+        ++L.j;
+    }
+
+    // Uniform point of light `li` at folded barycentrics (lu, lv).
+    __device__ V3 light_point(const float4* Lt, float lu, float lv) const {
+        const float4 l0 = Lt[0], l1 = Lt[1], l2 = Lt[2];
+        const float lw = 1.0f - lu - lv;
+        return V3{lw * l0.x + (lu * l0.w + lv * l1.z), lw * l0.y + (lu * l1.x + lv * l1.w),
+                  lw * l0.z + (lu * l1.y + lv * l2.x)};
+    }
+
+    // Draws candidates until one needs a shadow ray (kTrace) or all are done.
+    __device__ int next_candidate(Lane& L) {
+        const DevScene& sc = pt.sc;
+        while (L.j < L.Mc) {
+            const float pick = pt.urand() * sc.light_total;
+            const int li = pt.pick_light(pick);
+            const float4* Lt = sc.lights + size_t(li) * kLightWords;
+            float lu = pt.urand(), lv = pt.urand();
+            if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
+            const V3 q = light_point(Lt, lu, lv);
+            const float ldx = q.x - L.p.x, ldy = q.y - L.p.y, ldz = q.z - L.p.z;
+            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
+            if (dist > 0.003f) {
+                const float4 l2 = Lt[2], l3 = Lt[3];
+                const float inv = 1.0f / dist;
+                const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
+                const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
+                const float scos = L.pn.x * dxn + (L.pn.y * dyn + L.pn.z * dzn);
+                if (lcos > 1e-8f && scos > 1e-8f) {
+                    float ll = lum3(l3.x, l3.y, l3.z);
+                    if (ll < 1e-6f) ll = 1e-6f;
+                    const float qd = dist * dist * ll / (lcos * sc.light_total);
+                    if (qd > 1e-12f) {
+                        const Bsdf e = Tracer<S>::eval_bsdf(dxn, dyn, dzn, L.wo.x, L.wo.y, L.wo.z, L.pn.x, L.pn.y,
+                                                            L.pn.z, L.alpha2, L.met, L.spec, L.alb.x, L.alb.y,
+                                                            L.alb.z);
+                        const float ph = lum3(l3.x * e.r * scos * l3.w, l3.y * e.g * scos * l3.w,
+                                              l3.z * e.b * scos * l3.w);
+                        if (ph > 0.0f) {
+                            L.c_wu = ph / qd;
+                            L.c_ph = ph;
+                            L.c_light = li;
+                            L.c_lu = lu;
+                            L.c_lv = lv;
+                            const V3 so{L.p.x + L.pn.x * 0.002f, L.p.y + L.pn.y * 0.002f, L.p.z + L.pn.z * 0.002f};
+                            set_ray(L, kShadow, so, V3{dxn, dyn, dzn}, dist - 0.003f);
+                            return kTrace;
+                        }
+                        finish_candidate(L, 0.0f, 0.0f, ph, li, lu, lv);
+                        continue;
+                    }
+                }
+            }
+            finish_candidate(L, 0.0f, 0.0f, 0.0f, li, lu, lv);
+        }
+        if (L.have && L.sumWs > 0.0f && L.sumWu > 0.0f && L.sel_ph > 0.0f) {
+            const int px = L.pixel;
+            const float4* Lt = sc.lights + size_t(L.sel_light) * kLightWords;
+            const float4 l2 = Lt[2], l3 = Lt[3];
+            const V3 q = light_point(Lt, L.sel_lu, L.sel_lv);
+            ri.r0[px] = make_float4(q.x, q.y, q.z, L.sel_ph);
+            ri.r1[px] = make_float4(l2.y, l2.z, l2.w, l3.w);
+            ri.r2[px] = make_float4(l3.x, l3.y, l3.z, L.sumWs);
+            ri.r3[px] = make_float4(L.sumWu, __uint_as_float(uint32_t(L.Mc)), __uint_as_float(1u), 0.0f);
+        }
+        return kPixelDone;
+    }
+
+    __device__ int primary_hit(Lane& L, const Surface& hit) {
+        const int px = L.pixel;
+        const V3 rd = L.rd;
+        float pnx = hit.n_interp.x, pny = hit.n_interp.y, pnz = hit.n_interp.z;
+        const float geom = hit.n_geom.x * pnx + (hit.n_geom.y * pny + hit.n_geom.z * pnz);
+        if (geom < 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
+        const float ndot = rd.x * pnx + (rd.y * pny + rd.z * pnz);
+        if (ndot > 0.0f) { pnx = -pnx; pny = -pny; pnz = -pnz; }
+        const float wox = -rd.x, woy = -rd.y, woz = -rd.z;
+        float rough = hit.rough;
+        if (rough < 0.0f) rough = 0.0f;
+        if (rough > 1.0f) rough = 1.0f;
+        float metallic = hit.metallic;
+        if (metallic < 0.0f) metallic = 0.0f;
+        if (metallic > 1.0f) metallic = 1.0f;
+        const bool dielectric = hit.ior > 1.0001f;
+        gb.p0[px] = make_float4(hit.p.x, hit.p.y, hit.p.z, __int_as_float(dielectric ? 3 : 1));
+        gb.p1[px] = make_float4(pnx, pny, pnz, rough);
+        gb.p2[px] = make_float4(hit.ar, hit.ag, hit.ab, metallic);
+        gb.p3[px] = make_float4(wox, woy, woz, dielectric ? hit.ior : 1.0f);
+        gb.p4[px] = make_float4(hit.er, hit.eg, hit.eb, 0.0f);
+        if (dielectric || pt.sc.nlights == 0 || !(pt.sc.light_total > 1e-12f)) return kPixelDone;
+        int Mc = P.restir_candidates;
+        if (Mc < 1) Mc = 1;
+        if (Mc > 32) Mc = 32;
+        float r2 = rough;
+        if (r2 < 0.02f) r2 = 0.02f;
+        float spec_prob = 0.05f + 0.95f * metallic;
+        if (spec_prob > 0.98f) spec_prob = 0.98f;
+        if (metallic >= 0.999f) spec_prob = 1.0f;
+        L.Mc = Mc;
+        L.j = 0;
+        L.p = hit.p;
+        L.pn = V3{pnx, pny, pnz};
+        L.wo = V3{wox, woy, woz};
+        L.alb = V3{hit.ar, hit.ag, hit.ab};
+        L.alpha2 = (r2 * r2) * (r2 * r2);
+        L.met = metallic;
+        L.spec = spec_prob;
+        L.sumWs = L.sumWu = L.sel_ph = 0.0f;
+        L.have = false;
+        return next_candidate(L);
+    }
+
+    __device__ int step(Lane& L, Hit h) {
+        if (!pt.sc.opaque && h.tri >= 0) {
+            const float opacity = pt.tri_alpha(h.tri);
+            if (!(opacity >= 1.0f || (opacity > 0.0f && pt.urand() < opacity))) {
+                L.traveled += h.t + 0.0001f;
+                if (L.traveled < L.seq_dist) {
+                    L.ro = L.base + L.rd * L.traveled;
+                    L.tmax = L.seq_dist - L.traveled;
+                    return kTrace;
+                }
+                h.tri = -1;
+            }
+        }
+        if (L.phase == kShadow) {
+            finish_candidate(L, L.c_wu, h.tri < 0 ? L.c_wu : 0.0f, L.c_ph, L.c_light, L.c_lu, L.c_lv);
+            return next_candidate(L);
+        }
+        if (h.tri < 0) {
+            const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const int px = L.pixel;
+            gb.p0[px] = zero; gb.p1[px] = zero; gb.p2[px] = zero; gb.p3[px] = zero; gb.p4[px] = zero;
+            return kPixelDone;
+        }
+        return primary_hit(L, pt.surface_at(h, L.ro, L.rd));
+    }
+    __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
+};
+
+// The persistent loop shared by the engines: every iteration traces one ray
+// per active lane, then advances each lane; idle lanes take new pixels from
+// `queue` (one atomic per wave) until the image is exhausted.
+template <int S, class E>
+__global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene sc, FrameParams P, PassIO io,
+                                                        unsigned int* __restrict__ queue, unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
-    Engine<S, SHADE> eng{pt, P, gb, rm, hdr, rgb};
+    E eng(pt, P, io);
     const int npx = P.width * P.height;
     const unsigned lane = __lane_id();
-    Lane L;
+    typename E::Lane L;
     L.phase = kIdle;
     bool exhausted = false;
     for (;;) {
@@ -472,8 +705,8 @@ __global__ __launch_bounds__(kBlock) void k_wave(DevScene sc, FrameParams P, GBu
                                                                     __builtin_amdgcn_mbcnt_lo(unsigned(need), 0u));
                     const unsigned px = base + rank;
                     if (px < unsigned(npx)) {
-                        L.phase = kExt;  // provisional; start_pixel sets the real phase
-                        if (eng.start_pixel(L, int(px)) != kTrace) L.phase = kIdle;
+                        L.phase = kExt;  // start() sets the real phase
+                        if (eng.start(L, int(px)) != kTrace) L.phase = kIdle;
                     }
                 }
                 if (base + unsigned(count) >= unsigned(npx)) exhausted = true;
@@ -484,51 +717,58 @@ __global__ __launch_bounds__(kBlock) void k_wave(DevScene sc, FrameParams P, GBu
             continue;
         }
         if (L.phase != kIdle) {
-            const Hit h = pt.intersect(L.ro, L.rd, L.tmax, L.phase == kShadow && sc.opaque);
-            int r = eng.on_trace(L, h);
-            if (r == kPathEnd) r = eng.next_sample(L);
-            if (r == kPixelDone) L.phase = kIdle;
+            const Hit h = pt.intersect(L.ro, L.rd, L.tmax, eng.any_hit(L));
+            if (eng.step(L, h) != kTrace) L.phase = kIdle;
         }
     }
-    pt.flush(cnt, SHADE ? kCntShade : kCntMega);
+    pt.flush(cnt, E::kPass);
+}
+
+template <int S>
+int resident_blocks(const void* kernel) {
+    int dev = 0, cus = 1, occ = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+            cus = prop.multiProcessorCount;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ < 1) occ = 1;
+    return cus * occ;
+}
+
+// Launches k_persistent<S, E> with one resident wave set (cached per instance).
+template <int S, class E>
+void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& io, unsigned* queue,
+                       unsigned long long* cnt, hipStream_t st) {
+    static int resident = 0;
+    const void* kernel = reinterpret_cast<const void*>(&k_persistent<S, E>);
+    if (resident == 0) resident = resident_blocks<S>(kernel);
+    const int npx = P.width * P.height;
+    const int need = (npx + kBlock - 1) / kBlock;
+    const dim3 grid(unsigned(need < resident ? need : resident)), block(kBlock);
+    hipLaunchKernelGGL((k_persistent<S, E>), grid, block, 0, st, sc, P, io, queue, cnt);
 }
 
 }  // namespace
 
-int wave_blocks(int npx, int stack) {
-    static int per_cu[2][2] = {{0, 0}, {0, 0}};
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-        if (cus <= 0) cus = 1;
-    }
-    const int si = stack <= 32 ? 0 : 1;
-    int& occ = per_cu[si][0];
-    if (occ == 0) {
-        int a = 0;
-        if (si == 0) hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wave<32, true>, kBlock, 0);
-        else hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wave<64, true>, kBlock, 0);
-        occ = a > 0 ? a : 1;
-    }
-    const int need = (npx + kBlock - 1) / kBlock;
-    return need < cus * occ ? need : cus * occ;
-}
-
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
                  const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, unsigned int* queue, unsigned long long* cnt,
                  hipStream_t st) {
-    const int npx = P.width * P.height;
-    const dim3 grid(unsigned(wave_blocks(npx, stack))), block(kBlock);
+    const PassIO io{gb, rm, hdr, rgb};
     if (stack <= 32) {
-        if (shade) hipLaunchKernelGGL((k_wave<32, true>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
-        else hipLaunchKernelGGL((k_wave<32, false>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+        if (shade) launch_persistent<32, PathEngine<32, true>>(sc, P, io, queue, cnt, st);
+        else launch_persistent<32, PathEngine<32, false>>(sc, P, io, queue, cnt, st);
     } else {
-        if (shade) hipLaunchKernelGGL((k_wave<64, true>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
-        else hipLaunchKernelGGL((k_wave<64, false>), grid, block, 0, st, sc, P, gb, rm, hdr, rgb, queue, cnt);
+        if (shade) launch_persistent<64, PathEngine<64, true>>(sc, P, io, queue, cnt, st);
+        else launch_persistent<64, PathEngine<64, false>>(sc, P, io, queue, cnt, st);
     }
+}
+
+void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
+                     unsigned int* queue, unsigned long long* cnt, hipStream_t st) {
+    const PassIO io{gb, ri, nullptr, nullptr};
+    if (stack <= 32) launch_persistent<32, GenEngine<32>>(sc, P, io, queue, cnt, st);
+    else launch_persistent<64, GenEngine<64>>(sc, P, io, queue, cnt, st);
 }
 
 }  // namespace rmh
