@@ -13,6 +13,15 @@ rm_render call including the 2.7 MB RGB8 readback.
 Prints one JSON line: Mrays/s (rays actually traced, counted on the GPU),
 ms/frame, the roofline of the dominant kernel measured with HIP events, and
 the CPU oracle timed on a bounded sample of the same workload.
+
+--workload heightfield measures the animated large-mesh case instead: a
+1024x1024 height field (2,093,058 triangles + 256 emissive ones) displaced by
+0.05 sin(2 pi t + x) every frame, 1920x1080, 8 spp, depth 6, ReSTIR defaults
+(8 candidates, 4 spatial neighbours, radius 16) and the denoiser. Every frame
+goes through the drop-in boundary: the host marshals the whole scene, the
+backend uploads what changed and refits the resident tree on the GPU (a
+rebuild from the host tree every 32nd frame), then renders. ms/frame is that boundary path;
+the scene program's own vertex update and FP64 host refit are reported beside.
 """
 from __future__ import annotations
 
@@ -59,27 +68,135 @@ def frame_settings(frame_index: int) -> rm.Frame:
     return f
 
 
-def cpu_baseline(exported) -> dict:
-    """The CPU oracle on a bounded sample: the full 1280x720 frame 0 of the same
-    scene and settings at 1 spp (instead of 16), all host threads up to 16."""
+def cpu_baseline(exported, f: rm.Frame, what: str) -> dict:
+    """The CPU oracle on a bounded sample: one full frame of the same scene and
+    settings at 1 spp instead of the GPU's count, all host threads up to 16."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_bindings as ob
     threads = min(16, os.cpu_count() or 1)
-    f = frame_settings(0)
+    gpu_spp = f.spp
     f.spp = 1
     t0 = time.perf_counter()
     _, _, info = ob.render_frame(exported, f, 12345, threads=threads)
     dt = time.perf_counter() - t0
     return {"value": round(info["rays"] / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle, full 1280x720 frame 0 at 1 spp (16 on the GPU), same scene and ReSTIR/denoise "
-                      f"settings: {info['rays']} rays in {dt:.2f} s"}
+            "sample": f"oracle, full {f.width}x{f.height} {what} at 1 spp ({gpu_spp} on the GPU), same scene and "
+                      f"ReSTIR/denoise settings: {info['rays']} rays in {dt:.2f} s"}
+
+
+def heightfield_frame(frame_index: int) -> rm.Frame:
+    f = rm.Frame()
+    f.width, f.height, f.spp, f.bounces = 1920, 1080, 8, 6
+    f.resolution, f.clamp_value = 1.0, 16.0
+    f.adapt_min, f.adapt_step, f.adapt_rel, f.adapt_abs = 16, 8, 0.02, 0.002
+    f.nomis = 0
+    f.restir_candidates, f.restir_spatial, f.restir_mcap, f.restir_radius = 8, 4, 128, 16.0
+    f.denoise, f.denoise_iterations, f.temporal = 1, 2, 1
+    f.camera = scenes.heightfield_camera(frame_index)
+    return f
+
+
+def run_heightfield(args) -> dict:
+    """Animated height field through the boundary; frame f is at t = f / 60 and
+    the host tree is rebuilt when f % 32 == 0 (Objects::render's policy)."""
+    scene = scenes.heightfield(1024, seed=0)
+    scene.build_bvh()
+    anim = scenes.HeightFieldAnimation(1024, seed=0)
+    backend = rm.Backend(0)
+    backend.set_scene(scene)
+    backend.set_profiling(True)
+
+    def one(f_index):
+        pos = anim.positions(f_index / 60.0)
+        t0 = time.perf_counter()
+        scene.set_positions(pos)
+        if f_index % 32 == 0:
+            scene.build_bvh()
+        else:
+            scene.refit_bvh()
+        t1 = time.perf_counter()
+        backend.set_scene(scene)  # rm_scene_upload: marshal to 176-B records, then rm_set_scene
+        t3 = time.perf_counter()
+        backend.render(heightfield_frame(f_index), 12345 + f_index, want_hdr=False)
+        t4 = time.perf_counter()
+        st = backend.stats()
+        return {"update": t1 - t0, "upload": t3 - t1, "render": t4 - t3,
+                "boundary": t4 - t1, "rays": st.rays[5], "refit": st.refit, "gpu_ms": st.ms[7],
+                "upload_ms": st.scene_ms[0], "refit_ms": st.scene_ms[1], "pass_ms": [st.ms[i] for i in range(8)],
+                "nodes": [st.node_visits[i] for i in range(6)], "tris": [st.tri_tests[i] for i in range(6)],
+                "prays": [st.rays[i] for i in range(6)], "bvh_nodes": st.bvh_nodes}
+
+    for k in range(1, 1 + args.warmup):
+        one(k)
+    recs = [one(k) for k in range(1 + args.warmup, 1 + args.warmup + args.steps)]
+    rebuild = one(32)
+    backend.close()
+    if any(r["refit"] != 1 for r in recs):
+        raise RuntimeError(f"expected GPU refits, got {[r['refit'] for r in recs]}")
+    mean = {k: float(np.mean([r[k] for r in recs])) for k in ("update", "upload", "render", "boundary",
+                                                               "rays", "gpu_ms", "upload_ms", "refit_ms")}
+    pm = np.mean(np.array([r["pass_ms"] for r in recs]), axis=0)
+    nd = np.mean(np.array([r["nodes"] for r in recs], np.float64), axis=0)
+    tr = np.mean(np.array([r["tris"] for r in recs], np.float64), axis=0)
+    ry = np.mean(np.array([r["prays"] for r in recs], np.float64), axis=0)
+    tracing = {1: "restir_generate", 2: "restir_spatial", 3: "restir_shade"}
+    dom = max(tracing, key=lambda i: pm[i])
+    name = tracing[dom]
+    px = 1920 * 1080
+    alg_bytes = nd[dom] * NODE_BYTES + tr[dom] * TRI_BYTES + px * PIXEL_BYTES[name]
+    achieved = alg_bytes / (pm[dom] / 1e3) / 1e9
+    ntri = scene.counts()[0]
+    nnodes = recs[0]["bvh_nodes"]
+    # refit: every triangle's 48 B read once, every node written and read once by its parent
+    refit_bytes = ntri * TRI_BYTES + 2 * nnodes * NODE_BYTES
+    ms_per_frame = mean["boundary"] * 1e3
+    amortized = (31 * ms_per_frame + rebuild["boundary"] * 1e3) / 32
+    line = {
+        "metric": METRIC,
+        "value": round(mean["rays"] / mean["boundary"] / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_frame, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural height field, seed 0)",
+        "config": {"workload": "animated 1024x1024 height field, 256 emissive triangles, refit every frame, "
+                               "rebuild every 32", "width": 1920, "height": 1080, "spp": 8, "max_depth": 6,
+                   "triangles": ntri, "restir": {"candidates": 8, "spatial": 4, "radius_px": 16, "temporal": True},
+                   "atrous_iterations": 2, "rays_per_frame": int(mean["rays"]),
+                   "host_ms": {"scene_update_and_fp64_refit": round(mean["update"] * 1e3, 1),
+                               "marshal_and_set_scene": round(mean["upload"] * 1e3, 1),
+                               "render_call": round(mean["render"] * 1e3, 1)},
+                   "gpu_ms": {"upload_copies": round(mean["upload_ms"], 3), "refit_kernels": round(mean["refit_ms"], 3),
+                              "frame_passes": round(mean["gpu_ms"], 3)},
+                   "gpu_mrays_per_s": round(mean["rays"] / (mean["gpu_ms"] / 1e3) / 1e6, 2),
+                   "refit_gbs": round(refit_bytes / (mean["refit_ms"] / 1e3) / 1e9, 1) if mean["refit_ms"] > 0 else None,
+                   "rebuild_frame_ms": round(rebuild["boundary"] * 1e3, 1),
+                   "rebuild_frame_gpu_tree": {0: "unchanged", 1: "refit (same topology)", 2: "rebuilt"}[rebuild["refit"]],
+                   "rebuild_host_tree_ms": round(rebuild["update"] * 1e3, 1),
+                   "amortized_ms_per_frame": round(amortized, 1),
+                   "pass_ms": {PASS_NAMES[i]: round(float(pm[i]), 3) for i in range(1, 7)},
+                   "pass_rays": {tracing[i]: int(ry[i]) for i in tracing},
+                   "pass_nodes_per_ray": {tracing[i]: round(float(nd[i] / max(ry[i], 1.0)), 2) for i in tracing}},
+        "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(alg_bytes), "kernel_ms": round(float(pm[dom]), 3)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(scene.export(), heightfield_frame(32), "heightfield frame 32")
+    return line
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None, help="timed frames (10; heightfield 8)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed frames first (2; heightfield 1)")
+    ap.add_argument("--workload", choices=("mixed", "heightfield"), default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (scales HBM counter readings)")
@@ -97,6 +214,17 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 3}))
         return 0
 
+    hf = args.workload == "heightfield"
+    if args.steps is None:
+        args.steps = 8 if hf else 10
+    if args.warmup is None:
+        args.warmup = 1 if hf else 2
+    line = run_heightfield(args) if hf else run_mixed(args)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_mixed(args) -> dict:
     scene = scenes.mixed_spheres(seed=0)
     scene.build_bvh()
     exported = scene.export()
@@ -163,9 +291,8 @@ def main() -> int:
                      "bytes_per_launch": int(alg_bytes), "kernel_ms": round(float(pm[dom]), 3)},
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(exported)
-    print(json.dumps(line), flush=True)
-    return 0
+        line["cpu_baseline"] = cpu_baseline(exported, frame_settings(0), "frame 0")
+    return line
 
 
 if __name__ == "__main__":

@@ -103,7 +103,7 @@ typedef struct {
     int32_t path;         /* 0 megakernel (render_paths), 1 ReSTIR + denoise */
     int32_t lanes;        /* megakernel lanes per pixel: 1, 8 or 32 */
     int32_t has_history;  /* temporal reuse was active */
-    int32_t refit;        /* last rm_set_scene: 0 unchanged, 1 refit, 2 rebuild */
+    int32_t refit;        /* last rm_set_scene's tree: 0 unchanged, 1 refit, 2 rebuild */
     /* Per pass: megakernel, generate, spatial, shade, probe, and the total. */
     uint64_t rays[6];        /* rays traced */
     uint64_t node_visits[6]; /* 4-wide BVH nodes fetched */
@@ -112,6 +112,7 @@ typedef struct {
                              temporal, atrous (all passes), tonemap, whole frame */
     int32_t bvh_nodes;    /* 4-wide nodes resident */
     int32_t stack_depth;  /* traversal stack entries the resident tree needs */
+    float scene_ms[2];    /* last rm_set_scene, GPU time: host-to-device copies, refit kernels */
 } rm_stats;
 
 typedef struct rm_ctx rm_ctx;
@@ -124,9 +125,11 @@ void rm_destroy(rm_ctx* ctx);
 const char* rm_last_error(const rm_ctx* ctx);
 
 /* Uploads a scene (metal.mm:109-186 marshalling + upload, 194-231 refit or
- * rebuild policy). Identical content is not re-uploaded. With the same
- * triangle count and BVH topology the resident 4-wide tree is refit on the GPU,
- * at most 31 times in a row; otherwise it is rebuilt from the given nodes. */
+ * rebuild policy). Identical content is not re-uploaded, and of a changed
+ * scene only the parts that changed are (geometry, materials, lights, tree).
+ * When vertex positions changed and the BVH topology is the same, the
+ * resident 4-wide tree is refit on the GPU, at most 31 times in a row;
+ * otherwise it is rebuilt from the given nodes. */
 int rm_set_scene(rm_ctx* ctx, const rm_scene_desc* scene);
 
 /* One frame (render_image_metal, metal.mm:95-414): rgb8 receives W*H*3 bytes,

@@ -35,7 +35,9 @@ int rm_scene_add_triangles(rm_scene* s, size_t n, const double* pos, const doubl
                            const double* mat, int32_t* index_out);
 /* Replaces the vertices (and, if given, normals) of the existing triangles,
  * recomputing the per-triangle precompute and the light CDF; triangles keep
- * their materials. Triangles that became degenerate are still kept. */
+ * their materials. Fails with RM_ERR_INVALID, leaving the scene unchanged,
+ * when n differs from the triangle count or a triangle would become
+ * degenerate (which would change the triangle list). */
 int rm_scene_set_positions(rm_scene* s, size_t n, const double* pos, const double* normals);
 
 int rm_scene_build_bvh(rm_scene* s, int max_leaf_tris);

@@ -79,6 +79,7 @@ class Stats(ctypes.Structure):
         ("refit", ctypes.c_int32),
         ("rays", ctypes.c_uint64 * 6), ("node_visits", ctypes.c_uint64 * 6), ("tri_tests", ctypes.c_uint64 * 6),
         ("ms", ctypes.c_float * 8), ("bvh_nodes", ctypes.c_int32), ("stack_depth", ctypes.c_int32),
+        ("scene_ms", ctypes.c_float * 2),
     ]
 
 

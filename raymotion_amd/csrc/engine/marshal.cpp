@@ -6,6 +6,8 @@
 #include <string>
 #include <unordered_map>
 
+#include "rm_parallel.h"
+
 namespace raymotion {
 namespace {
 
@@ -35,25 +37,40 @@ rm_scene_desc MarshalledScene::desc(const Scene& scene) const {
 }
 
 void marshal_scene(const Scene& scene, MarshalledScene& out) {
-    out.tris.resize(scene.tris.size());
+    const size_t n = scene.tris.size();
+    out.tris.resize(n);
     out.texels.clear();
+    // Field conversion in parallel chunks; each chunk lists its textured
+    // triangles, which then get atlas offsets in triangle order.
+    constexpr size_t kChunk = size_t(1) << 15;
+    const size_t chunks = (n + kChunk - 1) / kChunk;
+    std::vector<std::vector<uint32_t>> textured(chunks);
+    rmpar::parallel_chunks(chunks, [&](size_t c) {
+        const size_t a = c * kChunk, b = std::min(n, a + kChunk);
+        for (size_t i = a; i < b; ++i) {
+            const Triangle& s = scene.tris[i];
+            rm_triangle& t = out.tris[i];
+            t.v0 = pack(s.v0); t.v1 = pack(s.v1); t.v2 = pack(s.v2);
+            t.n = pack(s.n); t.nn = pack(s.nn);
+            t.n0 = pack(s.n0); t.n1 = pack(s.n1); t.n2 = pack(s.n2);
+            t.ar = float(s.ar); t.ag = float(s.ag); t.ab = float(s.ab);
+            t.er = float(s.er); t.eg = float(s.eg); t.eb = float(s.eb);
+            t.ior = float(s.ior); t.rough = float(s.rough); t.metallic = float(s.metallic);
+            t.alpha = float(s.alpha);
+            t.shader = s.shader;
+            t.tu0 = float(s.tu0); t.tv0 = float(s.tv0); t.tu1 = float(s.tu1);
+            t.tv1 = float(s.tv1); t.tu2 = float(s.tu2); t.tv2 = float(s.tv2);
+            t.texture_offset = -1;
+            t.texture_width = 0;
+            t.texture_height = 0;
+            if (s.texture) textured[c].push_back(uint32_t(i));
+        }
+    });
     std::unordered_map<const Texture*, int> atlas;
-    for (size_t i = 0; i < scene.tris.size(); ++i) {
-        const Triangle& s = scene.tris[i];
-        rm_triangle& t = out.tris[i];
-        t.v0 = pack(s.v0); t.v1 = pack(s.v1); t.v2 = pack(s.v2);
-        t.n = pack(s.n); t.nn = pack(s.nn);
-        t.n0 = pack(s.n0); t.n1 = pack(s.n1); t.n2 = pack(s.n2);
-        t.ar = float(s.ar); t.ag = float(s.ag); t.ab = float(s.ab);
-        t.er = float(s.er); t.eg = float(s.eg); t.eb = float(s.eb);
-        t.ior = float(s.ior); t.rough = float(s.rough); t.metallic = float(s.metallic); t.alpha = float(s.alpha);
-        t.shader = s.shader;
-        t.tu0 = float(s.tu0); t.tv0 = float(s.tv0); t.tu1 = float(s.tu1);
-        t.tv1 = float(s.tv1); t.tu2 = float(s.tu2); t.tv2 = float(s.tv2);
-        t.texture_offset = -1;
-        t.texture_width = 0;
-        t.texture_height = 0;
-        if (s.texture) {
+    for (const auto& list : textured)
+        for (uint32_t i : list) {
+            const Triangle& s = scene.tris[i];
+            rm_triangle& t = out.tris[i];
             auto ins = atlas.emplace(s.texture.get(), int(out.texels.size() / 3));
             if (ins.second)
                 for (const Vec3& p : s.texture->pixels) {
@@ -65,7 +82,6 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
             t.texture_width = s.texture->width;
             t.texture_height = s.texture->height;
         }
-    }
     // Preorder with escape links: a node is followed by its left subtree, then
     // its right subtree; `right` holds the index just past its own subtree.
     out.nodes.clear();

@@ -1,4 +1,5 @@
 // C-ABI over the host scene engine (include/rm_scene.h).
+#include <cmath>
 #include <cstring>
 
 #include "marshal.hpp"
@@ -50,8 +51,19 @@ int rm_scene_add_triangles(rm_scene* s, size_t n, const double* pos, const doubl
 int rm_scene_set_positions(rm_scene* s, size_t n, const double* pos, const double* normals) {
     if (!s || n != s->scene.tris.size() || (n > 0 && !pos)) return RM_ERR_INVALID;
     raymotion::Scene& sc = s->scene;
-    const std::vector<raymotion::Triangle> old = sc.tris;
+    // Reject the update before touching the scene if a triangle would become
+    // degenerate (the same test as Scene::add_triangle), so the count stays.
+    for (size_t i = 0; i < n; ++i) {
+        const double* p = pos + 9 * i;
+        const Vec3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
+        const Vec3 e1 = b - a, e2 = c - a;
+        const double g00 = dot(e1, e1), g01 = dot(e1, e2), g11 = dot(e2, e2);
+        if (std::abs(g00 * g11 - g01 * g01) <= 1e-12) return RM_ERR_INVALID;
+    }
+    std::vector<raymotion::Triangle> old;
+    old.swap(sc.tris);
     sc.clear_triangles();
+    sc.tris.reserve(n);
     for (size_t i = 0; i < n; ++i) {
         const raymotion::Triangle& t = old[i];
         const double* p = pos + 9 * i;
@@ -65,8 +77,7 @@ int rm_scene_set_positions(rm_scene* s, size_t n, const double* pos, const doubl
         }
         const int idx = sc.add_triangle(a, b, c, t.tu0, t.tv0, t.tu1, t.tv1, t.tu2, t.tv2, n0, n1, n2, t.ar, t.ag,
                                         t.ab, t.er, t.eg, t.eb, t.metallic, t.ior, t.rough, t.shader, t.alpha);
-        if (idx < 0) return RM_ERR_INVALID;  // the topology would change
-        sc.tris[size_t(idx)].texture = t.texture;
+        sc.tris[size_t(idx)].texture = t.texture;  // idx >= 0: checked above
     }
     return RM_OK;
 }

@@ -91,10 +91,13 @@ struct rm_ctx {
     std::vector<int32_t> level_start;  // refit launch ranges into level_nodes, deepest level first
     int32_t nnodes = 0, nlights = 0, opaque = 1, stack_need = 0, ntri = 0;
     float light_total = 0.0f;
-    uint64_t content = 0, hist_key = 0;
+    SceneKeys keys;  // content keys of the resident scene
+    PackedScene pack;  // host staging of the last upload
+    uint64_t hist_key = 0;
     std::vector<int32_t> prev_bvh_tri, prev_topo;
     int refits = 0;
     int last_refit = 0;
+    float scene_ms[2] = {0.0f, 0.0f};  // last rm_set_scene: copies, refit (profiling on)
 
     // Frame buffers.
     GBufSet gbuf, hist_gbuf;
@@ -259,68 +262,102 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     if (!c || !s) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
-    const uint64_t content = content_key(*s);
-    if (c->has_scene && content == c->content && size_t(c->ntri) == s->ntri) {
+    std::string perr;
+    if (!validate_scene(*s, false, false, false, perr)) return c->fail(RM_ERR_INVALID, perr);
+    const SceneKeys keys = scene_keys(*s);
+    const bool had = c->has_scene;
+    if (had && keys == c->keys) {
         c->last_refit = 0;
         c->stats.refit = 0;
         return RM_OK;  // identical scene already resident
     }
-    PackedScene ps;
-    std::string perr;
-    if (!pack_scene(*s, ps, perr)) return c->fail(RM_ERR_INVALID, perr);
-    if (ps.stack_need > kMaxStack)
-        return c->fail(RM_ERR_INVALID, "BVH too deep: the traversal stack would need " + std::to_string(ps.stack_need) +
-                                           " entries (at most " + std::to_string(kMaxStack) + ")");
-    // Refit in place when the topology is unchanged, at most 31 times in a row
-    // (metal.mm:211); otherwise upload the tree collapsed from the given nodes.
-    const bool refit = c->has_scene && c->refits < 31 && same_topology(*s, c->prev_bvh_tri, c->prev_topo);
-    c->has_scene = false;
-    bool ok = upload(c, c->geo, ps.geo) && upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn) &&
-              upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
-              upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf) && upload(c, c->texels, ps.texels);
-    if (ok && !refit) {
+    const bool same_count = had && keys.ntri == c->keys.ntri;
+    const bool topo_same = same_count && same_topology(*s, c->prev_bvh_tri, c->prev_topo);
+    const bool moved = !same_count || keys.positions != c->keys.positions;
+    const bool normals = !same_count || keys.normals != c->keys.normals;
+    const bool materials = !same_count || keys.material != c->keys.material || keys.ntexel != c->keys.ntexel;
+    // The tree follows the reference's acceleration-structure policy
+    // (metal.mm:194-231): when positions changed and the topology is the same,
+    // refit the resident 4-wide tree on the GPU, at most 31 times in a row;
+    // otherwise rebuild it from the caller's nodes.
+    const bool rebuild = !topo_same || (moved && c->refits >= 31);
+    const bool refit = !rebuild && moved;
+    const bool geometry = moved || normals || !topo_same;  // leaf order follows the tree
+    if (!validate_scene(*s, !topo_same, materials, true, perr)) return c->fail(RM_ERR_INVALID, perr);
+
+    PackedScene& ps = c->pack;  // reused: an animated frame re-packs into the same host buffers
+    if (rebuild) {
+        pack_tree(*s, ps);
+        if (ps.stack_need > kMaxStack)
+            return c->fail(RM_ERR_INVALID, "BVH too deep: the traversal stack would need " +
+                                               std::to_string(ps.stack_need) + " entries (at most " +
+                                               std::to_string(kMaxStack) + ")");
+    }
+    if (geometry) pack_geometry(*s, ps);
+    pack_lights(*s, ps);
+    if (materials) pack_materials(*s, ps);
+
+    c->has_scene = false;  // until every part is resident again
+    const bool prof = c->profiling;
+    if (prof) (void)hipEventRecord(c->ev[8], c->stream);
+    bool ok = upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
+    if (ok && geometry) ok = upload(c, c->geo, ps.geo) && upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn);
+    if (ok && materials)
+        ok = upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
+             upload(c, c->texels, ps.texels);
+    std::vector<int32_t> order;
+    if (ok && rebuild) {
         ok = upload(c, c->nodes, ps.nodes);
         // Nodes grouped by depth, deepest level first, for the refit launches.
         const int nn4 = int(ps.depth.size());
         int maxd = 0;
         for (int d : ps.depth) maxd = std::max(maxd, d);
-        std::vector<int32_t> order;
-        order.reserve(size_t(nn4));
-        c->level_start.assign(1, 0);
-        for (int d = maxd; d >= 0 && nn4 > 0; --d) {
-            for (int i = 0; i < nn4; ++i)
-                if (ps.depth[size_t(i)] == d) order.push_back(i);
-            c->level_start.push_back(int32_t(order.size()));
-        }
+        std::vector<int32_t> count(nn4 > 0 ? size_t(maxd) + 2 : 1, 0);
+        for (int d : ps.depth) ++count[size_t(maxd - d) + 1];
+        for (size_t l = 1; l < count.size(); ++l) count[l] += count[l - 1];
+        c->level_start.assign(count.begin(), count.end());
+        order.resize(size_t(nn4));
+        std::vector<int32_t> fill(count.begin(), count.end() - 1);
+        for (int i = 0; i < nn4; ++i) order[size_t(fill[size_t(maxd - ps.depth[size_t(i)])]++)] = i;
         ok = ok && upload(c, c->level_nodes, order);
-        c->refits = 0;
     }
     if (!ok) {
         c->drop_history();
         return c->fail(RM_ERR_ALLOC, "HIP scene buffer allocation failed");
     }
-    if (refit) {
+    if (prof) (void)hipEventRecord(c->ev[9], c->stream);
+    if (refit)
         for (size_t l = 0; l + 1 < c->level_start.size(); ++l)
             launch_refit_level(c->nodes.as<float4>(), c->geo.as<float4>(),
                                c->level_nodes.as<int32_t>() + c->level_start[l],
                                c->level_start[l + 1] - c->level_start[l], c->stream);
-        ++c->refits;
-    }
+    if (prof) (void)hipEventRecord(c->ev[10], c->stream);
     if (int rc = check(c, hipStreamSynchronize(c->stream), "scene upload")) return rc;
-    c->nnodes = int32_t(ps.depth.size());
+    c->scene_ms[0] = c->scene_ms[1] = 0.0f;
+    if (prof) {
+        if (hipEventElapsedTime(&c->scene_ms[0], c->ev[8], c->ev[9]) != hipSuccess) c->scene_ms[0] = 0.0f;
+        if (hipEventElapsedTime(&c->scene_ms[1], c->ev[9], c->ev[10]) != hipSuccess) c->scene_ms[1] = 0.0f;
+    }
+    if (rebuild) {
+        c->nnodes = int32_t(ps.depth.size());
+        c->stack_need = ps.stack_need;
+        c->refits = 0;
+        topology_of(*s, c->prev_bvh_tri, c->prev_topo);
+    }
+    if (refit) ++c->refits;
+    if (materials) c->opaque = ps.opaque;
     c->nlights = int32_t(s->nlight);
-    c->opaque = ps.opaque;
-    c->stack_need = ps.stack_need;
     c->light_total = ps.light_total;
     c->ntri = int32_t(s->ntri);
-    c->content = content;
-    c->hist_key = history_key(*s);
-    topology_of(*s, c->prev_bvh_tri, c->prev_topo);
+    c->keys = keys;
+    c->hist_key = history_key(keys, s->light_total);
     c->has_scene = true;
-    c->last_refit = refit ? 1 : 2;
+    c->last_refit = refit ? 1 : rebuild ? 2 : 0;
     c->stats.refit = c->last_refit;
     c->stats.bvh_nodes = c->nnodes;
     c->stats.stack_depth = c->stack_need;
+    c->stats.scene_ms[0] = c->scene_ms[0];
+    c->stats.scene_ms[1] = c->scene_ms[1];
     return RM_OK;
 }
 
@@ -370,6 +407,8 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
     st.refit = c->last_refit;
+    st.scene_ms[0] = c->scene_ms[0];
+    st.scene_ms[1] = c->scene_ms[1];
     st.has_history = P.has_history;
     const bool prof = c->profiling;
     auto mark = [&](int i) {

@@ -1,13 +1,14 @@
-// Scene packing: reference-layout arrays -> device layouts (see rm_layout.h).
+// Scene packing: reference-layout arrays -> device layouts (see rm_layout.h),
+// and the content keys that decide what a new scene re-uploads.
 #include "rm_pack.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <limits>
-#include <unordered_map>
 
 #include "rm_layout.h"
+#include "rm_parallel.h"
 
 namespace rmh {
 namespace {
@@ -100,53 +101,275 @@ void collapse(const BinTree& t, int b, int dst, int depth, int stack_here, Packe
     w[7] = {ibf(kind[0]), ibf(kind[1]), ibf(kind[2]), ibf(kind[3])};
 }
 
-struct MatKey {
-    uint32_t w[14];
-    bool operator==(const MatKey& o) const { return std::memcmp(w, o.w, sizeof(w)) == 0; }
-};
-struct MatHash {
-    size_t operator()(const MatKey& k) const { return size_t(hash_bytes(k.w, sizeof(k.w), 0x51ED27)); }
+using rmpar::parallel_chunks;
+
+constexpr size_t kChunk = size_t(1) << 15;  // triangles per parallel chunk
+constexpr uint64_t kMul = 0x100000001B3ULL;
+
+// Four independent multiply-xor lanes over 8-byte words, so consecutive words
+// do not wait for each other's multiply.
+struct Mix4 {
+    uint64_t h[4];
+    explicit Mix4(uint64_t seed) {
+        for (int i = 0; i < 4; ++i) h[i] = seed + 0x9E3779B97F4A7C15ULL * uint64_t(i + 1);
+    }
+    void words(const unsigned char* p, size_t nwords) {
+        for (size_t k = 0; k < nwords; ++k) {
+            uint64_t w;
+            std::memcpy(&w, p + 8 * k, 8);
+            const uint64_t x = (h[k & 3] ^ w) * kMul;
+            h[k & 3] = x ^ (x >> 29);
+        }
+    }
+    uint64_t done() const { return fold(fold(fold(h[0], h[1]), h[2]), h[3]); }
+    static uint64_t fold(uint64_t a, uint64_t b) {
+        const uint64_t x = (a ^ b) * 0xBF58476D1CE4E5B9ULL;
+        return x ^ (x >> 31);
+    }
 };
 
-}  // namespace
-
-uint64_t hash_bytes(const void* data, size_t n, uint64_t h) {
+// Hash of a byte array: 1 MiB pieces hashed in parallel, folded in order.
+uint64_t hash_span(const void* data, size_t n, uint64_t seed) {
     const unsigned char* p = static_cast<const unsigned char*>(data);
-    size_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        uint64_t w;
-        std::memcpy(&w, p + i, 8);
-        h ^= w;
-        h *= 0x100000001B3ULL;
-        h ^= h >> 29;
-    }
-    for (; i < n; ++i) {
-        h ^= p[i];
-        h *= 0x100000001B3ULL;
-    }
-    return h ^ (h >> 32);
-}
-
-uint64_t history_key(const rm_scene_desc& s) {
-    uint64_t h = uint64_t(s.ntri) * 0x9E3779B97F4A7C15ULL ^ uint64_t(s.nlight) * 0xBF58476D1CE4E5B9ULL;
-    uint32_t lt;
-    std::memcpy(&lt, &s.light_total, 4);
-    h ^= lt;
-    return hash_bytes(s.tris, s.ntri * sizeof(rm_triangle), h);
-}
-
-uint64_t content_key(const rm_scene_desc& s) {
-    uint64_t h = history_key(s);
-    h = hash_bytes(s.nodes, s.nnode * sizeof(rm_bvh_node), h ^ s.nnode);
-    h = hash_bytes(s.bvh_tri, s.ntri * sizeof(int32_t), h);
-    h = hash_bytes(s.light_tri, s.nlight * sizeof(int32_t), h);
-    h = hash_bytes(s.light_cdf, s.nlight * sizeof(float), h);
-    h = hash_bytes(s.texels, s.ntexel * 3 * sizeof(float), h ^ s.ntexel);
+    constexpr size_t kPiece = size_t(1) << 20;
+    const size_t pieces = (n + kPiece - 1) / kPiece;
+    std::vector<uint64_t> part(pieces);
+    parallel_chunks(pieces, [&](size_t c) {
+        const size_t a = c * kPiece, len = std::min(kPiece, n - a);
+        Mix4 m(seed ^ c);
+        m.words(p + a, len / 8);
+        uint64_t tail = 0;
+        std::memcpy(&tail, p + a + (len & ~size_t(7)), len & 7);
+        part[c] = Mix4::fold(m.done(), tail ^ len);
+    });
+    uint64_t h = seed ^ n;
+    for (uint64_t v : part) h = Mix4::fold(h, v);
     return h;
 }
 
+}  // namespace
+
+SceneKeys scene_keys(const rm_scene_desc& s) {
+    SceneKeys k;
+    k.ntri = s.ntri;
+    k.nnode = s.nnode;
+    k.nlight = s.nlight;
+    k.ntexel = s.ntexel;
+    static_assert(sizeof(rm_triangle) == 176, "triangle record");
+    const size_t chunks = (s.ntri + kChunk - 1) / kChunk;
+    std::vector<uint64_t> kp(chunks), kn(chunks), km(chunks);
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(s.tris);
+    parallel_chunks(chunks, [&](size_t c) {
+        Mix4 mp(0x5EED01 ^ c), mn(0x5EED02 ^ c), mm(0x5EED03 ^ c);
+        const size_t a = c * kChunk, b = std::min(s.ntri, a + kChunk);
+        for (size_t i = a; i < b; ++i) {
+            const unsigned char* r = base + i * sizeof(rm_triangle);
+            uint32_t split[2];  // bytes 32..35 (v2.z) and 36..39 (n.x)
+            std::memcpy(split, r + 32, 8);
+            const uint64_t w_pos = split[0], w_nrm = split[1];
+            mp.words(r, 4);  // bytes 0..31
+            mp.words(reinterpret_cast<const unsigned char*>(&w_pos), 1);
+            mn.words(reinterpret_cast<const unsigned char*>(&w_nrm), 1);
+            mn.words(r + 40, 7);   // bytes 40..95
+            mm.words(r + 96, 10);  // bytes 96..175
+        }
+        kp[c] = mp.done();
+        kn[c] = mn.done();
+        km[c] = mm.done();
+    });
+    uint64_t hp = 0x51ED0001 ^ s.ntri, hn = 0x51ED0002 ^ s.ntri, hm = 0x51ED0003 ^ s.ntri;
+    for (size_t c = 0; c < chunks; ++c) {
+        hp = Mix4::fold(hp, kp[c]);
+        hn = Mix4::fold(hn, kn[c]);
+        hm = Mix4::fold(hm, km[c]);
+    }
+    k.positions = hp;
+    k.normals = hn;
+    k.material = hash_span(s.texels, s.ntexel * 3 * sizeof(float), hm ^ s.ntexel);
+    k.tree = hash_span(s.bvh_tri, (s.nnode > 0 ? s.ntri : 0) * sizeof(int32_t),
+                       hash_span(s.nodes, s.nnode * sizeof(rm_bvh_node), 0x7EE ^ s.nnode));
+    uint32_t lt;
+    std::memcpy(&lt, &s.light_total, 4);
+    k.lights = hash_span(s.light_cdf, s.nlight * sizeof(float),
+                         hash_span(s.light_tri, s.nlight * sizeof(int32_t), 0x11647 ^ s.nlight) ^ lt);
+    return k;
+}
+
+uint64_t history_key(const SceneKeys& k, float light_total) {
+    uint32_t lt;
+    std::memcpy(&lt, &light_total, 4);
+    uint64_t h = k.ntri * 0x9E3779B97F4A7C15ULL ^ k.nlight * 0xBF58476D1CE4E5B9ULL ^ lt;
+    h = Mix4::fold(h, k.positions);
+    h = Mix4::fold(h, k.normals);
+    return Mix4::fold(h, k.material);
+}
+
+bool validate_scene(const rm_scene_desc& s, bool tree, bool materials, bool lights, std::string& err) {
+    if (s.ntri > 0 && !s.tris) { err = "scene has triangles but no triangle array"; return false; }
+    if (s.ntri >= (size_t(1) << 31) || s.nnode >= (size_t(1) << 31)) { err = "scene too large"; return false; }
+    if (s.nnode > 0 && (!s.nodes || !s.bvh_tri)) { err = "scene has BVH nodes but no node or index array"; return false; }
+    if (s.nlight > 0 && (!s.light_tri || !s.light_cdf)) { err = "scene has lights but no light arrays"; return false; }
+    if (s.ntexel > 0 && !s.texels) { err = "scene has texels but no texel array"; return false; }
+    if (tree) {
+        // Preorder escape links and leaf ranges.
+        for (size_t i = 0; i < s.nnode; ++i) {
+            const rm_bvh_node& n = s.nodes[i];
+            if (n.right <= int32_t(i) || size_t(n.right) > s.nnode) { err = "BVH escape link out of range"; return false; }
+            if (n.count < 0) { err = "negative BVH leaf count"; return false; }
+            if (n.count > 0) {
+                if (n.offset < 0 || size_t(n.offset) + size_t(n.count) > s.ntri) { err = "BVH leaf range out of range"; return false; }
+                if (n.right != int32_t(i) + 1) { err = "BVH leaf escape link is not the next node"; return false; }
+            } else if (i + 1 >= s.nnode || s.nodes[i + 1].right >= n.right) {
+                err = "BVH inner node without two children";
+                return false;
+            }
+        }
+        if (s.nnode > 0 && size_t(s.nodes[0].right) != s.nnode) { err = "BVH root does not span the node array"; return false; }
+        for (size_t k = 0; k < (s.nnode > 0 ? s.ntri : 0); ++k)
+            if (s.bvh_tri[k] < 0 || size_t(s.bvh_tri[k]) >= s.ntri) { err = "bvh_tri entry out of range"; return false; }
+    }
+    if (lights)
+        for (size_t k = 0; k < s.nlight; ++k)
+            if (s.light_tri[k] < 0 || size_t(s.light_tri[k]) >= s.ntri) { err = "light index out of range"; return false; }
+    if (materials)
+        for (size_t i = 0; i < s.ntri; ++i) {
+            const rm_triangle& t = s.tris[i];
+            if (t.texture_offset >= 0 &&
+                (t.texture_width <= 0 || t.texture_height <= 0 ||
+                 size_t(t.texture_offset) + size_t(t.texture_width) * size_t(t.texture_height) > s.ntexel)) {
+                err = "texture range out of range";
+                return false;
+            }
+        }
+    return true;
+}
+
+void pack_tree(const rm_scene_desc& s, PackedScene& out) {
+    out.nodes.clear();
+    out.depth.clear();
+    out.stack_need = 0;
+    if (s.nnode == 0) return;
+    BinTree bt{s.nodes, s.nnode};
+    out.nodes.reserve(s.nnode / 2 * kNodeWords);
+    out.nodes.resize(kNodeWords);
+    out.depth.push_back(0);
+    collapse(bt, 0, 0, 0, 0, out);
+}
+
+void pack_geometry(const rm_scene_desc& s, PackedScene& out) {
+    out.geo.resize(s.ntri * kTriGeoWords);
+    out.nrm.resize(s.ntri * kTriNrmWords);
+    out.nn.resize(s.ntri);
+    parallel_chunks((s.ntri + kChunk - 1) / kChunk, [&](size_t c) {
+        const size_t a = c * kChunk, b = std::min(s.ntri, a + kChunk);
+        for (size_t k = a; k < b; ++k) {  // vertices in leaf order for traversal
+            const int32_t j = s.nnode > 0 ? s.bvh_tri[k] : int32_t(k);
+            const rm_triangle& t = s.tris[j];
+            out.geo[3 * k] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
+            out.geo[3 * k + 1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
+            out.geo[3 * k + 2] = {t.v2.z, ibf(j), 0.0f, 0.0f};
+        }
+        for (size_t i = a; i < b; ++i) {  // shading normals in scene order
+            const rm_triangle& t = s.tris[i];
+            out.nrm[3 * i] = {t.n0.x, t.n0.y, t.n0.z, t.n.x};
+            out.nrm[3 * i + 1] = {t.n1.x, t.n1.y, t.n1.z, t.n.y};
+            out.nrm[3 * i + 2] = {t.n2.x, t.n2.y, t.n2.z, t.n.z};
+            out.nn[i] = {t.nn.x, t.nn.y, t.nn.z, 0.0f};
+        }
+    });
+}
+
+void pack_lights(const rm_scene_desc& s, PackedScene& out) {
+    out.lights.resize(s.nlight * kLightWords);
+    out.cdf.assign(s.light_cdf, s.light_cdf + s.nlight);
+    for (size_t k = 0; k < s.nlight; ++k) {  // CDF order
+        const rm_triangle& t = s.tris[s.light_tri[k]];
+        out.lights[4 * k] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
+        out.lights[4 * k + 1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
+        out.lights[4 * k + 2] = {t.v2.z, t.nn.x, t.nn.y, t.nn.z};
+        out.lights[4 * k + 3] = {t.er, t.eg, t.eb, t.alpha};
+    }
+    out.light_total = s.light_total;
+}
+
+void pack_materials(const rm_scene_desc& s, PackedScene& out) {
+    constexpr int kKeyWords = 14;
+    auto key_of = [](const rm_triangle& t, uint32_t* w) {
+        const int32_t texoff = t.texture_offset >= 0 ? t.texture_offset : -1;
+        const int32_t tw = texoff >= 0 ? t.texture_width : 0, th = texoff >= 0 ? t.texture_height : 0;
+        const float v[kKeyWords] = {t.ar, t.ag, t.ab, t.alpha, t.er, t.eg, t.eb, t.ior, t.rough, t.metallic,
+                                    ibf(t.shader), ibf(texoff), ibf(tw), ibf(th)};
+        std::memcpy(w, v, sizeof(v));
+    };
+    // Per-triangle key hashes in parallel, then ids by first occurrence through
+    // an open-addressing table (sequential, so ids do not depend on threads).
+    const size_t n = s.ntri;
+    std::vector<uint64_t> hk(n);
+    const size_t chunks = (n + kChunk - 1) / kChunk;
+    std::vector<uint8_t> chunk_opaque(chunks, 1), chunk_textured(chunks, 0);
+    parallel_chunks(chunks, [&](size_t c) {
+        const size_t a = c * kChunk, b = std::min(n, a + kChunk);
+        for (size_t i = a; i < b; ++i) {
+            uint32_t w[kKeyWords + 2] = {};
+            key_of(s.tris[i], w);
+            Mix4 m(0xA7);
+            m.words(reinterpret_cast<const unsigned char*>(w), (kKeyWords + 1) / 2);
+            hk[i] = m.done();
+            if (!(s.tris[i].alpha >= 1.0f)) chunk_opaque[c] = 0;
+            if (s.tris[i].texture_offset >= 0) chunk_textured[c] = 1;
+        }
+    });
+    size_t cap = 1024;
+    while (cap < 2 * n) cap <<= 1;
+    std::vector<int32_t> slot(cap, -1);
+    std::vector<uint32_t> keys;  // kKeyWords per material
+    out.matid.resize(n);
+    out.mats.clear();
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t w[kKeyWords];
+        key_of(s.tris[i], w);
+        size_t at = size_t(hk[i]) & (cap - 1);
+        int32_t id = -1;
+        while (slot[at] >= 0) {
+            if (std::memcmp(&keys[size_t(slot[at]) * kKeyWords], w, sizeof(w)) == 0) {
+                id = slot[at];
+                break;
+            }
+            at = (at + 1) & (cap - 1);
+        }
+        if (id < 0) {
+            id = int32_t(keys.size() / kKeyWords);
+            slot[at] = id;
+            keys.insert(keys.end(), w, w + kKeyWords);
+            float f[kKeyWords];
+            std::memcpy(f, w, sizeof(w));
+            out.mats.push_back({f[0], f[1], f[2], f[3]});
+            out.mats.push_back({f[4], f[5], f[6], f[7]});
+            out.mats.push_back({f[8], f[9], f[10], f[11]});
+            out.mats.push_back({f[12], f[13], 0.0f, 0.0f});
+        }
+        out.matid[i] = id;
+    }
+    out.opaque = 1;
+    bool textured = false;
+    for (size_t c = 0; c < chunks; ++c) {
+        if (!chunk_opaque[c]) out.opaque = 0;
+        if (chunk_textured[c]) textured = true;
+    }
+    out.uv.clear();
+    if (textured) {
+        out.uv.resize(n * 2);
+        for (size_t i = 0; i < n; ++i) {
+            const rm_triangle& t = s.tris[i];
+            out.uv[2 * i] = {t.tu0, t.tv0, t.tu1, t.tv1};
+            out.uv[2 * i + 1] = {t.tu2, t.tv2, 0.0f, 0.0f};
+        }
+    }
+    out.texels.assign(s.texels, s.texels + 3 * s.ntexel);
+}
+
 void topology_of(const rm_scene_desc& s, std::vector<int32_t>& bvh_tri, std::vector<int32_t>& topo) {
-    bvh_tri.assign(s.bvh_tri, s.bvh_tri + s.ntri);
+    bvh_tri.assign(s.bvh_tri, s.bvh_tri + (s.nnode > 0 ? s.ntri : 0));
     topo.resize(s.nnode * 3);
     for (size_t i = 0; i < s.nnode; ++i) {
         topo[3 * i] = s.nodes[i].right;
@@ -163,115 +386,6 @@ bool same_topology(const rm_scene_desc& s, const std::vector<int32_t>& prev_bvh_
         if (prev_topo[3 * i] != s.nodes[i].right || prev_topo[3 * i + 1] != s.nodes[i].offset ||
             prev_topo[3 * i + 2] != s.nodes[i].count)
             return false;
-    return true;
-}
-
-bool pack_scene(const rm_scene_desc& s, PackedScene& out, std::string& err) {
-    out = PackedScene{};
-    if (s.ntri > 0 && !s.tris) { err = "scene has triangles but no triangle array"; return false; }
-    if (s.ntri >= (size_t(1) << 31) || s.nnode >= (size_t(1) << 31)) { err = "scene too large"; return false; }
-    if (s.nnode > 0 && (!s.nodes || !s.bvh_tri)) { err = "scene has BVH nodes but no node or index array"; return false; }
-    if (s.nlight > 0 && (!s.light_tri || !s.light_cdf)) { err = "scene has lights but no light arrays"; return false; }
-    // Tree validation: preorder escape links and leaf ranges.
-    for (size_t i = 0; i < s.nnode; ++i) {
-        const rm_bvh_node& n = s.nodes[i];
-        if (n.right <= int32_t(i) || size_t(n.right) > s.nnode) { err = "BVH escape link out of range"; return false; }
-        if (n.count < 0) { err = "negative BVH leaf count"; return false; }
-        if (n.count > 0) {
-            if (n.offset < 0 || size_t(n.offset) + size_t(n.count) > s.ntri) { err = "BVH leaf range out of range"; return false; }
-            if (n.right != int32_t(i) + 1) { err = "BVH leaf escape link is not the next node"; return false; }
-        } else if (i + 1 >= s.nnode || s.nodes[i + 1].right >= n.right) {
-            err = "BVH inner node without two children";
-            return false;
-        }
-    }
-    if (s.nnode > 0 && size_t(s.nodes[0].right) != s.nnode) { err = "BVH root does not span the node array"; return false; }
-    for (size_t k = 0; k < (s.nnode > 0 ? s.ntri : 0); ++k)
-        if (s.bvh_tri[k] < 0 || size_t(s.bvh_tri[k]) >= s.ntri) { err = "bvh_tri entry out of range"; return false; }
-    for (size_t k = 0; k < s.nlight; ++k)
-        if (s.light_tri[k] < 0 || size_t(s.light_tri[k]) >= s.ntri) { err = "light index out of range"; return false; }
-    for (size_t i = 0; i < s.ntri; ++i) {
-        const rm_triangle& t = s.tris[i];
-        if (t.texture_offset >= 0) {
-            if (t.texture_width <= 0 || t.texture_height <= 0 ||
-                size_t(t.texture_offset) + size_t(t.texture_width) * size_t(t.texture_height) > s.ntexel) {
-                err = "texture range out of range";
-                return false;
-            }
-        }
-    }
-
-    // 4-wide tree.
-    if (s.nnode > 0) {
-        BinTree bt{s.nodes, s.nnode};
-        out.nodes.resize(kNodeWords);
-        out.depth.push_back(0);
-        collapse(bt, 0, 0, 0, 0, out);
-    }
-
-    // Triangles in leaf order for traversal (bvh_tri order).
-    out.geo.resize(s.ntri * kTriGeoWords);
-    for (size_t k = 0; k < s.ntri; ++k) {
-        const int32_t j = s.nnode > 0 ? s.bvh_tri[k] : int32_t(k);
-        const rm_triangle& t = s.tris[j];
-        out.geo[3 * k] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
-        out.geo[3 * k + 1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
-        out.geo[3 * k + 2] = {t.v2.z, ibf(j), 0.0f, 0.0f};
-    }
-    // Shading data in scene order.
-    out.nrm.resize(s.ntri * kTriNrmWords);
-    out.nn.resize(s.ntri);
-    out.matid.resize(s.ntri);
-    bool textured = false;
-    std::unordered_map<MatKey, int32_t, MatHash> mat_index;
-    for (size_t i = 0; i < s.ntri; ++i) {
-        const rm_triangle& t = s.tris[i];
-        out.nrm[3 * i] = {t.n0.x, t.n0.y, t.n0.z, t.n.x};
-        out.nrm[3 * i + 1] = {t.n1.x, t.n1.y, t.n1.z, t.n.y};
-        out.nrm[3 * i + 2] = {t.n2.x, t.n2.y, t.n2.z, t.n.z};
-        out.nn[i] = {t.nn.x, t.nn.y, t.nn.z, 0.0f};
-        if (!(t.alpha >= 1.0f)) out.opaque = 0;
-        const int32_t texoff = t.texture_offset >= 0 ? t.texture_offset : -1;
-        const int32_t tw = texoff >= 0 ? t.texture_width : 0, th = texoff >= 0 ? t.texture_height : 0;
-        if (texoff >= 0) textured = true;
-        const float vals[14] = {t.ar, t.ag, t.ab, t.alpha, t.er, t.eg, t.eb, t.ior, t.rough, t.metallic,
-                                ibf(t.shader), ibf(texoff), ibf(tw), ibf(th)};
-        MatKey key;
-        std::memcpy(key.w, vals, sizeof(vals));
-        auto it = mat_index.find(key);
-        int32_t id;
-        if (it == mat_index.end()) {
-            id = int32_t(out.mats.size() / kMatWords);
-            mat_index.emplace(key, id);
-            out.mats.push_back({vals[0], vals[1], vals[2], vals[3]});
-            out.mats.push_back({vals[4], vals[5], vals[6], vals[7]});
-            out.mats.push_back({vals[8], vals[9], vals[10], vals[11]});
-            out.mats.push_back({vals[12], vals[13], 0.0f, 0.0f});
-        } else {
-            id = it->second;
-        }
-        out.matid[i] = id;
-    }
-    if (textured) {
-        out.uv.resize(s.ntri * 2);
-        for (size_t i = 0; i < s.ntri; ++i) {
-            const rm_triangle& t = s.tris[i];
-            out.uv[2 * i] = {t.tu0, t.tv0, t.tu1, t.tv1};
-            out.uv[2 * i + 1] = {t.tu2, t.tv2, 0.0f, 0.0f};
-        }
-    }
-    // Lights in CDF order.
-    out.lights.resize(s.nlight * kLightWords);
-    out.cdf.assign(s.light_cdf, s.light_cdf + s.nlight);
-    for (size_t k = 0; k < s.nlight; ++k) {
-        const rm_triangle& t = s.tris[s.light_tri[k]];
-        out.lights[4 * k] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
-        out.lights[4 * k + 1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
-        out.lights[4 * k + 2] = {t.v2.z, t.nn.x, t.nn.y, t.nn.z};
-        out.lights[4 * k + 3] = {t.er, t.eg, t.eb, t.alpha};
-    }
-    out.light_total = s.light_total;
-    if (s.ntexel) out.texels.assign(s.texels, s.texels + 3 * s.ntexel);
     return true;
 }
 

@@ -1,6 +1,15 @@
 // Host-side conversion of a reference-layout scene (rm_scene_desc) into the
 // device layouts of rm_layout.h. Plain C++: no HIP calls, so it can be checked
 // on a machine without a GPU.
+//
+// A scene is packed in three independent parts so that a frame which only
+// moves vertices (the animated case) re-packs and re-uploads only geometry:
+//   tree      4-wide nodes collapsed from the caller's binary tree;
+//   geometry  leaf-ordered vertices and the normals (plus the small light
+//             table, re-packed with any change);
+//   materials deduplicated material table, per-triangle ids, uvs, texels.
+// Each part has a content key computed over fixed-size chunks in parallel
+// (deterministic for any thread count).
 #pragma once
 #include <cstdint>
 #include <string>
@@ -17,36 +26,52 @@ struct F4 {
 struct PackedScene {
     std::vector<F4> nodes;        // kNodeWords per 4-wide node, preorder (parents before children)
     std::vector<int32_t> depth;   // tree depth of each 4-wide node
+    int32_t stack_need = 0;       // deepest possible traversal stack
     std::vector<F4> geo;          // kTriGeoWords per triangle, leaf order
     std::vector<F4> nrm;          // kTriNrmWords per triangle, scene order
     std::vector<F4> nn;           // 1 per triangle
+    std::vector<F4> lights;       // kLightWords per light, CDF order
+    std::vector<float> cdf;
+    float light_total = 0.0f;
     std::vector<int32_t> matid;   // 1 per triangle
     std::vector<F4> mats;         // kMatWords per material
     std::vector<F4> uv;           // 2 per triangle when any triangle is textured, else empty
-    std::vector<F4> lights;       // kLightWords per light, CDF order
-    std::vector<float> cdf;
     std::vector<float> texels;
-    float light_total = 0.0f;
     int32_t opaque = 1;
-    int32_t stack_need = 0;       // deepest possible traversal stack
 };
 
-// Validates the arrays and fills `out`. Returns false with a message in `err`
-// when the scene is malformed (indices out of range, inconsistent tree).
-bool pack_scene(const rm_scene_desc& s, PackedScene& out, std::string& err);
+// Content keys. `positions` covers bytes 0..35 of every triangle record (the
+// vertices), `normals` bytes 36..95, `material` bytes 96..175 (everything else)
+// and the texels, `tree` the node array and bvh_tri, `lights` light_tri, the
+// CDF and the light total. Equal bytes give equal keys.
+struct SceneKeys {
+    uint64_t positions = 0, normals = 0, material = 0, tree = 0, lights = 0;
+    uint64_t ntri = 0, nnode = 0, nlight = 0, ntexel = 0;
+    bool operator==(const SceneKeys& o) const {
+        return positions == o.positions && normals == o.normals && material == o.material && tree == o.tree &&
+               lights == o.lights && ntri == o.ntri && nnode == o.nnode && nlight == o.nlight && ntexel == o.ntexel;
+    }
+};
+SceneKeys scene_keys(const rm_scene_desc& s);
+
+// Temporal-history key: the whole triangle records, the triangle and light
+// counts and the light total (the fields metal.mm:241-248 hashes).
+uint64_t history_key(const SceneKeys& k, float light_total);
+
+// Null arrays and counts (always), tree links and leaf ranges (`tree`),
+// texture ranges (`materials`), light indices (`lights`). Returns false with a
+// message in `err` when the scene is malformed.
+bool validate_scene(const rm_scene_desc& s, bool tree, bool materials, bool lights, std::string& err);
+
+void pack_tree(const rm_scene_desc& s, PackedScene& out);       // nodes, depth, stack_need
+void pack_geometry(const rm_scene_desc& s, PackedScene& out);   // geo (leaf order: needs the tree's bvh_tri), nrm, nn
+void pack_lights(const rm_scene_desc& s, PackedScene& out);     // lights, cdf, light_total
+void pack_materials(const rm_scene_desc& s, PackedScene& out);  // matid, mats, uv, texels, opaque
 
 // Same triangle count and identical tree topology (escape links, leaf ranges
 // and triangle order), i.e. the resident 4-wide tree can be refit in place.
-bool same_topology(const rm_scene_desc& a_desc, const std::vector<int32_t>& prev_bvh_tri,
+bool same_topology(const rm_scene_desc& s, const std::vector<int32_t>& prev_bvh_tri,
                    const std::vector<int32_t>& prev_topo);
 void topology_of(const rm_scene_desc& s, std::vector<int32_t>& bvh_tri, std::vector<int32_t>& topo);
-
-// 64-bit content hashes (8 bytes per step).
-uint64_t hash_bytes(const void* data, size_t n, uint64_t h);
-// Temporal-history key: triangle records, triangle and light counts and the
-// light total, the fields metal.mm:241-248 covers.
-uint64_t history_key(const rm_scene_desc& s);
-// Everything that is uploaded.
-uint64_t content_key(const rm_scene_desc& s);
 
 }  // namespace rmh

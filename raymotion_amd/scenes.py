@@ -192,13 +192,11 @@ def heightfield_positions(n: int = 1024, seed: int = 0, t: float = 0.0, extent: 
     return np.concatenate([t1, t2])
 
 
-def heightfield(n: int = 1024, seed: int = 0, t: float = 0.0, scene: Scene | None = None) -> Scene:
+def _heightfield_extras(n: int, seed: int):
+    """Per-triangle grid roughness, and the 256 emissive triangles with their
+    material rows (emission log-uniform over [1, 100])."""
     rng = np.random.default_rng(seed + 1)
-    s = scene or Scene()
-    grid = heightfield_positions(n, seed, t)
-    mat = material_rows(len(grid), albedo=(0.7, 0.7, 0.7), metallic=1.0)
-    mat[:, 8] = rng.uniform(0.1, 0.8, len(grid))
-    s.add_triangles(grid, mat)
+    rough = rng.uniform(0.1, 0.8, 2 * (n - 1) * (n - 1))
     k = 256
     centers = np.stack([rng.uniform(-7, 7, k), rng.uniform(3.0, 5.0, k), rng.uniform(-7, 7, k)], 1)
     size = 0.3
@@ -206,8 +204,36 @@ def heightfield(n: int = 1024, seed: int = 0, t: float = 0.0, scene: Scene | Non
     emis = np.exp(rng.uniform(np.log(1.0), np.log(100.0), (k, 3)))
     lm = material_rows(k, albedo=(0.8, 0.8, 0.8), rough=1.0)
     lm[:, 3:6] = emis
-    s.add_triangles(tri, lm)
+    return rough, tri, lm
+
+
+def heightfield(n: int = 1024, seed: int = 0, t: float = 0.0, scene: Scene | None = None) -> Scene:
+    """n x n height field, all GGX metal with roughness U[0.1, 0.8], lit by 256
+    emissive triangles; the animated displacement is at time t."""
+    s = scene or Scene()
+    grid = heightfield_positions(n, seed, t)
+    rough, light_tri, light_mat = _heightfield_extras(n, seed)
+    mat = material_rows(len(grid), albedo=(0.7, 0.7, 0.7), metallic=1.0)
+    mat[:, 8] = rough
+    s.add_triangles(grid, mat)
+    s.add_triangles(light_tri, light_mat)
     return s
+
+
+class HeightFieldAnimation:
+    """Triangle positions of heightfield(n, seed) at time t (grid, then the
+    static lights), with the noise evaluated once."""
+
+    def __init__(self, n: int = 1024, seed: int = 0):
+        self.base = heightfield_positions(n, seed, 0.0)
+        self.x = self.base[:, :, 0].copy()
+        self.h0 = self.base[:, :, 1] - 0.05 * np.sin(self.x)
+        self.lights = _heightfield_extras(n, seed)[1]
+
+    def positions(self, t: float) -> np.ndarray:
+        p = self.base.copy()
+        p[:, :, 1] = self.h0 + 0.05 * np.sin(2.0 * np.pi * t + self.x)
+        return np.concatenate([p, self.lights])
 
 
 def heightfield_camera(frame: int = 0) -> Camera:
