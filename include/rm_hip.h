@@ -149,6 +149,19 @@ int rm_trace(rm_ctx* ctx, const float* origins, const float* dirs, const float* 
  * before denoising (W*H*3). Any pointer may be NULL. */
 int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade);
 
+/* Path-replay adjoint of the path-tracing frame (the megakernel path of
+ * rm_render: no ReSTIR, legacy light sampling, no adaptive stop) with the
+ * same frame and seed: the samples rm_render averaged are re-traced from
+ * their seeds and grad receives, per parameter group, the gradient of
+ * sum_px d_hdr . hdr with respect to albedo r, g, b, roughness and emission
+ * scale (5 doubles per group). tri_group gives each triangle's group (scene
+ * order, -1 for none); a group's emission is scale * group_emission (3 floats
+ * per group). Sampling, MIS weights and roulette are held fixed. The
+ * differentiable counterpart of the reference's forward
+ * (pathtrace.metal:552-769); it has no reference counterpart. */
+int rm_render_adjoint(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, const float* d_hdr, const int32_t* tri_group,
+                      const float* group_emission, int ngroup, double* grad);
+
 /* Per-pass GPU timing with HIP events (off by default). */
 int rm_set_profiling(rm_ctx* ctx, int enable);
 int rm_get_stats(rm_ctx* ctx, rm_stats* out);

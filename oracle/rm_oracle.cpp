@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -112,7 +113,14 @@ template <class R> struct Tracer {
     Stats* stats = nullptr;
 
     struct Bsdf { R r = 0, g = 0, b = 0, pdf = 0; };
-    struct DielectricSample { bool valid = true; R n_x = 0, n_y = 0, n_z = 0, eta_scale = 1; };
+    struct DielectricSample { bool valid = true; R n_x = 0, n_y = 0, n_z = 0, eta_scale = 1, dlw_da2 = 0; };
+    // Derivatives of the opaque BSDF value: each channel by its own albedo
+    // channel (da) and by alpha2 (dl). Not part of the reference.
+    struct BsdfGrad { R da[3] = {0, 0, 0}, dl[3] = {0, 0, 0}; };
+    // Legacy NEE sample that contributed: light triangle, its emission, the
+    // BSDF value toward it, the factor k (contribution = le * e * k) and the
+    // direction. Filled for the adjoint only.
+    struct NeeOut { int tri = -1; R le[3] = {0, 0, 0}, e[3] = {0, 0, 0}; R k = 0; V3<R> dir; };
 
     R urand() { return R(rng_.uniform01()); }
 
@@ -285,6 +293,54 @@ template <class R> struct Tracer {
         return out;
     }
 
+    // d f / d albedo and d f / d alpha2 of evaluate_opaque_bsdf from the
+    // textbook derivatives: Schlick F' = m (1 - om), GGX
+    // D' = (dd - 2 alpha2 cos_h^2) / (pi dd^3), Smith G1' = -c (1 - c^2) / (s (c + s)^2),
+    // s = sqrt(c^2 + alpha2 (1 - c^2)). The pdf is not differentiated.
+    static BsdfGrad evaluate_opaque_bsdf_grad(R wi_x, R wi_y, R wi_z, R wo_x, R wo_y, R wo_z, R n_x, R n_y, R n_z,
+                                              R alpha2, R metallic, R ar, R ag, R ab) {
+        const R kpi = R(3.141592653589793);
+        BsdfGrad out;
+        const R cos_i = wi_x * n_x + (wi_y * n_y + wi_z * n_z);
+        const R cos_o = wo_x * n_x + (wo_y * n_y + wo_z * n_z);
+        if (!(cos_i > R(1e-8) && cos_o > R(1e-8))) return out;
+        R hx = wi_x + wo_x, hy = wi_y + wo_y, hz = wi_z + wo_z;
+        const R hl = std::sqrt(hx * hx + hy * hy + hz * hz);
+        if (!(hl > R(1e-8))) return out;
+        const R inv = R(1) / hl;
+        hx *= inv; hy *= inv; hz *= inv;
+        R cos_h = hx * n_x + (hy * n_y + hz * n_z);
+        R vo_h = wo_x * hx + (wo_y * hy + wo_z * hz);
+        if (cos_h < 0) { cos_h = -cos_h; vo_h = -vo_h; }
+        if (!(vo_h > R(1e-8))) return out;
+        const R dd = cos_h * cos_h * (alpha2 - 1) + 1;
+        const R D = alpha2 / (kpi * dd * dd);
+        const R so = std::sqrt(cos_o * cos_o + alpha2 * (1 - cos_o * cos_o));
+        const R si = std::sqrt(cos_i * cos_i + alpha2 * (1 - cos_i * cos_i));
+        const R g1 = 2 * cos_o / (cos_o + so);
+        const R g2 = 2 * cos_i / (cos_i + si);
+        const R G = g1 * g2;
+        R om = 1 - vo_h;
+        om *= om;
+        om *= om;
+        om *= 1 - vo_h;
+        const R di = 1 - metallic;
+        const R a[3] = {ar, ag, ab};
+        const R base = D * G / (4 * cos_i * cos_o);
+        const R dF = metallic * (1 - om);
+        const R dD = (dd - 2 * alpha2 * cos_h * cos_h) / (kpi * dd * dd * dd);
+        const R dg1 = -cos_o * (1 - cos_o * cos_o) / (so * (cos_o + so) * (cos_o + so));
+        const R dg2 = -cos_i * (1 - cos_i * cos_i) / (si * (cos_i + si) * (cos_i + si));
+        const R dbase = (dD * G + D * (dg1 * g2 + g1 * dg2)) / (4 * cos_i * cos_o);
+        for (int c = 0; c < 3; ++c) {
+            const R f0 = R(0.04) * di + a[c] * metallic;
+            const R F = f0 + (1 - f0) * om;
+            out.da[c] = (di * (1 - F) - a[c] * di * dF) / kpi + base * dF;
+            out.dl[c] = dbase * F;
+        }
+        return out;
+    }
+
     int pick_light(R pick) {  // binary search over the CDF (pathtrace.metal:323-330)
         int lo = 0, hi = P.lights - 1;
         while (lo < hi) {
@@ -301,7 +357,7 @@ template <class R> struct Tracer {
 
     void direct_lighting(R p_x, R p_y, R p_z, R n_x, R n_y, R n_z, R wo_x, R wo_y, R wo_z, R rough,
                          R metallic, R spec_prob, R ar, R ag, R ab, R& or_, R& og, R& ob, R& out_sumW,
-                         uint32_t& out_M) {
+                         uint32_t& out_M, NeeOut* no = nullptr) {
         or_ = og = ob = 0;
         out_sumW = 0;
         out_M = 0;
@@ -319,7 +375,8 @@ template <class R> struct Tracer {
         const R total = R(P.light_total);
         if (Mc <= 1) {  // legacy single-sample NEE (pathtrace.metal:319-384)
             const R pick = urand() * total;
-            const orc_triangle& t = sc->tris[pick_light(pick)];
+            const int light_tri = pick_light(pick);
+            const orc_triangle& t = sc->tris[light_tri];
             R lu = urand(), lv = urand();
             if (lu + lv > 1) { lu = 1 - lu; lv = 1 - lv; }
             const R lw = 1 - lu - lv;
@@ -360,6 +417,13 @@ template <class R> struct Tracer {
             or_ = R(t.er) * e.r * k;
             og = R(t.eg) * e.g * k;
             ob = R(t.eb) * e.b * k;
+            if (no) {
+                no->tri = light_tri;
+                no->le[0] = R(t.er); no->le[1] = R(t.eg); no->le[2] = R(t.eb);
+                no->e[0] = e.r; no->e[1] = e.g; no->e[2] = e.b;
+                no->k = k;
+                no->dir = {ldx, ldy, ldz};
+            }
             return;
         }
         // RIS over Mc candidates, one shadow ray (pathtrace.metal:385-469).
@@ -480,9 +544,14 @@ template <class R> struct Tracer {
             const R cos_h = std::fabs(hx * n_x + (hy * n_y + hz * n_z));
             const R vo_h = std::fabs(wo_x * hx + (wo_y * hy + wo_z * hz));
             if (cos_i > R(1e-8) && cos_o > R(1e-8) && cos_h > R(1e-8) && vo_h > R(1e-8)) {
-                const R g1 = 2 * cos_o / (cos_o + std::sqrt(cos_o * cos_o + alpha2 * (1 - cos_o * cos_o)));
-                const R g2 = 2 * cos_i / (cos_i + std::sqrt(cos_i * cos_i + alpha2 * (1 - cos_i * cos_i)));
+                const R so = std::sqrt(cos_o * cos_o + alpha2 * (1 - cos_o * cos_o));
+                const R si = std::sqrt(cos_i * cos_i + alpha2 * (1 - cos_i * cos_i));
+                const R g1 = 2 * cos_o / (cos_o + so);
+                const R g2 = 2 * cos_i / (cos_i + si);
                 R w = g1 * g2 * vo_h / (cos_o * cos_h);
+                if (!(w > 10))  // d log(G1 G2) / d alpha2, zero where the weight is clamped
+                    out.dlw_da2 = -(1 - cos_o * cos_o) / (2 * so * (cos_o + so)) -
+                                  (1 - cos_i * cos_i) / (2 * si * (cos_i + si));
                 if (w > 10) w = 10;
                 out.eta_scale *= w;
             } else {
@@ -545,7 +614,18 @@ template <class R> struct Tracer {
         return P.nomis ? R(0) : ((last_pdf + pl > R(1e-20)) ? last_pdf / (last_pdf + pl) : R(0));
     }
 
-    V3<R> pathtrace(V3<R> ro, V3<R> rd, int max_bounces) {  // pathtrace.metal:552-728
+    struct NoAdjoint {
+        template <class... T> void emission(const T&...) {}
+        template <class... T> void nee(const T&...) {}
+        template <class... T> void scatter(const T&...) {}
+        template <class... T> void dielectric(const T&...) {}
+    };
+    V3<R> pathtrace(V3<R> ro, V3<R> rd, int max_bounces) {
+        NoAdjoint none;
+        return pathtrace_t(ro, rd, max_bounces, none);
+    }
+    // pathtrace.metal:552-728; the hooks of `adj` see each term of the estimate.
+    template <class A> V3<R> pathtrace_t(V3<R> ro, V3<R> rd, int max_bounces, A& adj) {
         R ox = ro.x, oy = ro.y, oz = ro.z, dx = rd.x, dy = rd.y, dz = rd.z;
         R accr = 0, accg = 0, accb = 0, thr_r = 1, thr_g = 1, thr_b = 1;
         int bounce = 0, prev_delta = 1;
@@ -573,15 +653,18 @@ template <class R> struct Tracer {
             if (metallic > 1) metallic = 1;
             if (bounce == 0 || prev_delta == 1) {
                 accr += hit.er * thr_r; accg += hit.eg * thr_g; accb += hit.eb * thr_b;
+                adj.emission(hit, thr_r, thr_g, thr_b, R(1));
             } else {
                 const R w = emission_mis(hit, ox, oy, oz, last_pdf, prev_sumW, prev_M, prev_br, prev_bg, prev_bb,
                                          prev_cos);
                 accr += hit.er * thr_r * w; accg += hit.eg * thr_g * w; accb += hit.eb * thr_b * w;
+                adj.emission(hit, thr_r, thr_g, thr_b, w);
             }
             if (hit.ior > R(1.0001)) {
                 const DielectricSample ds =
                     sample_dielectric(wox, woy, woz, pnx, pny, pnz, dx, dy, dz, hit.ior, front, rough, alpha2);
                 if (!ds.valid) break;
+                adj.dielectric(hit, ds, accr, accg, accb);
                 thr_r *= ds.eta_scale; thr_g *= ds.eta_scale; thr_b *= ds.eta_scale;
                 R s = 1;
                 if (ds.n_x * pnx + (ds.n_y * pny + ds.n_z * pnz) < 0) s = -1;
@@ -599,9 +682,11 @@ template <class R> struct Tracer {
                 uint32_t cur_M = 0;
                 {
                     R dlr, dlg, dlb;
+                    NeeOut no;
                     direct_lighting(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rough, metallic,
-                                    spec_prob, hit.ar, hit.ag, hit.ab, dlr, dlg, dlb, cur_sumW, cur_M);
+                                    spec_prob, hit.ar, hit.ag, hit.ab, dlr, dlg, dlb, cur_sumW, cur_M, &no);
                     accr += dlr * thr_r; accg += dlg * thr_g; accb += dlb * thr_b;
+                    if (no.tri >= 0) adj.nee(hit, no, thr_r, thr_g, thr_b, wox, woy, woz, pnx, pny, pnz, alpha2, metallic);
                 }
                 R nx2, ny2, nz2;
                 if (urand() < spec_prob) {
@@ -624,6 +709,7 @@ template <class R> struct Tracer {
                 const Bsdf e = evaluate_opaque_bsdf(nx2, ny2, nz2, wox, woy, woz, pnx, pny, pnz, alpha2, metallic,
                                                     spec_prob, hit.ar, hit.ag, hit.ab);
                 if (!(e.pdf > R(1e-12))) break;
+                adj.scatter(hit, e, nx2, ny2, nz2, wox, woy, woz, pnx, pny, pnz, alpha2, metallic, accr, accg, accb);
                 last_pdf = e.pdf;
                 const R cos_i = nx2 * pnx + (ny2 * pny + nz2 * pnz);
                 thr_r *= e.r * cos_i / e.pdf;
@@ -787,6 +873,113 @@ template <class R> void render_paths_lanes(const Frame& f, uint32_t pixel, int L
     const V3<R> mean{sum.x / R(n), sum.y / R(n), sum.z / R(n)};
     out[3 * pixel] = tonemap8(mean.x); out[3 * pixel + 1] = tonemap8(mean.y); out[3 * pixel + 2] = tonemap8(mean.z);
     if (hdr) { hdr[3 * pixel] = mean.x; hdr[3 * pixel + 1] = mean.y; hdr[3 * pixel + 2] = mean.z; }
+}
+
+// Path-replay adjoint of render_paths (not part of the reference): detached
+// sampling (pdfs, MIS weights and roulette fixed), BSDF values and emission
+// differentiated; each sample is replayed from its saved RNG state after a
+// primal pass, and a bounce's scattering gets R * (df/dp) / f with R = C minus
+// the radiance accumulated before it.
+template <class R> struct AdjointHooks {
+    const orc_scene* sc;
+    const int32_t* tri_group;
+    const float* base;  // 3 per group
+    double* g;          // 5 per group
+    V3<R> C, dC;
+    static R drough(R r) { return (r > R(0.02) && r < R(1)) ? 4 * r * r * r : R(0); }
+    V3<R> albedo_factor(const HitInfo<R>& hit) const {
+        if (hit.shader == 1) return {0, 0, 0};
+        const orc_triangle& t = sc->tris[hit.tri];
+        return {t.ar != 0 ? hit.ar / R(t.ar) : R(0), t.ag != 0 ? hit.ag / R(t.ag) : R(0),
+                t.ab != 0 ? hit.ab / R(t.ab) : R(0)};
+    }
+    void add(int grp, int k, R v) { g[grp * 5 + k] += double(v); }
+    void emission(const HitInfo<R>& hit, R tr, R tg, R tb, R w) {
+        const int grp = tri_group[hit.tri];
+        if (grp < 0) return;
+        const float* b = base + 3 * grp;
+        add(grp, 4, (dC.x * tr * R(b[0]) + dC.y * tg * R(b[1]) + dC.z * tb * R(b[2])) * w);
+    }
+    template <class NO>
+    void nee(const HitInfo<R>& hit, const NO& no, R tr, R tg, R tb, R wox, R woy, R woz, R nx, R ny, R nz, R alpha2,
+             R metallic) {
+        const int gl = tri_group[no.tri];
+        if (gl >= 0) {
+            const float* b = base + 3 * gl;
+            add(gl, 4, (dC.x * tr * R(b[0]) * no.e[0] + dC.y * tg * R(b[1]) * no.e[1] + dC.z * tb * R(b[2]) * no.e[2]) *
+                           no.k);
+        }
+        const int grp = tri_group[hit.tri];
+        if (grp < 0) return;
+        const auto d = Tracer<R>::evaluate_opaque_bsdf_grad(no.dir.x, no.dir.y, no.dir.z, wox, woy, woz, nx, ny, nz,
+                                                            alpha2, metallic, hit.ar, hit.ag, hit.ab);
+        const R w[3] = {dC.x * tr * no.le[0] * no.k, dC.y * tg * no.le[1] * no.k, dC.z * tb * no.le[2] * no.k};
+        const V3<R> af = albedo_factor(hit);
+        add(grp, 0, w[0] * d.da[0] * af.x);
+        add(grp, 1, w[1] * d.da[1] * af.y);
+        add(grp, 2, w[2] * d.da[2] * af.z);
+        add(grp, 3, (w[0] * d.dl[0] + w[1] * d.dl[1] + w[2] * d.dl[2]) * drough(hit.rough));
+    }
+    template <class B>
+    void scatter(const HitInfo<R>& hit, const B& e, R ix, R iy, R iz, R wox, R woy, R woz, R nx, R ny, R nz, R alpha2,
+                 R metallic, R ar_, R ag_, R ab_) {
+        const int grp = tri_group[hit.tri];
+        if (grp < 0) return;
+        const R rr = dC.x * (C.x - ar_), rg = dC.y * (C.y - ag_), rb = dC.z * (C.z - ab_);
+        if (rr == 0 && rg == 0 && rb == 0) return;
+        const auto d = Tracer<R>::evaluate_opaque_bsdf_grad(ix, iy, iz, wox, woy, woz, nx, ny, nz, alpha2, metallic,
+                                                            hit.ar, hit.ag, hit.ab);
+        const R q[3] = {e.r != 0 ? rr / e.r : R(0), e.g != 0 ? rg / e.g : R(0), e.b != 0 ? rb / e.b : R(0)};
+        const V3<R> af = albedo_factor(hit);
+        add(grp, 0, q[0] * d.da[0] * af.x);
+        add(grp, 1, q[1] * d.da[1] * af.y);
+        add(grp, 2, q[2] * d.da[2] * af.z);
+        add(grp, 3, (q[0] * d.dl[0] + q[1] * d.dl[1] + q[2] * d.dl[2]) * drough(hit.rough));
+    }
+    template <class DS> void dielectric(const HitInfo<R>& hit, const DS& ds, R ar_, R ag_, R ab_) {
+        if (ds.dlw_da2 == 0) return;
+        const int grp = tri_group[hit.tri];
+        if (grp < 0) return;
+        add(grp, 3, (dC.x * (C.x - ar_) + dC.y * (C.y - ag_) + dC.z * (C.z - ab_)) * ds.dlw_da2 * drough(hit.rough));
+    }
+};
+
+// The samples render_paths (lanes == 1) or render_paths_lanes drew for one
+// pixel, replayed with the adjoint hooks; returns how many replays differed
+// from their primal value.
+template <class R>
+uint64_t adjoint_pixel(const Frame& f, uint32_t pixel, int lanes, const float* d_hdr, const int32_t* tri_group,
+                       const float* base, double* g) {
+    const orc_params& P = f.P;
+    const uint32_t x = pixel % uint32_t(P.width), y = pixel / uint32_t(P.width);
+    const R fn = R(P.spp);
+    const V3<R> dI{R(d_hdr[3 * pixel]) / fn, R(d_hdr[3 * pixel + 1]) / fn, R(d_hdr[3 * pixel + 2]) / fn};
+    const V3<R> cam{R(P.camera.x), R(P.camera.y), R(P.camera.z)};
+    uint64_t bad = 0;
+    for (int l = 0; l < lanes; ++l) {
+        Tracer<R> pt = make_tracer<R>(f, nullptr);
+        uint64_t sd = pixel_seed(f.seed, x, y);
+        if (lanes > 1) sd ^= uint64_t(l) * 0x94D049BB133111EBULL;
+        pt.rng_.seed(sd);
+        for (int k = l; k < P.spp; k += lanes) {
+            const Rng st0 = pt.rng_;
+            R sx, sy;
+            pt.jitter(x, y, sx, sy);
+            const V3<R> c = pt.sample_sensor(sx, sy, P.camera, P.bounces);
+            V3<R> d = dI;
+            if (P.clamp_value > 0) {
+                if (c.x > R(P.clamp_value)) d.x = 0;
+                if (c.y > R(P.clamp_value)) d.y = 0;
+                if (c.z > R(P.clamp_value)) d.z = 0;
+            }
+            pt.rng_ = st0;
+            pt.jitter(x, y, sx, sy);
+            AdjointHooks<R> hk{f.sc, tri_group, base, g, c, d};
+            const V3<R> c2 = pt.pathtrace_t(cam, pt.primary_dir(sx, sy, P.camera), P.bounces, hk);
+            if (!(c2.x == c.x && c2.y == c.y && c2.z == c.z)) ++bad;
+        }
+    }
+    return bad;
 }
 
 inline orc_vec3 fv(float x, float y, float z) { return {x, y, z}; }
@@ -1530,6 +1723,56 @@ int orc_render_reference_f64(const orc_scene* sc, const orc_frame* f, uint64_t s
     return 0;
 }
 
+// Adjoint of the path-tracing frame orc_render_frame draws for (f, seed): grad
+// (5 doubles per group: albedo r, g, b, roughness, emission scale) of
+// sum d_hdr . hdr. precision 32 follows the HIP kernel's arithmetic; 64
+// evaluates the same estimator in double. Returns -1 for a frame the adjoint
+// does not cover, else the number of samples whose replay differed.
+long long orc_render_adjoint(const orc_scene* sc, const orc_frame* f, uint64_t seed, const float* d_hdr,
+                             const int32_t* tri_group, const float* group_emission, int ngroup, int precision,
+                             int threads, double* grad) {
+    using namespace orc;
+    if (f->width <= 0 || f->height <= 0 || f->spp <= 0 || f->bounces < 0 || ngroup < 1) return -1;
+    const uint32_t px = uint32_t(f->width) * uint32_t(f->height);
+    orc_params P{};
+    P.camera = f->camera;
+    P.width = f->width; P.height = f->height; P.spp = f->spp; P.bounces = f->bounces;
+    P.nodes = int32_t(sc->nnode); P.lights = int32_t(sc->nlight);
+    P.adapt_min = f->adapt_min;
+    P.adapt_step = f->adapt_step > 0 ? f->adapt_step : 8;
+    P.nomis = f->nomis;
+    P.opaque = 1;
+    for (size_t i = 0; i < sc->ntri; ++i)
+        if (!(sc->tris[i].alpha >= 1)) { P.opaque = 0; break; }
+    P.light_total = sc->light_total;
+    P.resolution = f->resolution; P.clamp_value = f->clamp_value;
+    P.adapt_rel = f->adapt_rel; P.adapt_abs = f->adapt_abs;
+    P.restir_candidates = f->restir_candidates; P.restir_spatial = f->restir_spatial;
+    P.restir_mcap = f->restir_mcap; P.restir_radius = f->restir_radius;
+    const size_t restir_bytes = size_t(px) * (sizeof(orc_gbuffer) + 2 * sizeof(orc_reservoir));
+    const bool use_spatial = P.restir_spatial > 0 && P.lights > 0 && P.bounces > 0 && P.opaque &&
+                             restir_bytes <= size_t(512) * 1024 * 1024;
+    if (use_spatial || P.restir_candidates > 1 || (P.adapt_min > 0 && P.adapt_min <= P.spp)) return -1;
+    int lanes = 1;
+    if (f->spp >= 8 && (P.adapt_min <= 0 || P.adapt_step % 8 == 0)) lanes = 8;
+    if (f->spp >= 32 && (P.adapt_min <= 0 || P.adapt_step % 32 == 0)) lanes = 32;
+    Frame fr{sc, P, seed, nullptr};
+    std::mutex mu;
+    std::vector<double> total(size_t(ngroup) * 5, 0.0);
+    uint64_t bad = 0;
+    parallel_for(px, threads, [&](uint32_t i) {
+        std::vector<double> g(size_t(ngroup) * 5, 0.0);
+        const uint64_t b = precision == 64
+                               ? adjoint_pixel<double>(fr, i, lanes, d_hdr, tri_group, group_emission, g.data())
+                               : adjoint_pixel<float>(fr, i, lanes, d_hdr, tri_group, group_emission, g.data());
+        std::lock_guard<std::mutex> lock(mu);
+        for (size_t k = 0; k < g.size(); ++k) total[k] += g[k];
+        bad += b;
+    });
+    std::memcpy(grad, total.data(), total.size() * sizeof(double));
+    return (long long)bad;
+}
+
 // Closest-hit queries: tri/t/u/v per ray (tri = -1 on a miss). any != 0 runs
 // shadow (any-hit) queries instead: tri >= 0 means occluded. stats: [nodes, tris].
 int orc_trace(const orc_scene* sc, const float* o, const float* d, const float* tmax, size_t n, int any,
@@ -1615,6 +1858,19 @@ void orc_eval_bsdf(size_t n, const float* wi, const float* wo, const float* nrm,
             nrm[3 * i + 1], nrm[3 * i + 2], alpha2, metallic, spec_prob, albedo[0], albedo[1], albedo[2]);
         out[4 * i] = b.r; out[4 * i + 1] = b.g; out[4 * i + 2] = b.b; out[4 * i + 3] = b.pdf;
     }
+}
+
+// FP64 opaque BSDF value and its derivatives by albedo (own channel) and alpha2,
+// for the derivative pins.
+void orc_bsdf_grad_f64(const double* wi, const double* wo, const double* nrm, double alpha2, double metallic,
+                       const double* albedo, double* value, double* da, double* dl) {
+    using T = orc::Tracer<double>;
+    const T::Bsdf e = T::evaluate_opaque_bsdf(wi[0], wi[1], wi[2], wo[0], wo[1], wo[2], nrm[0], nrm[1], nrm[2], alpha2,
+                                              metallic, 0.5, albedo[0], albedo[1], albedo[2]);
+    const T::BsdfGrad g = T::evaluate_opaque_bsdf_grad(wi[0], wi[1], wi[2], wo[0], wo[1], wo[2], nrm[0], nrm[1], nrm[2],
+                                                       alpha2, metallic, albedo[0], albedo[1], albedo[2]);
+    value[0] = e.r; value[1] = e.g; value[2] = e.b;
+    for (int c = 0; c < 3; ++c) { da[c] = g.da[c]; dl[c] = g.dl[c]; }
 }
 
 float orc_fresnel(float ci, float eta_in, float eta_out) {

@@ -95,6 +95,7 @@ _SIGNATURES = [
     ("rm_render", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P]),
     ("rm_trace", _I, [_P, _P, _P, _P, _SZ, _I, _P, _P, _P, _P]),
     ("rm_read_restir", _I, [_P, _P, _P, _P, _P]),
+    ("rm_render_adjoint", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P, _P, _I, _P]),
     ("rm_set_profiling", _I, [_P, _I]),
     ("rm_get_stats", _I, [_P, ctypes.POINTER(Stats)]),
     ("rm_scene_create", _P, []),

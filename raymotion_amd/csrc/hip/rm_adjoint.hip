@@ -1,0 +1,158 @@
+// Path-replay adjoint of the path-tracing megakernel (render_paths,
+// pathtrace.metal:739-815 with pathtrace at 552-728): gradients of an image
+// loss with respect to per-material albedo, roughness and emission scale.
+//
+// Sampling is detached: pdfs, MIS weights and roulette probabilities are held
+// constant, and only the integrand's BSDF values and emission are
+// differentiated. Every sample is replayed from its saved RNG state (nothing
+// is stored per path): a primal pass gives the sample's radiance C, then a
+// second pass re-traces the same path, and at each vertex
+//   emission      dC/d scale   = thr * base_emission * w
+//   NEE           dC/d albedo  = thr * Le * k * df/d albedo      (at the light)
+//                 dC/d rough   = thr * Le * k * df/d alpha2 * 4 r^3
+//                 dC/d scale_l = thr * base_l * f * k
+//   scattering    dC/d p       = R * (df/dp) / f   with R = C - (radiance so far),
+//                                                   the part carried by this bounce
+//   rough glass   dC/d rough   = R . 1 * d log w / d alpha2 * 4 r^3
+// each dotted with dL/dC = dL/d(pixel) / n (zero for a channel the per-sample
+// clamp cut off).
+#include "rm_device.h"
+#include "rm_kernels.h"
+
+namespace rmh {
+namespace {
+
+// Bounce-loop hooks that accumulate gradients into the block's LDS table.
+struct AdjointHooks {
+    const DevScene& sc;
+    const AdjointArgs& A;
+    double* g;     // LDS: 5 per group
+    V3 C, dC;      // primal radiance of the sample and dL/dC
+
+    __device__ void add(int grp, int k, float v) {
+        if (v != 0.0f) atomicAdd(&g[grp * 5 + k], double(v));
+    }
+    // d alpha2 / d roughness for the material's raw roughness r
+    // (alpha2 = max(min(max(r, 0), 1), 0.02)^4).
+    static __device__ float drough(float r) { return (r > 0.02f && r < 1.0f) ? 4.0f * r * r * r : 0.0f; }
+    // d(hit albedo) / d(material albedo): the texel factor, 0 for the azimuth shader.
+    __device__ V3 albedo_factor(const Surface& hit) const {
+        if (hit.shader == 1) return {0.0f, 0.0f, 0.0f};
+        const float4 m = sc.mats[size_t(sc.matid[hit.tri]) * kMatWords];
+        return {m.x != 0.0f ? hit.ar / m.x : 0.0f, m.y != 0.0f ? hit.ag / m.y : 0.0f,
+                m.z != 0.0f ? hit.ab / m.z : 0.0f};
+    }
+
+    __device__ void emission(const Surface& hit, float tr, float tg, float tb, float w) {
+        const int grp = A.tri_group[hit.tri];
+        if (grp < 0) return;
+        const float* b = A.group_base + 3 * grp;
+        add(grp, 4, (dC.x * tr * b[0] + dC.y * tg * b[1] + dC.z * tb * b[2]) * w);
+    }
+
+    template <class NS>
+    __device__ void nee(const Surface& hit, const NS& ns, float tr, float tg, float tb, float wox, float woy, float woz,
+                        float nx, float ny, float nz, float alpha2, float metallic) {
+        if (ns.light < 0) return;
+        const int gl = A.light_group[ns.light];
+        if (gl >= 0) {
+            const float* b = A.group_base + 3 * gl;
+            add(gl, 4, (dC.x * tr * b[0] * ns.er + dC.y * tg * b[1] * ns.eg + dC.z * tb * b[2] * ns.eb) * ns.k);
+        }
+        const int grp = A.tri_group[hit.tri];
+        if (grp < 0) return;
+        const BsdfGrad d = Tracer<32>::eval_bsdf_grad(ns.d.x, ns.d.y, ns.d.z, wox, woy, woz, nx, ny, nz, alpha2,
+                                                      metallic, hit.ar, hit.ag, hit.ab);
+        const float wr = dC.x * tr * ns.ler * ns.k, wg = dC.y * tg * ns.leg * ns.k, wb = dC.z * tb * ns.leb * ns.k;
+        const V3 af = albedo_factor(hit);
+        add(grp, 0, wr * d.da_r * af.x);
+        add(grp, 1, wg * d.da_g * af.y);
+        add(grp, 2, wb * d.da_b * af.z);
+        add(grp, 3, (wr * d.dl_r + wg * d.dl_g + wb * d.dl_b) * drough(hit.rough));
+    }
+
+    __device__ void scatter(const Surface& hit, const Bsdf& e, float ix, float iy, float iz, float wox, float woy,
+                            float woz, float nx, float ny, float nz, float alpha2, float metallic, float ar_, float ag_,
+                            float ab_) {
+        const int grp = A.tri_group[hit.tri];
+        if (grp < 0) return;
+        const float rr = dC.x * (C.x - ar_), rg = dC.y * (C.y - ag_), rb = dC.z * (C.z - ab_);
+        if (rr == 0.0f && rg == 0.0f && rb == 0.0f) return;
+        const BsdfGrad d =
+            Tracer<32>::eval_bsdf_grad(ix, iy, iz, wox, woy, woz, nx, ny, nz, alpha2, metallic, hit.ar, hit.ag, hit.ab);
+        const float qr = e.r != 0.0f ? rr / e.r : 0.0f, qg = e.g != 0.0f ? rg / e.g : 0.0f,
+                    qb = e.b != 0.0f ? rb / e.b : 0.0f;
+        const V3 af = albedo_factor(hit);
+        add(grp, 0, qr * d.da_r * af.x);
+        add(grp, 1, qg * d.da_g * af.y);
+        add(grp, 2, qb * d.da_b * af.z);
+        add(grp, 3, (qr * d.dl_r + qg * d.dl_g + qb * d.dl_b) * drough(hit.rough));
+    }
+
+    __device__ void dielectric(const Surface& hit, const DielectricSample& ds, float ar_, float ag_, float ab_) {
+        if (ds.dlw_da2 == 0.0f) return;
+        const int grp = A.tri_group[hit.tri];
+        if (grp < 0) return;
+        const float rsum = dC.x * (C.x - ar_) + dC.y * (C.y - ag_) + dC.z * (C.z - ab_);
+        add(grp, 3, rsum * ds.dlw_da2 * drough(hit.rough));
+    }
+};
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_adjoint(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
+                                                     unsigned long long* cnt) {
+    __shared__ uint32_t stack[S * kBlock];
+    __shared__ double grad[kMaxGroups * 5];
+    for (int i = threadIdx.x; i < A.ngroup * 5; i += kBlock) grad[i] = 0.0;
+    __syncthreads();
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t pixel = gid / uint32_t(lanes), lane = gid % uint32_t(lanes);
+    const uint32_t npx = uint32_t(P.width) * uint32_t(P.height);
+    if (pixel < npx) {
+        const uint32_t x = pixel % uint32_t(P.width), y = pixel / uint32_t(P.width);
+        Tracer<S> pt(sc, P, stack + threadIdx.x);
+        uint64_t s = pixel_seed(P.seed, x, y);
+        if (lanes > 1) s ^= uint64_t(lane) * 0x94D049BB133111EBULL;  // render_paths_simd's lane streams
+        pt.rng.seed(s);
+        const float fn = float(P.spp);
+        const V3 dI{A.d_img[3 * pixel] / fn, A.d_img[3 * pixel + 1] / fn, A.d_img[3 * pixel + 2] / fn};
+        const V3 cam{P.cam.x, P.cam.y, P.cam.z};
+        uint32_t bad = 0;
+        for (int k = int(lane); k < P.spp; k += lanes) {
+            const uint32_t st0 = pt.rng.state;
+            float sx, sy;
+            pt.jitter(x, y, sx, sy);
+            const V3 c = pt.sample_sensor(sx, sy);
+            V3 d = dI;
+            if (P.clamp_value > 0.0f) {  // min(c, clamp) passes no gradient where it clamps
+                if (c.x > P.clamp_value) d.x = 0.0f;
+                if (c.y > P.clamp_value) d.y = 0.0f;
+                if (c.z > P.clamp_value) d.z = 0.0f;
+            }
+            pt.rng.state = st0;
+            pt.jitter(x, y, sx, sy);
+            AdjointHooks hk{sc, A, grad, c, d};
+            const V3 c2 = pt.pathtrace_t(cam, pt.primary_dir(sx, sy, false), P.bounces, hk);
+            if (__float_as_uint(c2.x) != __float_as_uint(c.x) || __float_as_uint(c2.y) != __float_as_uint(c.y) ||
+                __float_as_uint(c2.z) != __float_as_uint(c.z))
+                ++bad;
+        }
+        if (bad) atomicAdd(A.mismatch, (unsigned long long)bad);
+        pt.flush(cnt, kCntProbe);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < A.ngroup * 5; i += kBlock)
+        if (grad[i] != 0.0) atomicAdd(&A.grad[i], grad[i]);
+}
+
+}  // namespace
+
+void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams& P, const AdjointArgs& A,
+                    unsigned long long* cnt, hipStream_t st) {
+    const size_t threads = size_t(P.width) * size_t(P.height) * size_t(lanes);
+    const dim3 grid(unsigned((threads + kBlock - 1) / kBlock)), block(kBlock);
+    if (stack <= 32) hipLaunchKernelGGL((k_adjoint<32>), grid, block, 0, st, sc, P, A, lanes, cnt);
+    else hipLaunchKernelGGL((k_adjoint<64>), grid, block, 0, st, sc, P, A, lanes, cnt);
+}
+
+}  // namespace rmh

@@ -107,6 +107,8 @@ struct rm_ctx {
     DevBuf counters;
     DevBuf queue;  // work counters of the persistent passes: [0] generate, [1] shade / megakernel
     DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
+    DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
+    std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
 
     // Temporal history (metal.mm:36-40).
     rm_camera hist_cam{};
@@ -160,6 +162,37 @@ void collect(rm_stats& st, const unsigned long long* cnt) {
         st.node_visits[5] += st.node_visits[p];
         st.tri_tests[5] += st.tri_tests[p];
     }
+}
+
+FrameParams frame_params(const rm_frame& f, uint64_t seed) {
+    FrameParams P{};
+    P.cam = f.camera;
+    P.width = f.width; P.height = f.height; P.spp = f.spp; P.bounces = f.bounces;
+    P.adapt_min = f.adapt_min;
+    P.adapt_step = f.adapt_step > 0 ? f.adapt_step : 8;
+    P.nomis = f.nomis;
+    P.resolution = f.resolution; P.clamp_value = f.clamp_value;
+    P.adapt_rel = f.adapt_rel; P.adapt_abs = f.adapt_abs;
+    P.restir_candidates = f.restir_candidates; P.restir_spatial = f.restir_spatial;
+    P.restir_mcap = f.restir_mcap; P.restir_radius = f.restir_radius;
+    P.seed = seed;
+    return P;
+}
+
+// Branch choice (metal.mm:232-254), with the reference's record sizes.
+bool uses_restir(const rm_ctx* c, const FrameParams& P) {
+    const size_t px = size_t(P.width) * size_t(P.height);
+    const size_t restir_bytes = px * (sizeof(rm_gbuffer) + 2 * sizeof(rm_reservoir));
+    return P.restir_spatial > 0 && c->nlights > 0 && P.bounces > 0 && c->opaque &&
+           restir_bytes <= size_t(512) * 1024 * 1024;
+}
+
+// Megakernel lanes per pixel (metal.mm:390-394).
+int megakernel_lanes(const FrameParams& P) {
+    int lanes = 1;
+    if (P.spp >= 8 && (P.adapt_min <= 0 || P.adapt_step % 8 == 0)) lanes = 8;
+    if (P.spp >= 32 && (P.adapt_min <= 0 || P.adapt_step % 32 == 0)) lanes = 32;
+    return lanes;
 }
 
 template <class T>
@@ -224,7 +257,8 @@ void rm_destroy(rm_ctx* c) {
     DevBuf* bufs[] = {&c->nodes, &c->geo, &c->nrm, &c->nn, &c->matid, &c->mats, &c->uv, &c->lights, &c->cdf,
                       &c->texels, &c->level_nodes, &c->shade, &c->den_a, &c->hist_color, &c->rgb8, &c->hdr3,
                       &c->mega_hdr, &c->counters, &c->queue, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
-                      &c->probe_tuv};
+                      &c->probe_tuv, &c->adj_img, &c->adj_tri_group, &c->adj_light_group, &c->adj_base,
+                      &c->adj_grad, &c->adj_bad};
     for (DevBuf* b : bufs) b->release();
     c->gbuf.release();
     c->hist_gbuf.release();
@@ -347,6 +381,7 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     if (refit) ++c->refits;
     if (materials) c->opaque = ps.opaque;
     c->nlights = int32_t(s->nlight);
+    c->light_tri.assign(s->light_tri, s->light_tri + s->nlight);
     c->light_total = ps.light_total;
     c->ntri = int32_t(s->ntri);
     c->keys = keys;
@@ -372,23 +407,9 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     const size_t px = size_t(f->width) * size_t(f->height);
     const int npx = int(px);
 
-    FrameParams P{};
-    P.cam = f->camera;
-    P.width = f->width; P.height = f->height; P.spp = f->spp; P.bounces = f->bounces;
-    P.adapt_min = f->adapt_min;
-    P.adapt_step = f->adapt_step > 0 ? f->adapt_step : 8;
-    P.nomis = f->nomis;
-    P.resolution = f->resolution; P.clamp_value = f->clamp_value;
-    P.adapt_rel = f->adapt_rel; P.adapt_abs = f->adapt_abs;
-    P.restir_candidates = f->restir_candidates; P.restir_spatial = f->restir_spatial;
-    P.restir_mcap = f->restir_mcap; P.restir_radius = f->restir_radius;
-    P.seed = seed;
+    FrameParams P = frame_params(*f, seed);
     const DevScene sc = c->dev_scene();
-
-    // Branch choice (metal.mm:232-254), with the reference's record sizes.
-    const size_t restir_bytes = px * (sizeof(rm_gbuffer) + 2 * sizeof(rm_reservoir));
-    const bool use_spatial = P.restir_spatial > 0 && c->nlights > 0 && P.bounces > 0 && c->opaque &&
-                             restir_bytes <= size_t(512) * 1024 * 1024;
+    const bool use_spatial = uses_restir(c, P);
     bool have_history = use_spatial && c->hist_valid && c->hist_w == f->width && c->hist_h == f->height &&
                         seed != c->hist_seed && c->hist_key == c->hist_hash;
     if (!f->temporal) have_history = false;
@@ -486,9 +507,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         collect(st, host_cnt);
     } else {
         // Megakernel with 1, 8 or 32 lanes per pixel (metal.mm:390-394).
-        int lanes = 1;
-        if (f->spp >= 8 && (P.adapt_min <= 0 || P.adapt_step % 8 == 0)) lanes = 8;
-        if (f->spp >= 32 && (P.adapt_min <= 0 || P.adapt_step % 32 == 0)) lanes = 32;
+        const int lanes = megakernel_lanes(P);
         st.path = 0;
         st.lanes = lanes;
         if (hdr && !c->mega_hdr.ensure(px * sizeof(float4))) {
@@ -594,6 +613,83 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
     st.stack_depth = c->stack_need;
     if (c->profiling && hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]) != hipSuccess) st.ms[7] = 0.0f;
     c->stats = st;
+    return RM_OK;
+}
+
+int rm_render_adjoint(rm_ctx* c, const rm_frame* f, uint64_t seed, const float* d_hdr, const int32_t* tri_group,
+                      const float* group_emission, int ngroup, double* grad) {
+    if (!c || !f || !d_hdr || !tri_group || !group_emission || !grad) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (f->width <= 0 || f->height <= 0 || f->spp <= 0 || f->bounces < 0 ||
+        size_t(f->width) * size_t(f->height) > 100000000)
+        return c->fail(RM_ERR_INVALID, "Invalid render dimensions or sample count");
+    if (ngroup < 1 || ngroup > kMaxGroups)
+        return c->fail(RM_ERR_INVALID, "adjoint: ngroup must be in [1, " + std::to_string(kMaxGroups) + "]");
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    const FrameParams P = frame_params(*f, seed);
+    if (uses_restir(c, P))
+        return c->fail(RM_ERR_INVALID, "adjoint: the frame takes the ReSTIR path; differentiate the path-tracing "
+                                       "frame (restir_spatial = 0)");
+    if (P.restir_candidates > 1)
+        return c->fail(RM_ERR_INVALID, "adjoint: RIS light sampling is not differentiated (restir_candidates <= 1)");
+    if (P.adapt_min > 0 && P.adapt_min <= P.spp)
+        return c->fail(RM_ERR_INVALID, "adjoint: adaptive sampling must be off (adapt_min <= 0 or > spp)");
+    for (size_t i = 0; i < size_t(c->ntri); ++i)
+        if (tri_group[i] >= ngroup) return c->fail(RM_ERR_INVALID, "adjoint: triangle group out of range");
+    const size_t px = size_t(f->width) * size_t(f->height);
+    std::vector<int32_t> lg(c->light_tri.size());
+    for (size_t k = 0; k < lg.size(); ++k) lg[k] = tri_group[c->light_tri[k]];
+    const int lanes = megakernel_lanes(P);
+    hipStream_t st = c->stream;
+    if (!c->adj_img.ensure(px * 3 * sizeof(float)) || !c->adj_tri_group.ensure(size_t(c->ntri) * 4) ||
+        !c->adj_light_group.ensure(lg.size() * 4) || !c->adj_base.ensure(size_t(ngroup) * 3 * sizeof(float)) ||
+        !c->adj_grad.ensure(size_t(ngroup) * 5 * sizeof(double)) || !c->adj_bad.ensure(sizeof(unsigned long long)))
+        return c->fail(RM_ERR_ALLOC, "HIP adjoint buffer allocation failed");
+    unsigned long long* cnt = c->counters.as<unsigned long long>();
+    auto h2d = [&](DevBuf& b, const void* src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    if (!h2d(c->adj_img, d_hdr, px * 3 * sizeof(float)) || !h2d(c->adj_tri_group, tri_group, size_t(c->ntri) * 4) ||
+        !h2d(c->adj_light_group, lg.data(), lg.size() * 4) ||
+        !h2d(c->adj_base, group_emission, size_t(ngroup) * 3 * sizeof(float)) ||
+        hipMemsetAsync(c->adj_grad.ptr, 0, size_t(ngroup) * 5 * sizeof(double), st) != hipSuccess ||
+        hipMemsetAsync(c->adj_bad.ptr, 0, sizeof(unsigned long long), st) != hipSuccess ||
+        hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), st) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "adjoint: upload failed");
+    AdjointArgs A;
+    A.d_img = c->adj_img.as<float>();
+    A.tri_group = c->adj_tri_group.as<int32_t>();
+    A.light_group = c->adj_light_group.as<int32_t>();
+    A.group_base = c->adj_base.as<float>();
+    A.ngroup = ngroup;
+    A.grad = c->adj_grad.as<double>();
+    A.mismatch = c->adj_bad.as<unsigned long long>();
+    const DevScene sc = c->dev_scene();
+    if (c->profiling) (void)hipEventRecord(c->ev[0], st);
+    launch_adjoint(c->stack_need, lanes, sc, P, A, cnt, st);
+    if (c->profiling) (void)hipEventRecord(c->ev[1], st);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    unsigned long long host_cnt[kNumCounters], bad = 0;
+    if (int rc = check(c, hipMemcpyAsync(grad, A.grad, size_t(ngroup) * 5 * sizeof(double), hipMemcpyDeviceToHost, st),
+                       "readback"))
+        return rc;
+    if (int rc = check(c, hipMemcpyAsync(&bad, A.mismatch, sizeof(bad), hipMemcpyDeviceToHost, st), "readback"))
+        return rc;
+    if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, st), "counters"))
+        return rc;
+    if (int rc = check(c, hipStreamSynchronize(st), "adjoint")) return rc;
+    if (host_cnt[kCntOverflow]) return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+    if (bad) return c->fail(RM_ERR_DEVICE, "adjoint: " + std::to_string(bad) + " samples replayed differently");
+    rm_stats sts{};
+    collect(sts, host_cnt);
+    sts.path = 0;
+    sts.lanes = lanes;
+    sts.bvh_nodes = c->nnodes;
+    sts.stack_depth = c->stack_need;
+    if (c->profiling && hipEventElapsedTime(&sts.ms[7], c->ev[0], c->ev[1]) != hipSuccess) sts.ms[7] = 0.0f;
+    c->stats = sts;
+    c->err.clear();
     return RM_OK;
 }
 

@@ -77,6 +77,25 @@ struct Bsdf {
 struct DielectricSample {
     bool valid;
     float n_x, n_y, n_z, eta_scale;
+    float dlw_da2;  // d log(eta_scale) / d alpha2 (rough, unclamped weights; else 0)
+};
+
+// Derivatives of evaluate_opaque_bsdf's value: each channel by its own albedo
+// channel, and every channel by alpha2 (the pdf is not differentiated).
+struct BsdfGrad {
+    float da_r, da_g, da_b, dl_r, dl_g, dl_b;
+};
+
+// Hooks of the bounce loop that the forward leaves empty (pathtrace_t).
+struct NoAdjoint {
+    template <class... T>
+    __device__ __forceinline__ void emission(const T&...) {}
+    template <class... T>
+    __device__ __forceinline__ void nee(const T&...) {}
+    template <class... T>
+    __device__ __forceinline__ void scatter(const T&...) {}
+    template <class... T>
+    __device__ __forceinline__ void dielectric(const T&...) {}
 };
 
 // Per-lane tracer state. S is the capacity of the lane's LDS traversal stack;
@@ -354,6 +373,57 @@ struct Tracer {
     }
 
     // evaluate_opaque_bsdf (pathtrace.metal:243-295): Lambert + GGX/Smith/Schlick.
+    // Derivatives of eval_bsdf's value by albedo and alpha2, from the same
+    // intermediate quantities: Schlick F' = m (1 - om); GGX
+    // D' = (dd - 2 alpha2 cos_h^2) / (pi dd^3); Smith G1' = -c (1 - c^2) / (s (c + s)^2)
+    // with s = sqrt(c^2 + alpha2 (1 - c^2)).
+    static __device__ __forceinline__ BsdfGrad eval_bsdf_grad(float wi_x, float wi_y, float wi_z, float wo_x,
+                                                              float wo_y, float wo_z, float n_x, float n_y, float n_z,
+                                                              float alpha2, float metallic, float ar, float ag,
+                                                              float ab) {
+        const float kpi = 3.14159265358979323846f;
+        BsdfGrad out{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const float cos_i = wi_x * n_x + (wi_y * n_y + wi_z * n_z);
+        const float cos_o = wo_x * n_x + (wo_y * n_y + wo_z * n_z);
+        if (!(cos_i > 1e-8f && cos_o > 1e-8f)) return out;
+        float hx = wi_x + wo_x, hy = wi_y + wo_y, hz = wi_z + wo_z;
+        const float hl = sqrtf(hx * hx + hy * hy + hz * hz);
+        if (!(hl > 1e-8f)) return out;
+        const float inv = 1.0f / hl;
+        hx *= inv; hy *= inv; hz *= inv;
+        float cos_h = hx * n_x + (hy * n_y + hz * n_z);
+        float vo_h = wo_x * hx + (wo_y * hy + wo_z * hz);
+        if (cos_h < 0.0f) { cos_h = -cos_h; vo_h = -vo_h; }
+        if (!(vo_h > 1e-8f)) return out;
+        const float dd = cos_h * cos_h * (alpha2 - 1.0f) + 1.0f;
+        const float D = alpha2 / (kpi * dd * dd);
+        const float so = sqrtf(cos_o * cos_o + alpha2 * (1.0f - cos_o * cos_o));
+        const float si = sqrtf(cos_i * cos_i + alpha2 * (1.0f - cos_i * cos_i));
+        const float g1 = 2.0f * cos_o / (cos_o + so);
+        const float g2 = 2.0f * cos_i / (cos_i + si);
+        const float G = g1 * g2;
+        float om = 1.0f - vo_h;
+        om *= om;
+        om *= om;
+        om *= 1.0f - vo_h;
+        const float di = 1.0f - metallic;
+        const float f0r = 0.04f * di + ar * metallic, f0g = 0.04f * di + ag * metallic, f0b = 0.04f * di + ab * metallic;
+        const float fr = f0r + (1.0f - f0r) * om, fg = f0g + (1.0f - f0g) * om, fb = f0b + (1.0f - f0b) * om;
+        const float base = D * G / (4.0f * cos_i * cos_o);
+        const float dF = metallic * (1.0f - om);
+        out.da_r = (di * (1.0f - fr) - ar * di * dF) / kpi + base * dF;
+        out.da_g = (di * (1.0f - fg) - ag * di * dF) / kpi + base * dF;
+        out.da_b = (di * (1.0f - fb) - ab * di * dF) / kpi + base * dF;
+        const float dD = (dd - 2.0f * alpha2 * cos_h * cos_h) / (kpi * dd * dd * dd);
+        const float dg1 = -cos_o * (1.0f - cos_o * cos_o) / (so * (cos_o + so) * (cos_o + so));
+        const float dg2 = -cos_i * (1.0f - cos_i * cos_i) / (si * (cos_i + si) * (cos_i + si));
+        const float dbase = (dD * G + D * (dg1 * g2 + g1 * dg2)) / (4.0f * cos_i * cos_o);
+        out.dl_r = dbase * fr;
+        out.dl_g = dbase * fg;
+        out.dl_b = dbase * fb;
+        return out;
+    }
+
     static __device__ __forceinline__ Bsdf eval_bsdf(float wi_x, float wi_y, float wi_z, float wo_x, float wo_y, float wo_z, float n_x,
                                      float n_y, float n_z, float alpha2, float metallic, float spec_prob, float ar,
                                      float ag, float ab) {
@@ -414,6 +484,11 @@ struct Tracer {
         float dist;
         float r, g, b, sumW;
         uint32_t M;
+        // Legacy NEE with a contribution: light index (CDF order), its
+        // emission, the BSDF value toward it and the factor k with
+        // (r, g, b) = emission * bsdf * k. Read only by the adjoint.
+        int light;
+        float ler, leg, leb, er, eg, eb, k;
     };
     __device__ __forceinline__ NeeSample nee_prepare(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
                                      float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
@@ -424,6 +499,8 @@ struct Tracer {
         ns.dist = 0.0f;
         ns.r = ns.g = ns.b = ns.sumW = 0.0f;
         ns.M = 0;
+        ns.light = -1;
+        ns.ler = ns.leg = ns.leb = ns.er = ns.eg = ns.eb = ns.k = 0.0f;
         if (sc.nlights == 0 || !(sc.light_total > 1e-12f)) return ns;
         int Mc = P.restir_candidates;
         if (Mc < 1) Mc = 1;
@@ -438,7 +515,8 @@ struct Tracer {
         const V3 so{p_x + n_x * 0.002f, p_y + n_y * 0.002f, p_z + n_z * 0.002f};
         if (Mc <= 1) {
             const float pick = urand() * total;
-            const float4* L = sc.lights + size_t(pick_light(pick)) * kLightWords;
+            const int li = pick_light(pick);
+            const float4* L = sc.lights + size_t(li) * kLightWords;
             const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
             float lu = urand(), lv = urand();
             if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
@@ -474,6 +552,10 @@ struct Tracer {
                 ns.r = l3.x * e.r * k;
                 ns.g = l3.y * e.g * k;
                 ns.b = l3.z * e.b * k;
+                ns.light = li;
+                ns.ler = l3.x; ns.leg = l3.y; ns.leb = l3.z;
+                ns.er = e.r; ns.eg = e.g; ns.eb = e.b;
+                ns.k = k;
             }
             return ns;
         }
@@ -546,21 +628,6 @@ struct Tracer {
         return ns;
     }
 
-    // direct_lighting with its shadow ray traced in place.
-    __device__ void direct_lighting(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
-                                    float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
-                                    float ag, float ab, float& or_, float& og, float& ob, float& out_sumW,
-                                    uint32_t& out_M) {
-        const NeeSample ns = nee_prepare(p_x, p_y, p_z, n_x, n_y, n_z, wo_x, wo_y, wo_z, rough, metallic, spec_prob,
-                                         ar, ag, ab);
-        const bool lit = ns.shadow && !occluded(ns.o, ns.d, ns.dist);
-        or_ = lit ? ns.r : 0.0f;
-        og = lit ? ns.g : 0.0f;
-        ob = lit ? ns.b : 0.0f;
-        out_sumW = ns.sumW;
-        out_M = ns.M;
-    }
-
     static __device__ float fresnel(float ci, float eta_in, float eta_out) {  // pathtrace.metal:80-91
         if (ci < 0.0f) ci = 0.0f;
         if (ci > 1.0f) ci = 1.0f;
@@ -577,7 +644,7 @@ struct Tracer {
     __device__ DielectricSample sample_dielectric(float wo_x, float wo_y, float wo_z, float n_x, float n_y, float n_z,
                                                   float cdx, float cdy, float cdz, float ior, bool front, float rough,
                                                   float alpha2) {
-        DielectricSample out{true, 0.0f, 0.0f, 0.0f, 1.0f};
+        DielectricSample out{true, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f};
         float eta_i = 1.0f, eta_t = ior;
         if (!front) { eta_i = ior; eta_t = 1.0f; }
         float hx, hy, hz;
@@ -616,9 +683,15 @@ struct Tracer {
             const float cos_h = fabsf(hx * n_x + (hy * n_y + hz * n_z));
             const float vo_h = fabsf(wo_x * hx + (wo_y * hy + wo_z * hz));
             if (cos_i > 1e-8f && cos_o > 1e-8f && cos_h > 1e-8f && vo_h > 1e-8f) {
-                const float g1 = 2.0f * cos_o / (cos_o + sqrtf(cos_o * cos_o + alpha2 * (1.0f - cos_o * cos_o)));
-                const float g2 = 2.0f * cos_i / (cos_i + sqrtf(cos_i * cos_i + alpha2 * (1.0f - cos_i * cos_i)));
+                const float so = sqrtf(cos_o * cos_o + alpha2 * (1.0f - cos_o * cos_o));
+                const float si = sqrtf(cos_i * cos_i + alpha2 * (1.0f - cos_i * cos_i));
+                const float g1 = 2.0f * cos_o / (cos_o + so);
+                const float g2 = 2.0f * cos_i / (cos_i + si);
                 float w = g1 * g2 * vo_h / (cos_o * cos_h);
+                // (log G1)' = -(1 - c^2) / (2 s (c + s)); zero where the weight is clamped
+                if (!(w > 10.0f))
+                    out.dlw_da2 = -(1.0f - cos_o * cos_o) / (2.0f * so * (cos_o + so)) -
+                                  (1.0f - cos_i * cos_i) / (2.0f * si * (cos_i + si));
                 if (w > 10.0f) w = 10.0f;
                 out.eta_scale *= w;
             } else {
@@ -665,8 +738,14 @@ struct Tracer {
         return P.nomis ? 0.0f : ((last_pdf + pl > 1e-20f) ? last_pdf / (last_pdf + pl) : 0.0f);
     }
 
-    // pathtrace (pathtrace.metal:552-728)
+    // pathtrace (pathtrace.metal:552-728). The hooks of `adj` see every term
+    // of the estimate as it is added (the adjoint replays paths through them).
     __device__ V3 pathtrace(V3 ro, V3 rd, int max_bounces) {
+        NoAdjoint none;
+        return pathtrace_t(ro, rd, max_bounces, none);
+    }
+    template <class A>
+    __device__ __forceinline__ V3 pathtrace_t(V3 ro, V3 rd, int max_bounces, A& adj) {
         float ox = ro.x, oy = ro.y, oz = ro.z, dx = rd.x, dy = rd.y, dz = rd.z;
         float accr = 0.0f, accg = 0.0f, accb = 0.0f, thr_r = 1.0f, thr_g = 1.0f, thr_b = 1.0f;
         int bounce = 0, prev_delta = 1;
@@ -694,15 +773,18 @@ struct Tracer {
             if (metallic > 1.0f) metallic = 1.0f;
             if (bounce == 0 || prev_delta == 1) {
                 accr += hit.er * thr_r; accg += hit.eg * thr_g; accb += hit.eb * thr_b;
+                adj.emission(hit, thr_r, thr_g, thr_b, 1.0f);
             } else {
                 const float w = emission_mis(hit, ox, oy, oz, last_pdf, prev_sumW, prev_M, prev_br, prev_bg, prev_bb,
                                              prev_cos);
                 accr += hit.er * thr_r * w; accg += hit.eg * thr_g * w; accb += hit.eb * thr_b * w;
+                adj.emission(hit, thr_r, thr_g, thr_b, w);
             }
             if (hit.ior > 1.0001f) {
                 const DielectricSample ds =
                     sample_dielectric(wox, woy, woz, pnx, pny, pnz, dx, dy, dz, hit.ior, front, rough, alpha2);
                 if (!ds.valid) break;
+                adj.dielectric(hit, ds, accr, accg, accb);
                 thr_r *= ds.eta_scale; thr_g *= ds.eta_scale; thr_b *= ds.eta_scale;
                 float s = 1.0f;
                 if (ds.n_x * pnx + (ds.n_y * pny + ds.n_z * pnz) < 0.0f) s = -1.0f;
@@ -718,11 +800,15 @@ struct Tracer {
                 if (metallic >= 0.999f) spec_prob = 1.0f;
                 float cur_sumW = 0.0f;
                 uint32_t cur_M = 0;
-                {
-                    float dlr, dlg, dlb;
-                    direct_lighting(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rough, metallic,
-                                    spec_prob, hit.ar, hit.ag, hit.ab, dlr, dlg, dlb, cur_sumW, cur_M);
+                {  // direct_lighting
+                    const NeeSample ns = nee_prepare(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rough,
+                                                     metallic, spec_prob, hit.ar, hit.ag, hit.ab);
+                    const bool lit = ns.shadow && !occluded(ns.o, ns.d, ns.dist);
+                    cur_sumW = ns.sumW;
+                    cur_M = ns.M;
+                    const float dlr = lit ? ns.r : 0.0f, dlg = lit ? ns.g : 0.0f, dlb = lit ? ns.b : 0.0f;
                     accr += dlr * thr_r; accg += dlg * thr_g; accb += dlb * thr_b;
+                    if (lit) adj.nee(hit, ns, thr_r, thr_g, thr_b, wox, woy, woz, pnx, pny, pnz, alpha2, metallic);
                 }
                 float nx2, ny2, nz2;
                 if (urand() < spec_prob) {
@@ -745,6 +831,7 @@ struct Tracer {
                 const Bsdf e = eval_bsdf(nx2, ny2, nz2, wox, woy, woz, pnx, pny, pnz, alpha2, metallic, spec_prob,
                                          hit.ar, hit.ag, hit.ab);
                 if (!(e.pdf > 1e-12f)) break;
+                adj.scatter(hit, e, nx2, ny2, nz2, wox, woy, woz, pnx, pny, pnz, alpha2, metallic, accr, accg, accb);
                 last_pdf = e.pdf;
                 const float cos_i = nx2 * pnx + (ny2 * pny + nz2 * pnz);
                 thr_r *= e.r * cos_i / e.pdf;

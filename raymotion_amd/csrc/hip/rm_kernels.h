@@ -31,6 +31,21 @@ void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in,
 void launch_tonemap(int npx, const float4* in, uint8_t* rgb, float* hdr3, hipStream_t st);
 void launch_trace(int stack, const DevScene& sc, const FrameParams& P, int n, const float* o, const float* d,
                   const float* tmax, int any, int32_t* tri, float4* tuv, unsigned long long* cnt, hipStream_t st);
+// Path-replay adjoint of render_paths (1, 8 or 32 lanes per pixel, no adaptive
+// stop): gradients of sum(d_img * image) with respect to per-group material
+// parameters, accumulated into grad (5 doubles per group, zeroed by the caller).
+constexpr int kMaxGroups = 128;
+struct AdjointArgs {
+    const float* d_img;          // 3 per pixel: dL/d(pixel radiance)
+    const int32_t* tri_group;    // per triangle in scene order; -1: no parameters
+    const int32_t* light_group;  // per light in CDF order
+    const float* group_base;     // 3 per group: emission = scale * base
+    int ngroup;
+    double* grad;                // albedo r, g, b, roughness, emission scale per group
+    unsigned long long* mismatch;  // samples whose replay differed from their primal value
+};
+void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams& P, const AdjointArgs& A,
+                    unsigned long long* cnt, hipStream_t st);
 // Bottom-up refit of the nodes listed in `order` (one tree level, deepest first).
 void launch_refit_level(float4* nodes, const float4* geo, const int32_t* level_nodes, int count, hipStream_t st);
 

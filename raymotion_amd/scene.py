@@ -193,6 +193,19 @@ class Backend:
                                    ptr(v)), "rm_trace")
         return tri, t, u, v
 
+    def render_adjoint(self, frame: Frame, seed: int, d_hdr, tri_group, group_emission) -> np.ndarray:
+        """Gradients (ngroup, 5) of sum(d_hdr * hdr) for the frame render(frame, seed) draws:
+        albedo r, g, b, roughness and emission scale per group."""
+        d = np.ascontiguousarray(d_hdr, np.float32).reshape(-1)
+        if d.size != frame.width * frame.height * 3:
+            raise ValueError("d_hdr must hold width * height * 3 values")
+        tg = np.ascontiguousarray(tri_group, np.int32)
+        ge = np.ascontiguousarray(group_emission, np.float32).reshape(-1, 3)
+        grad = np.zeros((len(ge), 5), np.float64)
+        self._check(lib().rm_render_adjoint(self._h, ctypes.byref(frame), seed, ptr(d), ptr(tg), ptr(ge), len(ge),
+                                            ptr(grad)), "rm_render_adjoint")
+        return grad
+
     def read_restir(self, width: int, height: int):
         px = width * height
         g = np.zeros(px, GBUFFER_DTYPE)

@@ -155,6 +155,48 @@ def mixed_spheres(seed: int = 0, n_spheres: int = 64, subdivisions: int = 4, sce
     return s
 
 
+def inverse_scene(seed: int = 0, n_spheres: int = 64, subdivisions: int = 4):
+    """The mixed-scene generator reduced to 16 materials, for inverse rendering:
+    13 sphere materials drawn like the mixed scene's (40% Lambert, 30% GGX
+    metal, 30% dielectric of roughness 0 or 0.2), the ground (group 13) and
+    the 8 emissive quads in two groups of 4 (14, 15) with one emission each.
+    Lambert roughness is drawn in [0.4, 0.9] and metal albedo in [0.5, 0.95] so
+    that +-30% perturbations stay inside the parameter box. Returns
+    (scene with its BVH built, per-triangle group)."""
+    rng = np.random.default_rng(seed)
+    unit = icosphere(subdivisions)
+    mats = []
+    for _ in range(13):
+        kind = rng.uniform()
+        if kind < 0.4:
+            mats.append(material_rows(1, albedo=rng.uniform(0.1, 0.9, 3), rough=rng.uniform(0.4, 0.9)))
+        elif kind < 0.7:
+            mats.append(material_rows(1, albedo=rng.uniform(0.5, 0.95, 3), metallic=1.0, rough=rng.uniform(0.05, 0.6)))
+        else:
+            mats.append(material_rows(1, albedo=(1.0, 1.0, 1.0), ior=1.5, rough=float(rng.choice([0.0, 0.2]))))
+    s = Scene()
+    groups = []
+    for _ in range(n_spheres):
+        r = rng.uniform(0.2, 0.6)
+        c = rng.uniform(-4.0, 4.0, 3)
+        m = len(groups) if len(groups) < 13 else int(rng.integers(13))  # every material used once first
+        idx = s.add_triangles(c + r * unit, np.repeat(mats[m], len(unit), 0), normals=unit)
+        groups.append(np.full(len(idx), m, np.int32))
+    g = -4.6
+    s.add_triangles(_quad((-10, g, -10), (-10, g, 10), (10, g, 10), (10, g, -10)),
+                    material_rows(2, albedo=(0.5, 0.5, 0.5), rough=0.8))
+    groups.append(np.full(2, 13, np.int32))
+    emission = [rng.uniform(5.0, 20.0, 3), rng.uniform(5.0, 20.0, 3)]
+    for k in range(8):
+        x, z = rng.uniform(-4.0, 4.0, 2)
+        y, h = 4.9, 0.2
+        quad = _quad((x - h, y, z - h), (x + h, y, z - h), (x + h, y, z + h), (x - h, y, z + h))
+        s.add_triangles(quad, material_rows(2, albedo=(0.8, 0.8, 0.8), emission=emission[k // 4], rough=1.0))
+        groups.append(np.full(2, 14 + k // 4, np.int32))
+    s.build_bvh()
+    return s, np.concatenate(groups)
+
+
 def mixed_camera(frame: int = 0, frames: int = 34) -> Camera:
     """Orbit at radius 10, height 3 around the origin."""
     return orbit_camera((0.0, 0.0, 0.0), 10.0, 3.0, 360.0 * frame / frames, 45.0)

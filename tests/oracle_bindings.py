@@ -46,6 +46,10 @@ def orc() -> ctypes.CDLL:
                                          _P, _P, _P]),
             ("orc_trace_brute", ctypes.c_int, [ctypes.POINTER(SceneDesc), _P, _P, _SZ, ctypes.c_int, _P, _P]),
             ("orc_rng_draws", None, [ctypes.c_uint64, ctypes.c_int, _P, _P]),
+            ("orc_render_adjoint", ctypes.c_longlong, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame),
+                                                       ctypes.c_uint64, _P, _P, _P, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_int, _P]),
+            ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
         ]
         for name, res, args in sigs:
@@ -129,6 +133,32 @@ def render_reference_f64(exported, frame: Frame, seed: int, threads: int = THREA
     out = np.zeros(frame.width * frame.height * 3, np.float64)
     orc().orc_render_reference_f64(ctypes.byref(d), ctypes.byref(frame), seed, threads, ptr(out))
     return out
+
+
+def render_adjoint(exported, frame: Frame, seed: int, d_hdr, tri_group, group_emission, precision: int = 64,
+                   threads: int = THREADS) -> np.ndarray:
+    """The oracle's path-replay adjoint: (ngroup, 5) gradients of sum(d_hdr * hdr)."""
+    d = exported.desc()
+    dh = np.ascontiguousarray(d_hdr, np.float32).reshape(-1)
+    tg = np.ascontiguousarray(tri_group, np.int32)
+    ge = np.ascontiguousarray(group_emission, np.float32).reshape(-1, 3)
+    grad = np.zeros((len(ge), 5), np.float64)
+    bad = orc().orc_render_adjoint(ctypes.byref(d), ctypes.byref(frame), seed, ptr(dh), ptr(tg), ptr(ge), len(ge),
+                                   precision, threads, ptr(grad))
+    if bad < 0:
+        raise RuntimeError("orc_render_adjoint: frame not covered by the adjoint")
+    if bad > 0:
+        raise RuntimeError(f"orc_render_adjoint: {bad} samples replayed differently")
+    return grad
+
+
+def bsdf_grad_f64(wi, wo, n, alpha2, metallic, albedo):
+    """FP64 opaque BSDF value, d value / d albedo (own channel) and d value / d alpha2."""
+    arr = [np.ascontiguousarray(v, np.float64) for v in (wi, wo, n, albedo)]
+    val, da, dl = np.zeros(3), np.zeros(3), np.zeros(3)
+    orc().orc_bsdf_grad_f64(ptr(arr[0]), ptr(arr[1]), ptr(arr[2]), float(alpha2), float(metallic), ptr(arr[3]),
+                            ptr(val), ptr(da), ptr(dl))
+    return val, da, dl
 
 
 def trace(exported, origins, dirs, tmax=None, any_hit=False, threads: int = THREADS):
