@@ -14,6 +14,11 @@ Prints one JSON line: Mrays/s (rays actually traced, counted on the GPU),
 ms/frame, the roofline of the dominant kernel measured with HIP events, and
 the CPU oracle timed on a bounded sample of the same workload.
 
+With the default workload the line also carries "adjoint": the path-replay
+adjoint's GPU time over the forward's on the 16-material inverse-rendering
+scene at 512x512, 16 spp. --workload inverse runs the 200-step inverse
+rendering (ms per iteration, loss and parameter error).
+
 --workload heightfield measures the animated large-mesh case instead: a
 1024x1024 height field (2,093,058 triangles + 256 emissive ones) displaced by
 0.05 sin(2 pi t + x) every frame, 1920x1080, 8 spp, depth 6, ReSTIR defaults
@@ -191,12 +196,90 @@ def run_heightfield(args) -> dict:
     return line
 
 
+def adjoint_cost(steps: int = 5) -> dict:
+    """Forward vs path-replay adjoint GPU time on the 16-material inverse-rendering
+    scene (the mixed generator, 327,698 triangles) at 512x512, 16 spp, depth 8."""
+    from raymotion_amd.diff import DiffScene, Renderer, path_frame
+    s, groups = scenes.inverse_scene(seed=0)
+    ds = DiffScene(s.export(), groups)
+    backend = rm.Backend(0)
+    backend.set_profiling(True)
+    r = Renderer(backend, ds, path_frame(512, 512, 16, 8, scenes.mixed_camera(0)))
+    d = np.random.default_rng(0).uniform(0, 1, 512 * 512 * 3).astype(np.float32)
+    fwd, adj, fwd_wall, adj_wall, rays_f, rays_a = [], [], [], [], [], []
+    for k in range(steps + 1):
+        t0 = time.perf_counter()
+        r.forward(ds.initial, 7 + k)
+        t1 = time.perf_counter()
+        st = backend.stats()
+        f_ms, f_rays = st.ms[7], st.rays[5]
+        r.adjoint(ds.initial, 7 + k, d)
+        t2 = time.perf_counter()
+        st = backend.stats()
+        if k:  # the first pair warms up
+            fwd.append(f_ms); adj.append(st.ms[7]); rays_f.append(f_rays); rays_a.append(st.rays[5])
+            fwd_wall.append((t1 - t0) * 1e3); adj_wall.append((t2 - t1) * 1e3)
+    backend.close()
+    return {"workload": "16-material mixed scene, 512x512, 16 spp, depth 8, path-tracing frame",
+            "parameters": int(ds.ngroup * 5), "forward_ms": round(float(np.mean(fwd)), 3),
+            "adjoint_ms": round(float(np.mean(adj)), 3), "ratio": round(float(np.mean(adj) / np.mean(fwd)), 3),
+            "forward_rays": int(np.mean(rays_f)), "adjoint_rays": int(np.mean(rays_a)),
+            "forward_wall_ms": round(float(np.mean(fwd_wall)), 2), "adjoint_wall_ms": round(float(np.mean(adj_wall)), 2)}
+
+
+def run_inverse(args) -> dict:
+    """Inverse rendering on the 16-material scene: target at 512x512, 64 spp from
+    the ground-truth parameters; start from the ground truth perturbed by
+    U[0.7, 1.3] (seed 2); Adam, lr 1e-2, pixel L2 at 16 spp."""
+    from raymotion_amd.diff import DiffScene, Renderer, optimize, path_frame
+    s, groups = scenes.inverse_scene(seed=0)
+    ds = DiffScene(s.export(), groups)
+    backend = rm.Backend(0)
+    cam = scenes.mixed_camera(0)
+    r = Renderer(backend, ds, path_frame(512, 512, 16, 8, cam))
+    hi = Renderer(backend, ds, path_frame(512, 512, 64, 8, cam))
+    gt = ds.initial
+    target = hi.forward(gt, 1)
+    p0 = ds.clamp(gt * np.random.default_rng(2).uniform(0.7, 1.3, gt.shape))
+    steps = args.steps
+    t0 = time.perf_counter()
+    p, losses = optimize(r, target, p0, steps=steps, lr=1e-2)
+    wall = time.perf_counter() - t0
+
+    def blocks(img, k=8):
+        return img.reshape(512 // k, k, 512 // k, k, 3).mean(axis=(1, 3))
+
+    def image_error(q):
+        return float(np.mean((blocks(hi.forward(q, 2)) - blocks(target)) ** 2))
+
+    e0, e1, floor = image_error(p0), image_error(p), image_error(gt)
+    g = np.abs(r.adjoint(gt, 3, np.ones(512 * 512 * 3, np.float32)))
+    backend.close()
+    strong = g >= 0.1 * g.max()
+    observed = g > 0
+    return {
+        "metric": METRIC, "value": round(wall / steps * 1e3, 2), "unit": "ms/iteration", "n_gpus": 1,
+        "steps": steps, "warmup": 0, "ms_per_step": round(wall / steps * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural scene, seed 0)",
+        "config": {"workload": "inverse rendering, 16-material mixed scene, 80 parameters", "width": 512,
+                   "height": 512, "spp": 16, "target_spp": 64, "max_depth": 8, "optimizer": "Adam lr 1e-2",
+                   "loss_first5": round(float(losses[:5].mean()), 6), "loss_last5": round(float(losses[-5:].mean()), 6),
+                   "block8_image_error": {"start": e0, "end": e1, "ground_truth": floor},
+                   "param_abs_error_observed": {"start": round(float(np.abs(p0 - gt)[observed].mean()), 4),
+                                                "end": round(float(np.abs(p - gt)[observed].mean()), 4)},
+                   "param_abs_error_strong": {"count": int(strong.sum()),
+                                              "start": round(float(np.abs(p0 - gt)[strong].mean()), 4),
+                                              "end": round(float(np.abs(p - gt)[strong].mean()), 4)}},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed frames (10; heightfield 8)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames first (2; heightfield 1)")
-    ap.add_argument("--workload", choices=("mixed", "heightfield"), default="mixed")
+    ap.add_argument("--workload", choices=("mixed", "heightfield", "inverse"), default="mixed")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint cost measurement in the mixed line")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (scales HBM counter readings)")
@@ -216,10 +299,15 @@ def main() -> int:
 
     hf = args.workload == "heightfield"
     if args.steps is None:
-        args.steps = 8 if hf else 10
+        args.steps = {"heightfield": 8, "inverse": 200}.get(args.workload, 10)
     if args.warmup is None:
         args.warmup = 1 if hf else 2
-    line = run_heightfield(args) if hf else run_mixed(args)
+    if args.workload == "inverse":
+        line = run_inverse(args)
+    else:
+        line = run_heightfield(args) if hf else run_mixed(args)
+        if args.workload == "mixed" and not args.no_extras:
+            line["adjoint"] = adjoint_cost()
     print(json.dumps(line), flush=True)
     return 0
 

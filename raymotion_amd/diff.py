@@ -173,16 +173,20 @@ def finite_differences(renderer: Renderer, params, seed: int, d_hdr, entries, h:
 
 
 def optimize(renderer: Renderer, target, p0, steps: int = 200, lr: float = 1e-2, seed0: int = 1000,
-             log_every: int = 0):
+             log_every: int = 0, final_lr_fraction: float = 0.1):
     """Adam on the pixel L2 loss against `target` (H, W, 3) with new seeds per
-    iteration (forward and adjoint decorrelated); returns (final params,
-    per-step losses)."""
+    iteration (forward and adjoint decorrelated); the learning rate follows a
+    cosine from lr to final_lr_fraction * lr (1.0: constant). Returns (final
+    params, per-step losses)."""
     import torch
     tgt = torch.as_tensor(np.asarray(target, np.float32))
     p = torch.tensor(np.asarray(p0, np.float64), dtype=torch.float32, requires_grad=True)
     opt = torch.optim.Adam([p], lr=lr)
     losses = []
     for it in range(steps):
+        c = 0.5 * (1.0 + np.cos(np.pi * it / max(steps - 1, 1)))
+        for grp in opt.param_groups:
+            grp["lr"] = lr * (final_lr_fraction + (1.0 - final_lr_fraction) * c)
         opt.zero_grad()
         img = render(p, renderer, seed0 + 2 * it, seed0 + 2 * it + 1)
         loss = torch.mean((img - tgt) ** 2)
