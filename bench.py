@@ -273,12 +273,100 @@ def run_inverse(args) -> dict:
     }
 
 
+WEIGHTS = ROOT / "raymotion_amd" / "data" / "kpn_mixed.npy"
+
+
+def run_denoiser_train(args) -> dict:
+    """Trains the learned denoiser: 512 views of random mixed scenes (256x256,
+    4-spp ReSTIR input, 4096-spp path-traced reference), Adam lr 1e-3 with
+    batch 16, then evaluates it against the a-trous path."""
+    from raymotion_amd import denoise
+    backend = rm.Backend(0)
+    backend.set_profiling(True)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    data_s = denoise.make_training_set(backend, args.views, size=256, noisy_spp=4, ref_spp=args.ref_spp,
+                                       views_per_scene=8, seed=0, subdivisions=4, log=log)
+    t0 = time.perf_counter()
+    losses = denoise.train(backend, denoise.init_params(0), args.steps, batch=16, lr=1e-3, seed=1, log=log)
+    train_s = time.perf_counter() - t0
+    weights = backend.kpn_get_weights()
+    if args.save_weights:
+        np.save(args.save_weights, weights)
+    ev = denoise.evaluate(backend, frames=(4, 20), log=log)
+    backend.close()
+    return {"metric": METRIC, "value": round(train_s / max(args.steps, 1) * 1e3, 3), "unit": "ms/training step",
+            "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": round(train_s / max(args.steps, 1) * 1e3, 3),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (random mixed scenes, rendered by the backend)",
+            "config": {"workload": "kernel-predicting denoiser training", "views": args.views, "crop": 256,
+                       "input_spp": 4, "reference_spp": args.ref_spp, "batch": 16, "optimizer": "Adam lr 1e-3",
+                       "training_set_s": round(data_s, 1), "training_s": round(train_s, 1),
+                       "loss_first100": float(np.mean(losses[:100])), "loss_last100": float(np.mean(losses[-100:]))},
+            "denoiser": ev}
+
+
+def denoiser_extras() -> dict | None:
+    """Equal-RMSE comparison with the committed trained weights (one evaluated
+    frame of the mixed-scene orbit)."""
+    from raymotion_amd import denoise
+    if not WEIGHTS.exists():
+        return None
+    backend = rm.Backend(0)
+    backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
+    ev = denoise.evaluate(backend, frames=(4,), svgf_spp=(4, 8, 16, 32))
+    backend.close()
+    return ev
+
+
+# Multiply-adds of the network per pixel (13 real input channels, 25 logits),
+# times 2 FLOP: 2 * 9 * (13*64 + 4*64*64 + 64*25).
+KPN_FLOP_PER_PIXEL = 2 * 9 * (13 * 64 + 4 * 64 * 64 + 64 * 25)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def run_denoiser_infer(args) -> dict:
+    """Frames of the mixed-scene orbit through ReSTIR at 4 spp and the learned
+    denoiser (committed weights) at 1280x720; the network's MFMA roofline."""
+    from raymotion_amd import denoise
+    scene = scenes.mixed_spheres(seed=0)
+    scene.build_bvh()
+    backend = rm.Backend(0)
+    backend.set_scene(scene)
+    backend.set_profiling(True)
+    backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
+    frame_ms, net_ms = [], []
+    for k in range(args.warmup + args.steps):
+        backend.render(denoise.restir_frame(1280, 720, 4, scenes.mixed_camera(k % 34), denoise=2), 12345 + k,
+                       want_hdr=False)
+        st = backend.stats()
+        if k >= args.warmup:
+            frame_ms.append(st.ms[7])
+            net_ms.append(st.ms[5])
+    backend.close()
+    ms = float(np.mean(net_ms))
+    flop = KPN_FLOP_PER_PIXEL * 1280 * 720
+    tflops = flop / (ms / 1e3) / 1e12
+    return {"metric": METRIC, "value": round(float(np.mean(frame_ms)), 3), "unit": "ms/frame", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(float(np.mean(frame_ms)), 3),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic (procedural scene, seed 0)",
+            "config": {"workload": "mixed-scene orbit, ReSTIR 4 spp + learned KPN denoiser, 1280x720",
+                       "network_ms": round(ms, 3)},
+            "roofline": {"kernel": "kpn network (6 k_conv + k_kpn_apply)", "bound": "mfma", "achieved": round(tflops, 1),
+                         "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+                         "traffic": None, "flop_per_launch": flop}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed frames (10; heightfield 8)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames first (2; heightfield 1)")
-    ap.add_argument("--workload", choices=("mixed", "heightfield", "inverse"), default="mixed")
+    ap.add_argument("--workload", choices=("mixed", "heightfield", "inverse", "denoiser-train", "denoiser-infer"),
+                    default="mixed")
+    ap.add_argument("--views", type=int, default=512, help="denoiser-train: training views")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="denoiser-train: reference samples per pixel")
+    ap.add_argument("--save-weights", default=None, help="denoiser-train: write the trained weights (.npy)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint cost measurement in the mixed line")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -299,15 +387,20 @@ def main() -> int:
 
     hf = args.workload == "heightfield"
     if args.steps is None:
-        args.steps = {"heightfield": 8, "inverse": 200}.get(args.workload, 10)
+        args.steps = {"heightfield": 8, "inverse": 200, "denoiser-train": 20000}.get(args.workload, 10)
     if args.warmup is None:
         args.warmup = 1 if hf else 2
     if args.workload == "inverse":
         line = run_inverse(args)
+    elif args.workload == "denoiser-train":
+        line = run_denoiser_train(args)
+    elif args.workload == "denoiser-infer":
+        line = run_denoiser_infer(args)
     else:
         line = run_heightfield(args) if hf else run_mixed(args)
         if args.workload == "mixed" and not args.no_extras:
             line["adjoint"] = adjoint_cost()
+            line["denoiser"] = denoiser_extras()
     print(json.dumps(line), flush=True)
     return 0
 

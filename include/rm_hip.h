@@ -92,7 +92,8 @@ typedef struct {
     int32_t nomis;
     int32_t restir_candidates, restir_spatial, restir_mcap;
     float restir_radius;
-    int32_t denoise;             /* RAYMOTION_DENOISE (metal.mm:269-271) */
+    int32_t denoise;             /* RAYMOTION_DENOISE (metal.mm:269-271): 0 off, 1 a-trous;
+                                    2 learned kernel-predicting network (needs rm_kpn_set_weights) */
     int32_t denoise_iterations;  /* a-trous passes; the reference runs 2 */
     int32_t temporal;            /* RAYMOTION_RESTIR_TEMPORAL (metal.mm:251-252) */
     rm_camera camera;
@@ -109,7 +110,8 @@ typedef struct {
     uint64_t node_visits[6]; /* 4-wide BVH nodes fetched */
     uint64_t tri_tests[6];   /* ray/triangle tests */
     float ms[8];          /* GPU time per pass: megakernel, generate, spatial, shade,
-                             temporal, atrous (all passes), tonemap, whole frame */
+                             temporal (or feature build for denoise = 2), atrous (all
+                             passes, or the network and its filter), tonemap, whole frame */
     int32_t bvh_nodes;    /* 4-wide nodes resident */
     int32_t stack_depth;  /* traversal stack entries the resident tree needs */
     float scene_ms[2];    /* last rm_set_scene, GPU time: host-to-device copies, refit kernels */
@@ -161,6 +163,34 @@ int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reserv
  * (pathtrace.metal:552-769); it has no reference counterpart. */
 int rm_render_adjoint(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, const float* d_hdr, const int32_t* tri_group,
                       const float* group_emission, int ngroup, double* grad);
+
+/* ---- Learned kernel-predicting denoiser (no reference counterpart; the
+ * reference's denoiser is the a-trous filter, pathtrace.metal:1263-1354, which
+ * stays the default and the fallback). rm_frame.denoise = 2 runs a 6-layer
+ * conv3x3 network (BF16 MFMA, FP32 accumulation) on per-pixel features of the
+ * ReSTIR frame and filters the noisy radiance with the predicted 5x5 kernels. */
+#define RM_KPN_FEATURES 16 /* per pixel: log(1+radiance) 3, albedo 3, normal 3,
+                              log(1+depth), log(1+luminance variance), motion 2, zero 3 */
+/* Parameter count (floats): per layer [cout][kp] weights, then cout biases. */
+size_t rm_kpn_param_count(void);
+/* Sets the weights (and restarts the optimizer state); required for denoise = 2. */
+int rm_kpn_set_weights(rm_ctx* ctx, const float* params, size_t n);
+int rm_kpn_get_weights(rm_ctx* ctx, float* params, size_t n);
+/* Renders a ReSTIR frame (no denoiser) and returns the network's inputs:
+ * features (W*H*RM_KPN_FEATURES) and noisy linear radiance (W*H*3). */
+int rm_render_features(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, float* features, float* noisy);
+/* Device-resident training set of equally sized images with references. */
+int rm_kpn_dataset_clear(rm_ctx* ctx);
+int rm_kpn_dataset_add(rm_ctx* ctx, const float* features, const float* noisy, const float* reference, int width,
+                       int height);
+/* Adam steps on random batches of the training set; loss per step:
+ * mean((log(1 + denoised) - log(1 + reference))^2). losses may be NULL. */
+int rm_kpn_train(rm_ctx* ctx, int steps, int batch, float lr, uint64_t seed, float* losses);
+/* Denoises one image given as features + noisy radiance; out W*H*3. */
+int rm_kpn_denoise(rm_ctx* ctx, const float* features, const float* noisy, int width, int height, float* out);
+/* Loss and parameter gradient of the current weights on `count` images. */
+int rm_kpn_gradient(rm_ctx* ctx, const float* features, const float* noisy, const float* reference, int count,
+                    int width, int height, float* grad, float* loss);
 
 /* Per-pass GPU timing with HIP events (off by default). */
 int rm_set_profiling(rm_ctx* ctx, int enable);

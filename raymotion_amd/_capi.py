@@ -96,6 +96,15 @@ _SIGNATURES = [
     ("rm_trace", _I, [_P, _P, _P, _P, _SZ, _I, _P, _P, _P, _P]),
     ("rm_read_restir", _I, [_P, _P, _P, _P, _P]),
     ("rm_render_adjoint", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P, _P, _I, _P]),
+    ("rm_kpn_param_count", _SZ, []),
+    ("rm_kpn_set_weights", _I, [_P, _P, _SZ]),
+    ("rm_kpn_get_weights", _I, [_P, _P, _SZ]),
+    ("rm_render_features", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P]),
+    ("rm_kpn_dataset_clear", _I, [_P]),
+    ("rm_kpn_dataset_add", _I, [_P, _P, _P, _P, _I, _I]),
+    ("rm_kpn_train", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
+    ("rm_kpn_denoise", _I, [_P, _P, _P, _I, _I, _P]),
+    ("rm_kpn_gradient", _I, [_P, _P, _P, _P, _I, _I, _I, _P, _P]),
     ("rm_set_profiling", _I, [_P, _I]),
     ("rm_get_stats", _I, [_P, ctypes.POINTER(Stats)]),
     ("rm_scene_create", _P, []),

@@ -15,6 +15,7 @@
 
 #include "rm_hip.h"
 #include "rm_kernels.h"
+#include "rm_kpn.h"
 #include "rm_layout.h"
 #include "rm_pack.h"
 
@@ -110,6 +111,17 @@ struct rm_ctx {
     DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
     std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
 
+    // Learned denoiser: parameters (fp32 master, bf16 copy), gradient, Adam
+    // moments, flipped weights for the input gradients, activations of the
+    // last forward pass and its scratch, and the device-resident training set.
+    bool kpn_ready = false;
+    int kpn_step = 0;
+    DevBuf kpn_p, kpn_pb, kpn_g, kpn_m, kpn_v, kpn_wd;
+    DevBuf kpn_x[kKpnLayers], kpn_logits, kpn_dz[2], kpn_dlogit, kpn_partial, kpn_loss;
+    DevBuf kpn_feat32, kpn_rad, kpn_ref, kpn_out, kpn_var, kpn_idx;
+    DevBuf ds_feat, ds_rad, ds_ref;
+    int ds_count = 0, ds_w = 0, ds_h = 0;
+
     // Temporal history (metal.mm:36-40).
     rm_camera hist_cam{};
     int hist_w = 0, hist_h = 0;
@@ -202,6 +214,64 @@ bool upload(rm_ctx* c, DevBuf& b, const std::vector<T>& v) {
     return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;
 }
 
+// Activation buffers for a forward (and backward) pass over `pixels` pixels.
+bool kpn_ensure(rm_ctx* c, size_t pixels, bool train) {
+    const KpnNet net = kpn_net();
+    bool ok = c->kpn_x[0].ensure(pixels * kKpnFeat * 2) && c->kpn_logits.ensure(pixels * 32 * sizeof(float));
+    for (int l = 1; l < kKpnLayers && ok; ++l) ok = c->kpn_x[l].ensure(pixels * kKpnHidden * 2);
+    if (ok && train) {
+        size_t scratch = 0;
+        for (int l = 0; l < kKpnLayers; ++l)
+            scratch = std::max(scratch, kpn_wgrad_scratch(net.layer[l].cin, net.layer[l].cout, int(pixels)));
+        ok = c->kpn_dz[0].ensure(pixels * kKpnHidden * 2) && c->kpn_dz[1].ensure(pixels * kKpnHidden * 2) &&
+             c->kpn_dlogit.ensure(pixels * 32 * 2) && c->kpn_partial.ensure(scratch * sizeof(float)) &&
+             c->kpn_wd.ensure(size_t(kKpnHidden) * 9 * kKpnHidden * 2) && c->kpn_loss.ensure(sizeof(float));
+    }
+    return ok;
+}
+
+// Forward pass over n images of h x w: features kpn_x[0] (bf16), noisy
+// radiance `rad`; with `ref`, the loss is added to *loss and the logits'
+// gradient is left in kpn_dlogit.
+void kpn_forward(rm_ctx* c, int n, int h, int w, const float* rad, const float* ref, float* out, float4* out4,
+                 float* loss) {
+    const KpnNet net = kpn_net();
+    const __bf16* pb = c->kpn_pb.as<__bf16>();
+    const float* p = c->kpn_p.as<float>();
+    for (int l = 0; l < kKpnLayers; ++l) {
+        const KpnLayer& L = net.layer[l];
+        const bool last = l == kKpnLayers - 1;
+        launch_conv(L.cin, L.cout, last ? 1 : 0, c->kpn_x[l].as<__bf16>(), pb + L.w_off, p + L.b_off, nullptr,
+                    last ? c->kpn_logits.ptr : c->kpn_x[l + 1].ptr, n, h, w, c->stream);
+    }
+    const double count = double(n) * h * w * 3;
+    launch_kpn_apply(c->kpn_logits.as<float>(), rad, ref, out, out4, ref ? c->kpn_dlogit.as<__bf16>() : nullptr, loss,
+                     float(1.0 / count), n, h, w, c->stream);
+}
+
+// Gradient of the last training forward pass into kpn_g (every entry written).
+void kpn_backward(rm_ctx* c, int n, int h, int w) {
+    const KpnNet net = kpn_net();
+    const __bf16* pb = c->kpn_pb.as<__bf16>();
+    float* g = c->kpn_g.as<float>();
+    (void)hipMemsetAsync(g, 0, net.nparam * sizeof(float), c->stream);  // padded weight entries stay 0
+    const __bf16* dz = c->kpn_dlogit.as<__bf16>();
+    int flip = 0;
+    for (int l = kKpnLayers - 1; l >= 0; --l) {
+        const KpnLayer& L = net.layer[l];
+        launch_wgrad(L.cin, L.cout, c->kpn_x[l].as<__bf16>(), dz, c->kpn_partial.as<float>(), g + L.w_off,
+                     g + L.b_off, n, h, w, c->stream);
+        if (l == 0) break;
+        // dL/d(pre-activation of layer l - 1) = (dz conv flipped weights) where x[l] > 0
+        launch_flip_weights(L.cin, L.cout, pb + L.w_off, c->kpn_wd.as<__bf16>(), c->stream);
+        __bf16* next = c->kpn_dz[flip].as<__bf16>();
+        launch_conv(L.cout, L.cin, 2, dz, c->kpn_wd.as<__bf16>(), nullptr, c->kpn_x[l].as<__bf16>(), next, n, h, w,
+                    c->stream);
+        dz = next;
+        flip ^= 1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -260,6 +330,12 @@ void rm_destroy(rm_ctx* c) {
                       &c->probe_tuv, &c->adj_img, &c->adj_tri_group, &c->adj_light_group, &c->adj_base,
                       &c->adj_grad, &c->adj_bad};
     for (DevBuf* b : bufs) b->release();
+    DevBuf* kbufs[] = {&c->kpn_p, &c->kpn_pb, &c->kpn_g, &c->kpn_m, &c->kpn_v, &c->kpn_wd, &c->kpn_logits,
+                       &c->kpn_dz[0], &c->kpn_dz[1], &c->kpn_dlogit, &c->kpn_partial, &c->kpn_loss, &c->kpn_feat32,
+                       &c->kpn_rad, &c->kpn_ref, &c->kpn_out, &c->kpn_var, &c->kpn_idx, &c->ds_feat, &c->ds_rad,
+                       &c->ds_ref};
+    for (DevBuf* b : kbufs) b->release();
+    for (auto& b : c->kpn_x) b.release();
     c->gbuf.release();
     c->hist_gbuf.release();
     c->rinit.release();
@@ -396,9 +472,14 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     return RM_OK;
 }
 
-int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr) {
-    if (!c || !f) return RM_ERR_INVALID;
-    std::lock_guard<std::mutex> lock(c->mu);
+}  // extern "C"
+
+namespace {
+
+// rm_render with the context locked; with feat_host / rad_host (ReSTIR frames
+// only) also the denoiser's input features and noisy radiance.
+int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr, float* feat_host,
+                  float* rad_host) {
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     if (f->width <= 0 || f->height <= 0 || f->spp <= 0 || f->bounces < 0 ||
         size_t(f->width) * size_t(f->height) > 100000000)
@@ -410,6 +491,11 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     FrameParams P = frame_params(*f, seed);
     const DevScene sc = c->dev_scene();
     const bool use_spatial = uses_restir(c, P);
+    const bool learned = use_spatial && f->denoise == 2;
+    const bool features = use_spatial && (learned || feat_host);
+    if ((feat_host || rad_host) && !use_spatial)
+        return c->fail(RM_ERR_INVALID, "denoiser features need a ReSTIR frame (restir_spatial > 0, lights, opaque)");
+    if (learned && !c->kpn_ready) return c->fail(RM_ERR_INVALID, "denoise = 2 needs weights (rm_kpn_set_weights)");
     bool have_history = use_spatial && c->hist_valid && c->hist_w == f->width && c->hist_h == f->height &&
                         seed != c->hist_seed && c->hist_key == c->hist_hash;
     if (!f->temporal) have_history = false;
@@ -451,12 +537,29 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         mark(1);
         launch_spatial(c->stack_need, sc, P, gb, ri, rmg, hg, hr, cnt, c->stream);
         mark(2);
-        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr, c->queue.as<unsigned>() + 1, cnt,
-                    c->stream);
+        if (features && (!c->kpn_var.ensure(px * sizeof(float)) || !c->kpn_feat32.ensure(px * kKpnFeat * 4) ||
+                         !c->kpn_rad.ensure(px * 3 * sizeof(float)) || (learned && !kpn_ensure(c, px, false)))) {
+            c->drop_history();
+            return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
+        }
+        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr,
+                    features ? c->kpn_var.as<float>() : nullptr, c->queue.as<unsigned>() + 1, cnt, c->stream);
         mark(3);
         const float4* final_img = c->shade.as<float4>();
         bool final_in_hist = false;
-        if (f->denoise) {
+        if (features)
+            launch_features(P, gb, c->shade.as<float4>(), c->kpn_var.as<float>(), c->kpn_feat32.as<float>(),
+                            c->kpn_rad.as<float>(), c->stream);
+        if (learned) {
+            // Kernel-predicting network in place of the a-trous filter.
+            launch_to_bf16(c->kpn_feat32.as<float>(), c->kpn_x[0].as<__bf16>(), px * kKpnFeat, c->stream);
+            mark(4);
+            kpn_forward(c, 1, f->height, f->width, c->kpn_rad.as<float>(), nullptr, nullptr, c->den_a.as<float4>(),
+                        nullptr);
+            mark(5);
+            denoise_passes = 1;
+            final_img = c->den_a.as<float4>();
+        } else if (f->denoise) {
             // Temporal pass into A, then a-trous passes ping-ponging between A
             // and the history color buffer (its old content has been consumed).
             float4* a = c->den_a.as<float4>();
@@ -486,6 +589,14 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
             if (int rc = check(c, hipMemcpyAsync(hdr, c->hdr3.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                                  c->stream), "readback"))
                 return rc;
+        if (feat_host)
+            if (int rc = check(c, hipMemcpyAsync(feat_host, c->kpn_feat32.ptr, px * kKpnFeat * 4, hipMemcpyDeviceToHost,
+                                                 c->stream), "readback"))
+                return rc;
+        if (rad_host)
+            if (int rc = check(c, hipMemcpyAsync(rad_host, c->kpn_rad.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                                                 c->stream), "readback"))
+                return rc;
         unsigned long long host_cnt[kNumCounters];
         if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
             return rc;
@@ -497,7 +608,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         // Publish this frame as the next frame's history (metal.mm:374-385).
         std::swap(c->gbuf, c->hist_gbuf);
         std::swap(c->rmerged, c->hist_res);
-        if (f->denoise && !final_in_hist) std::swap(c->den_a, c->hist_color);
+        if (f->denoise == 1 && !final_in_hist) std::swap(c->den_a, c->hist_color);
         c->hist_cam = f->camera;
         c->hist_w = f->width;
         c->hist_h = f->height;
@@ -516,7 +627,7 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
         }
         if (lanes == 1)
             launch_wave(c->stack_need, false, sc, P, GBufferSoA{}, ReservoirSoA{}, hdr ? c->mega_hdr.as<float4>() : nullptr,
-                        c->rgb8.as<uint8_t>(), c->queue.as<unsigned>() + 1, cnt, c->stream);
+                        c->rgb8.as<uint8_t>(), nullptr, c->queue.as<unsigned>() + 1, cnt, c->stream);
         else
             launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(),
                                 hdr ? c->mega_hdr.as<float4>() : nullptr, cnt, c->stream);
@@ -562,6 +673,213 @@ int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float*
     c->stats = st;
     c->err.clear();
     return RM_OK;
+}
+
+// Grow-only buffer that keeps its contents (the training set).
+bool grow_keep(DevBuf& b, size_t bytes, hipStream_t st) {
+    if (bytes <= b.cap) return true;
+    DevBuf n;
+    if (!n.ensure(std::max(bytes, 2 * b.cap))) return false;
+    if (b.ptr && b.cap && hipMemcpyAsync(n.ptr, b.ptr, b.cap, hipMemcpyDeviceToDevice, st) != hipSuccess) return false;
+    if (hipStreamSynchronize(st) != hipSuccess) return false;
+    b.release();
+    b = n;
+    n.ptr = nullptr;
+    n.cap = 0;
+    return true;
+}
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Uploads n feature images (fp32, kKpnFeat per pixel) as bf16 into kpn_x[0],
+// and their radiance (and reference) into kpn_rad (kpn_ref).
+int kpn_upload(rm_ctx* c, const float* feat, const float* noisy, const float* ref, size_t px) {
+    hipStream_t st = c->stream;
+    if (!c->kpn_feat32.ensure(px * kKpnFeat * 4) || !c->kpn_rad.ensure(px * 12) || (ref && !c->kpn_ref.ensure(px * 12)))
+        return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
+    if (hipMemcpyAsync(c->kpn_feat32.ptr, feat, px * kKpnFeat * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(c->kpn_rad.ptr, noisy, px * 12, hipMemcpyHostToDevice, st) != hipSuccess ||
+        (ref && hipMemcpyAsync(c->kpn_ref.ptr, ref, px * 12, hipMemcpyHostToDevice, st) != hipSuccess))
+        return c->fail(RM_ERR_DEVICE, "denoiser upload failed");
+    launch_to_bf16(c->kpn_feat32.as<float>(), c->kpn_x[0].as<__bf16>(), px * kKpnFeat, st);
+    return RM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rm_render(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr) {
+    if (!c || !f) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    return render_locked(c, f, seed, rgb8, hdr, nullptr, nullptr);
+}
+
+int rm_render_features(rm_ctx* c, const rm_frame* f, uint64_t seed, float* features, float* noisy) {
+    if (!c || !f || !features || !noisy) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    return render_locked(c, f, seed, nullptr, nullptr, features, noisy);
+}
+
+size_t rm_kpn_param_count(void) { return kpn_net().nparam; }
+
+int rm_kpn_set_weights(rm_ctx* c, const float* params, size_t n) {
+    if (!c || !params) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    const size_t np = kpn_net().nparam;
+    if (n != np) return c->fail(RM_ERR_INVALID, "rm_kpn_set_weights: expected " + std::to_string(np) + " floats");
+    const size_t bytes = np * sizeof(float);
+    if (!c->kpn_p.ensure(bytes) || !c->kpn_pb.ensure(np * 2) || !c->kpn_g.ensure(bytes) || !c->kpn_m.ensure(bytes) ||
+        !c->kpn_v.ensure(bytes))
+        return c->fail(RM_ERR_ALLOC, "HIP denoiser parameter allocation failed");
+    hipStream_t st = c->stream;
+    if (hipMemcpyAsync(c->kpn_p.ptr, params, bytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(c->kpn_m.ptr, 0, bytes, st) != hipSuccess || hipMemsetAsync(c->kpn_v.ptr, 0, bytes, st) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "denoiser upload failed");
+    launch_to_bf16(c->kpn_p.as<float>(), c->kpn_pb.as<__bf16>(), np, st);
+    if (int rc = check(c, hipStreamSynchronize(st), "denoiser weights")) return rc;
+    c->kpn_step = 0;
+    c->kpn_ready = true;
+    return RM_OK;
+}
+
+int rm_kpn_get_weights(rm_ctx* c, float* params, size_t n) {
+    if (!c || !params) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (!c->kpn_ready) return c->fail(RM_ERR_INVALID, "no denoiser weights set");
+    if (n != kpn_net().nparam) return c->fail(RM_ERR_INVALID, "rm_kpn_get_weights: wrong size");
+    if (int rc = check(c, hipMemcpy(params, c->kpn_p.ptr, n * sizeof(float), hipMemcpyDeviceToHost), "readback"))
+        return rc;
+    return RM_OK;
+}
+
+int rm_kpn_dataset_clear(rm_ctx* c) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->ds_count = 0;
+    return RM_OK;
+}
+
+int rm_kpn_dataset_add(rm_ctx* c, const float* features, const float* noisy, const float* reference, int width,
+                       int height) {
+    if (!c || !features || !noisy || !reference || width <= 0 || height <= 0) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (c->ds_count > 0 && (width != c->ds_w || height != c->ds_h))
+        return c->fail(RM_ERR_INVALID, "training images must share one size");
+    const size_t px = size_t(width) * height, at = size_t(c->ds_count) * px;
+    hipStream_t st = c->stream;
+    if (!grow_keep(c->ds_feat, (at + px) * kKpnFeat * 2, st) || !grow_keep(c->ds_rad, (at + px) * 12, st) ||
+        !grow_keep(c->ds_ref, (at + px) * 12, st) || !c->kpn_feat32.ensure(px * kKpnFeat * 4))
+        return c->fail(RM_ERR_ALLOC, "HIP training set allocation failed");
+    if (hipMemcpyAsync(c->kpn_feat32.ptr, features, px * kKpnFeat * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(c->ds_rad.as<float>() + at * 3, noisy, px * 12, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(c->ds_ref.as<float>() + at * 3, reference, px * 12, hipMemcpyHostToDevice, st) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "training set upload failed");
+    launch_to_bf16(c->kpn_feat32.as<float>(), c->ds_feat.as<__bf16>() + at * kKpnFeat, px * kKpnFeat, st);
+    if (int rc = check(c, hipStreamSynchronize(st), "training set upload")) return rc;
+    c->ds_w = width;
+    c->ds_h = height;
+    ++c->ds_count;
+    return RM_OK;
+}
+
+int rm_kpn_train(rm_ctx* c, int steps, int batch, float lr, uint64_t seed, float* losses) {
+    if (!c || steps < 0 || batch <= 0) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->kpn_ready) return c->fail(RM_ERR_INVALID, "no denoiser weights set");
+    if (c->ds_count == 0) return c->fail(RM_ERR_INVALID, "empty training set");
+    if (steps == 0) return RM_OK;
+    const int h = c->ds_h, w = c->ds_w;
+    const size_t ppi = size_t(h) * w, px = ppi * size_t(batch);
+    hipStream_t st = c->stream;
+    std::vector<int32_t> idx(size_t(steps) * batch);
+    uint64_t s = seed;
+    for (auto& v : idx) v = int32_t(splitmix(s) % uint64_t(c->ds_count));
+    if (!kpn_ensure(c, px, true) || !c->kpn_rad.ensure(px * 12) || !c->kpn_ref.ensure(px * 12) ||
+        !c->kpn_idx.ensure(idx.size() * 4) || !c->kpn_loss.ensure(size_t(steps) * sizeof(float)))
+        return c->fail(RM_ERR_ALLOC, "HIP denoiser training allocation failed");
+    if (hipMemcpyAsync(c->kpn_idx.ptr, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(c->kpn_loss.ptr, 0, size_t(steps) * sizeof(float), st) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "denoiser training upload failed");
+    const size_t np = kpn_net().nparam;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], st);
+    for (int k = 0; k < steps; ++k) {
+        launch_gather(c->ds_feat.as<__bf16>(), c->ds_rad.as<float>(), c->ds_ref.as<float>(),
+                      c->kpn_idx.as<int32_t>() + size_t(k) * batch, batch, int(ppi), c->kpn_x[0].as<__bf16>(),
+                      c->kpn_rad.as<float>(), c->kpn_ref.as<float>(), st);
+        kpn_forward(c, batch, h, w, c->kpn_rad.as<float>(), c->kpn_ref.as<float>(), nullptr, nullptr,
+                    c->kpn_loss.as<float>() + k);
+        kpn_backward(c, batch, h, w);
+        launch_adam(c->kpn_p.as<float>(), c->kpn_g.as<float>(), c->kpn_m.as<float>(), c->kpn_v.as<float>(),
+                    c->kpn_pb.as<__bf16>(), np, lr, 0.9f, 0.999f, 1e-8f, ++c->kpn_step, st);
+    }
+    if (c->profiling) (void)hipEventRecord(c->ev[1], st);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    if (losses)
+        if (int rc = check(c, hipMemcpyAsync(losses, c->kpn_loss.ptr, size_t(steps) * sizeof(float),
+                                             hipMemcpyDeviceToHost, st), "readback"))
+            return rc;
+    if (int rc = check(c, hipStreamSynchronize(st), "denoiser training")) return rc;
+    rm_stats sts{};
+    if (c->profiling && hipEventElapsedTime(&sts.ms[7], c->ev[0], c->ev[1]) != hipSuccess) sts.ms[7] = 0.0f;
+    c->stats = sts;
+    return RM_OK;
+}
+
+int rm_kpn_denoise(rm_ctx* c, const float* features, const float* noisy, int width, int height, float* out) {
+    if (!c || !features || !noisy || !out || width <= 0 || height <= 0) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->kpn_ready) return c->fail(RM_ERR_INVALID, "no denoiser weights set");
+    const size_t px = size_t(width) * height;
+    if (!kpn_ensure(c, px, false) || !c->kpn_out.ensure(px * 12))
+        return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
+    if (int rc = kpn_upload(c, features, noisy, nullptr, px)) return rc;
+    if (c->profiling) (void)hipEventRecord(c->ev[0], c->stream);
+    kpn_forward(c, 1, height, width, c->kpn_rad.as<float>(), nullptr, c->kpn_out.as<float>(), nullptr, nullptr);
+    if (c->profiling) (void)hipEventRecord(c->ev[1], c->stream);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    if (int rc = check(c, hipMemcpyAsync(out, c->kpn_out.ptr, px * 12, hipMemcpyDeviceToHost, c->stream), "readback"))
+        return rc;
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "denoise")) return rc;
+    rm_stats sts{};
+    if (c->profiling && hipEventElapsedTime(&sts.ms[5], c->ev[0], c->ev[1]) != hipSuccess) sts.ms[5] = 0.0f;
+    c->stats = sts;
+    return RM_OK;
+}
+
+int rm_kpn_gradient(rm_ctx* c, const float* features, const float* noisy, const float* reference, int count, int width,
+                    int height, float* grad, float* loss) {
+    if (!c || !features || !noisy || !reference || !grad || count <= 0 || width <= 0 || height <= 0)
+        return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->kpn_ready) return c->fail(RM_ERR_INVALID, "no denoiser weights set");
+    const size_t px = size_t(width) * height * size_t(count);
+    if (!kpn_ensure(c, px, true)) return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
+    if (int rc = kpn_upload(c, features, noisy, reference, px)) return rc;
+    if (int rc = check(c, hipMemsetAsync(c->kpn_loss.ptr, 0, sizeof(float), c->stream), "loss")) return rc;
+    kpn_forward(c, count, height, width, c->kpn_rad.as<float>(), c->kpn_ref.as<float>(), nullptr, nullptr,
+                c->kpn_loss.as<float>());
+    kpn_backward(c, count, height, width);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    const size_t np = kpn_net().nparam;
+    if (int rc = check(c, hipMemcpyAsync(grad, c->kpn_g.ptr, np * sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                       "readback"))
+        return rc;
+    if (loss)
+        if (int rc = check(c, hipMemcpyAsync(loss, c->kpn_loss.ptr, sizeof(float), hipMemcpyDeviceToHost, c->stream),
+                           "readback"))
+            return rc;
+    return check(c, hipStreamSynchronize(c->stream), "denoiser gradient");
 }
 
 int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tmax, size_t n, int any, int32_t* tri,

@@ -19,12 +19,13 @@ void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const 
 void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                     const ReservoirSoA& ro, const GBufferSoA& hg, const ReservoirSoA& hr, unsigned long long* cnt,
                     hipStream_t st);
-// Persistent path engine: restir_shade (shade = true, writes hdr) or
-// render_paths with one lane per pixel (writes rgb, and hdr if not null).
-// `queue` must hold 0 when the kernel starts.
+// Persistent path engine: restir_shade (shade = true, writes hdr and, if not
+// null, the per-pixel sample variance of luminance) or render_paths with one
+// lane per pixel (writes rgb, and hdr if not null). `queue` must hold 0 when
+// the kernel starts.
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
-                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, unsigned int* queue, unsigned long long* cnt,
-                 hipStream_t st);
+                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
+                 unsigned long long* cnt, hipStream_t st);
 void launch_temporal(const FrameParams& P, const GBufferSoA& gb, const GBufferSoA& hg, const float4* in,
                      const float4* hist, float4* out, hipStream_t st);
 void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in, float4* out, int step, hipStream_t st);

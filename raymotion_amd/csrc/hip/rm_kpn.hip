@@ -1,0 +1,465 @@
+// Kernels of the kernel-predicting denoiser on gfx950: BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation) convolutions for the forward
+// pass and both gradients, the per-pixel 5x5 kernel application with its
+// loss and gradient, Adam, and the feature builder.
+//
+// Activations are NHWC in bf16: a pixel's 64 channels are one 128-byte line.
+// A conv3x3 is a GEMM over (pixels) x (output channels) with K = 9 taps x cin;
+// a lane's A fragment is 8 consecutive channels of one (shifted) pixel, one
+// 16-byte load, and its B fragment 8 consecutive K of one output channel.
+#include "rm_kpn.h"
+
+#include <algorithm>
+
+namespace rmh {
+
+KpnNet kpn_net() {
+    KpnNet net{};
+    const int cin[kKpnLayers] = {kKpnFeat, kKpnHidden, kKpnHidden, kKpnHidden, kKpnHidden, kKpnHidden};
+    const int cout[kKpnLayers] = {kKpnHidden, kKpnHidden, kKpnHidden, kKpnHidden, kKpnHidden, 32};
+    size_t off = 0;
+    for (int l = 0; l < kKpnLayers; ++l) {
+        KpnLayer& L = net.layer[l];
+        L.cin = cin[l];
+        L.cout = cout[l];
+        L.kp = (9 * cin[l] + 31) / 32 * 32;
+        L.w_off = off;
+        off += size_t(L.cout) * size_t(L.kp);
+        L.b_off = off;
+        off += size_t(L.cout);
+    }
+    net.nparam = off;
+    return net;
+}
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) z[j] = (__bf16)0.0f;
+    return z;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// One wave computes 32 pixels x COUT channels: two 16-pixel row tiles times
+// COUT / 16 column tiles; a block is 4 waves (128 pixels).
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, const __bf16* __restrict__ Wt,
+                                              const float* __restrict__ bias, const __bf16* __restrict__ act,
+                                              void* __restrict__ out, int N, int H, int W) {
+    constexpr int KP = (9 * CIN + 31) / 32 * 32;
+    constexpr int NT = COUT / 16;
+    const int P = N * H * W;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = (blockIdx.x * 4 + wave) * 32;
+    if (m0 >= P) return;  // whole wave out of range (no barriers below)
+    const int r16 = lane & 15, kq = lane >> 4;
+    int pn[2], py[2], px[2];
+    bool pv[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int p = m0 + 16 * j + r16;
+        pv[j] = p < P;
+        const int pp = pv[j] ? p : 0;
+        pn[j] = pp / (H * W);
+        const int rem = pp - pn[j] * H * W;
+        py[j] = rem / W;
+        px[j] = rem - py[j] * W;
+    }
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < KP; ks += 32) {
+        const int k = ks + 8 * kq;
+        const int tap = k / CIN, ci = k - tap * CIN;
+        bf16x8 a[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            a[j] = zero8();
+            if (tap < 9 && pv[j]) {
+                const int yy = py[j] + tap / 3 - 1, xx = px[j] + tap % 3 - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    a[j] = *reinterpret_cast<const bf16x8*>(X + ((size_t(pn[j]) * H + yy) * W + xx) * CIN + ci);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wt + size_t(t * 16 + r16) * KP + k);
+            acc[0][t] = mfma(a[0], b, acc[0][t]);
+            acc[1][t] = mfma(a[1], b, acc[1][t]);
+        }
+    }
+    // acc[j][t][r] is output pixel m0 + 16 j + 4 kq + r, channel 16 t + r16.
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + 16 * j + 4 * kq + r;
+                if (m >= P) continue;
+                const int co = 16 * t + r16;
+                const size_t o = size_t(m) * COUT + co;
+                const float v = acc[j][t][r];
+                if (MODE == 0) {
+                    static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v + bias[co], 0.0f);
+                } else if (MODE == 1) {
+                    static_cast<float*>(out)[o] = v + bias[co];
+                } else {
+                    static_cast<__bf16*>(out)[o] = float(act[o]) > 0.0f ? (__bf16)v : (__bf16)0.0f;
+                }
+            }
+}
+
+template <int CIN, int COUT>
+__global__ void k_flip(const __bf16* __restrict__ w, __bf16* __restrict__ wd) {
+    constexpr int KP = (9 * CIN + 31) / 32 * 32;
+    constexpr int KPD = (9 * COUT + 31) / 32 * 32;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over CIN * KPD
+    if (i >= CIN * KPD) return;
+    const int ci = i / KPD, k = i - ci * KPD;
+    __bf16 v = (__bf16)0.0f;
+    if (k < 9 * COUT) {
+        const int t = k / COUT, co = k - t * COUT;
+        v = w[size_t(co) * KP + (8 - t) * CIN + ci];
+    }
+    wd[i] = v;
+}
+
+constexpr int kWgradChunk = 2048;  // pixels per wgrad block
+
+// Weight gradient of one tap: C[ci][co] = sum_p x[p + off(tap)][ci] dz[p][co]
+// over a chunk of pixels; 32-pixel K steps staged transposed in LDS.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad(const __bf16* __restrict__ X, const __bf16* __restrict__ dZ,
+                                               float* __restrict__ partial, int N, int H, int W) {
+    constexpr int MT = CIN / 16, NT = COUT / 16, TILES = MT * NT, TPW = (TILES + 3) / 4;
+    constexpr int PITCH = 40;  // 32 pixels + 8 pad (keeps 16-byte alignment)
+    __shared__ __align__(16) __bf16 sx[CIN * PITCH];
+    __shared__ __align__(16) __bf16 sd[COUT * PITCH];
+    const int tap = blockIdx.y, dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int P = N * H * W;
+    const int p_begin = blockIdx.x * kWgradChunk, p_end = min(P, p_begin + kWgradChunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r16 = lane & 15, kq = lane >> 4;
+    f32x4 acc[TPW];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int p0 = p_begin; p0 < p_end; p0 += 32) {
+        for (int v = threadIdx.x; v < 32 * CIN / 8; v += 256) {
+            const int pl = v / (CIN / 8), cg = (v - pl * (CIN / 8)) * 8;
+            const int p = p0 + pl;
+            bf16x8 val = zero8();
+            if (p < p_end) {
+                const int n = p / (H * W), rem = p - n * H * W, y = rem / W, x = rem - y * W;
+                const int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    val = *reinterpret_cast<const bf16x8*>(X + ((size_t(n) * H + yy) * W + xx) * CIN + cg);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sx[(cg + j) * PITCH + pl] = val[j];
+        }
+        for (int v = threadIdx.x; v < 32 * COUT / 8; v += 256) {
+            const int pl = v / (COUT / 8), cg = (v - pl * (COUT / 8)) * 8;
+            const int p = p0 + pl;
+            bf16x8 val = zero8();
+            if (p < p_end) val = *reinterpret_cast<const bf16x8*>(dZ + size_t(p) * COUT + cg);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sd[(cg + j) * PITCH + pl] = val[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) {
+            const int tile = wave + 4 * i;
+            if (tile < TILES) {
+                const int mt = tile / NT, nt = tile - mt * NT;
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sx[(mt * 16 + r16) * PITCH + 8 * kq]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sd[(nt * 16 + r16) * PITCH + 8 * kq]);
+                acc[i] = mfma(a, b, acc[i]);
+            }
+        }
+        __syncthreads();
+    }
+    // partial[tap][chunk][co][ci]; acc[i][r] is ci = 16 mt + 4 kq + r, co = 16 nt + r16.
+    float* base = partial + (size_t(tap) * gridDim.x + blockIdx.x) * (CIN * COUT);
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+        const int tile = wave + 4 * i;
+        if (tile >= TILES) continue;
+        const int mt = tile / NT, nt = tile - mt * NT;
+        const int co = nt * 16 + r16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) base[co * CIN + mt * 16 + 4 * kq + r] = acc[i][r];
+    }
+}
+
+// grad_w[co][tap * CIN + ci] = sum over chunks of partial[tap][chunk][co][ci].
+template <int CIN, int COUT>
+__global__ void k_wgrad_reduce(const float* __restrict__ partial, float* __restrict__ grad_w, int chunks) {
+    constexpr int KP = (9 * CIN + 31) / 32 * 32;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // over 9 * COUT * CIN
+    if (i >= 9 * COUT * CIN) return;
+    const int tap = i / (COUT * CIN), rem = i - tap * COUT * CIN, co = rem / CIN, ci = rem - co * CIN;
+    const float* src = partial + size_t(tap) * chunks * (CIN * COUT) + co * CIN + ci;
+    float s = 0.0f;
+    for (int c = 0; c < chunks; ++c) s += src[size_t(c) * (CIN * COUT)];
+    grad_w[size_t(co) * KP + tap * CIN + ci] = s;
+}
+
+// Bias gradient partials: partial[chunk][co] = sum of dz over the chunk's pixels.
+template <int COUT>
+__global__ __launch_bounds__(256) void k_bgrad(const __bf16* __restrict__ dZ, float* __restrict__ partial, int P) {
+    constexpr int ROWS = 256 / COUT;
+    __shared__ float s[256];
+    const int c = threadIdx.x % COUT, row = threadIdx.x / COUT;
+    const int p_begin = blockIdx.x * kWgradChunk, p_end = min(P, p_begin + kWgradChunk);
+    float acc = 0.0f;
+    for (int p = p_begin + row; p < p_end; p += ROWS) acc += float(dZ[size_t(p) * COUT + c]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (row == 0) {
+        for (int r = 1; r < ROWS; ++r) acc += s[r * COUT + c];
+        partial[size_t(blockIdx.x) * COUT + c] = acc;
+    }
+}
+
+template <int COUT>
+__global__ void k_bgrad_reduce(const float* __restrict__ partial, float* __restrict__ grad_b, int chunks) {
+    const int c = threadIdx.x;
+    if (c >= COUT) return;
+    float s = 0.0f;
+    for (int k = 0; k < chunks; ++k) s += partial[size_t(k) * COUT + c];
+    grad_b[c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_kpn_apply(const float* __restrict__ logits, const float* __restrict__ rad,
+                                                   const float* __restrict__ ref, float* __restrict__ out,
+                                                   float4* __restrict__ out4, __bf16* __restrict__ dlogit,
+                                                   float* __restrict__ loss, float inv_count, int N, int H, int W) {
+    __shared__ float red[256];
+    const int P = N * H * W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float my_loss = 0.0f;
+    if (p < P) {
+        const int n = p / (H * W), rem = p - n * H * W, y = rem / W, x = rem - y * W;
+        const float* lg = logits + size_t(p) * 32;
+        float w[kKpnTaps];
+        float mx = lg[0];
+        for (int t = 1; t < kKpnTaps; ++t) mx = fmaxf(mx, lg[t]);
+        float sum = 0.0f;
+        for (int t = 0; t < kKpnTaps; ++t) {
+            w[t] = __expf(lg[t] - mx);
+            sum += w[t];
+        }
+        const float inv = 1.0f / sum;
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        for (int t = 0; t < kKpnTaps; ++t) {
+            w[t] *= inv;
+            const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
+            const float* r = rad + ((size_t(n) * H + yy) * W + xx) * 3;
+            o[0] += w[t] * r[0];
+            o[1] += w[t] * r[1];
+            o[2] += w[t] * r[2];
+        }
+        if (out) {
+            out[size_t(p) * 3] = o[0];
+            out[size_t(p) * 3 + 1] = o[1];
+            out[size_t(p) * 3 + 2] = o[2];
+        }
+        if (out4) out4[p] = make_float4(o[0], o[1], o[2], 0.0f);
+        if (ref) {
+            float g[3];
+            for (int c = 0; c < 3; ++c) {
+                const float d = log1pf(fmaxf(o[c], 0.0f)) - log1pf(fmaxf(ref[size_t(p) * 3 + c], 0.0f));
+                my_loss += d * d;
+                g[c] = 2.0f * d / (1.0f + fmaxf(o[c], 0.0f)) * inv_count;
+            }
+            float dw[kKpnTaps], dot = 0.0f;
+            for (int t = 0; t < kKpnTaps; ++t) {
+                const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
+                const float* r = rad + ((size_t(n) * H + yy) * W + xx) * 3;
+                dw[t] = g[0] * r[0] + g[1] * r[1] + g[2] * r[2];
+                dot += w[t] * dw[t];
+            }
+            __bf16* dl = dlogit + size_t(p) * 32;
+            for (int t = 0; t < kKpnTaps; ++t) dl[t] = (__bf16)(w[t] * (dw[t] - dot));
+            for (int t = kKpnTaps; t < 32; ++t) dl[t] = (__bf16)0.0f;
+        }
+    }
+    if (ref && loss) {
+        red[threadIdx.x] = my_loss;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_count);
+    }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       __bf16* __restrict__ pb, size_t count, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    p[i] = pi;
+    pb[i] = (__bf16)pi;
+}
+
+__global__ void k_to_bf16(const float* __restrict__ s, __bf16* __restrict__ d, size_t count) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) d[i] = (__bf16)s[i];
+}
+
+__global__ void k_features(FrameParams P, GBufferSoA gb, const float4* __restrict__ shade,
+                           const float* __restrict__ lum_var, float* __restrict__ feat, float* __restrict__ rad) {
+    const int px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= P.width * P.height) return;
+    const float4 g0 = gb.p0[px], g1 = gb.p1[px], g2 = gb.p2[px];
+    const int kind = __float_as_int(g0.w);
+    const float4 c = shade[px];
+    float f[kKpnFeat];
+    for (int i = 0; i < kKpnFeat; ++i) f[i] = 0.0f;
+    f[0] = log1pf(fmaxf(c.x, 0.0f));
+    f[1] = log1pf(fmaxf(c.y, 0.0f));
+    f[2] = log1pf(fmaxf(c.z, 0.0f));
+    if (kind != 0) {
+        f[3] = g2.x; f[4] = g2.y; f[5] = g2.z;
+        f[6] = g1.x; f[7] = g1.y; f[8] = g1.z;
+        const float dx = g0.x - P.cam.x, dy = g0.y - P.cam.y, dz = g0.z - P.cam.z;
+        f[9] = log1pf(sqrtf(dx * dx + dy * dy + dz * dz));
+        // Screen motion: where the hit projected through the previous camera.
+        if (P.has_history) {
+            const rm_camera& q = P.prev;
+            const float qx = g0.x - q.x, qy = g0.y - q.y, qz = g0.z - q.z;
+            const float cx = q.m0 * qx + q.m3 * qy + q.m6 * qz, cy = q.m1 * qx + q.m4 * qy + q.m7 * qz,
+                        cz = q.m2 * qx + q.m5 * qy + q.m8 * qz;
+            if (cz > 1e-6f) {
+                const float sx = q.focal * cx / cz + float(P.width) * 0.5f;
+                const float sy = float(P.height) * 0.5f - q.focal * cy / cz;
+                const float x = float(px % P.width) + 0.5f, y = float(px / P.width) + 0.5f;
+                f[11] = fminf(fmaxf((sx - x) * 0.125f, -8.0f), 8.0f);
+                f[12] = fminf(fmaxf((sy - y) * 0.125f, -8.0f), 8.0f);
+            }
+        }
+    }
+    f[10] = lum_var ? log1pf(fmaxf(lum_var[px], 0.0f)) : 0.0f;
+    for (int i = 0; i < kKpnFeat; ++i) feat[size_t(px) * kKpnFeat + i] = f[i];
+    rad[size_t(px) * 3] = fmaxf(c.x, 0.0f);
+    rad[size_t(px) * 3 + 1] = fmaxf(c.y, 0.0f);
+    rad[size_t(px) * 3 + 2] = fmaxf(c.z, 0.0f);
+}
+
+__global__ void k_gather(const __bf16* __restrict__ feat_all, const float* __restrict__ rad_all,
+                         const float* __restrict__ ref_all, const int32_t* __restrict__ idx, int batch, int ppi,
+                         __bf16* __restrict__ feat, float* __restrict__ rad, float* __restrict__ ref) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;  // over batch * ppi pixels
+    if (i >= size_t(batch) * ppi) return;
+    const int b = int(i / ppi);
+    const size_t src = size_t(idx[b]) * ppi + (i - size_t(b) * ppi);
+    const bf16x8* fs = reinterpret_cast<const bf16x8*>(feat_all + src * kKpnFeat);
+    bf16x8* fd = reinterpret_cast<bf16x8*>(feat + i * kKpnFeat);
+    fd[0] = fs[0];
+    fd[1] = fs[1];
+    for (int c = 0; c < 3; ++c) {
+        rad[i * 3 + c] = rad_all[src * 3 + c];
+        ref[i * 3 + c] = ref_all[src * 3 + c];
+    }
+}
+
+unsigned blocks_of(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, const float* b, const __bf16* act,
+                 void* out, int n, int h, int wdt, hipStream_t st) {
+    const size_t P = size_t(n) * h * wdt;
+    const dim3 grid(blocks_of(P, 128)), block(256);
+#define RM_CONV(CI, CO, MO) \
+    hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
+    if (cin == 16 && cout == 64 && mode == 0) RM_CONV(16, 64, 0);
+    else if (cin == 64 && cout == 64 && mode == 0) RM_CONV(64, 64, 0);
+    else if (cin == 64 && cout == 32 && mode == 1) RM_CONV(64, 32, 1);
+    else if (cin == 64 && cout == 64 && mode == 2) RM_CONV(64, 64, 2);
+    else if (cin == 32 && cout == 64 && mode == 2) RM_CONV(32, 64, 2);
+#undef RM_CONV
+}
+
+void launch_flip_weights(int cin, int cout, const __bf16* w, __bf16* wd, hipStream_t st) {
+    const int kpd = (9 * cout + 31) / 32 * 32;
+    const dim3 grid(blocks_of(size_t(cin) * kpd, 256)), block(256);
+    if (cin == 64 && cout == 64) hipLaunchKernelGGL((k_flip<64, 64>), grid, block, 0, st, w, wd);
+    else if (cin == 64 && cout == 32) hipLaunchKernelGGL((k_flip<64, 32>), grid, block, 0, st, w, wd);
+}
+
+size_t kpn_wgrad_scratch(int cin, int cout, int pixels) {
+    const size_t chunks = (size_t(pixels) + kWgradChunk - 1) / kWgradChunk;
+    return std::max(9 * chunks * size_t(cin) * cout, chunks * size_t(cout));
+}
+
+void launch_wgrad(int cin, int cout, const __bf16* x, const __bf16* dz, float* partial, float* grad_w, float* grad_b,
+                  int n, int h, int wdt, hipStream_t st) {
+    const int P = n * h * wdt;
+    const int chunks = (P + kWgradChunk - 1) / kWgradChunk;
+    const dim3 grid(chunks, 9), block(256);
+    const dim3 rgrid(blocks_of(size_t(9) * cin * cout, 256));
+#define RM_WGRAD(CI, CO)                                                                               \
+    do {                                                                                               \
+        hipLaunchKernelGGL((k_wgrad<CI, CO>), grid, block, 0, st, x, dz, partial, n, h, wdt);          \
+        hipLaunchKernelGGL((k_wgrad_reduce<CI, CO>), rgrid, dim3(256), 0, st, partial, grad_w, chunks); \
+        hipLaunchKernelGGL((k_bgrad<CO>), dim3(chunks), block, 0, st, dz, partial, P);                 \
+        hipLaunchKernelGGL((k_bgrad_reduce<CO>), dim3(1), dim3(64), 0, st, partial, grad_b, chunks);   \
+    } while (0)
+    if (cin == 16 && cout == 64) RM_WGRAD(16, 64);
+    else if (cin == 64 && cout == 64) RM_WGRAD(64, 64);
+    else if (cin == 64 && cout == 32) RM_WGRAD(64, 32);
+#undef RM_WGRAD
+}
+
+void launch_kpn_apply(const float* logits, const float* rad, const float* ref, float* out, float4* out4,
+                      __bf16* dlogit, float* loss, float inv_count, int n, int h, int wdt, hipStream_t st) {
+    const size_t P = size_t(n) * h * wdt;
+    hipLaunchKernelGGL(k_kpn_apply, dim3(blocks_of(P, 256)), dim3(256), 0, st, logits, rad, ref, out, out4, dlogit,
+                       loss, inv_count, n, h, wdt);
+}
+
+void launch_adam(float* p, const float* g, float* m, float* v, __bf16* pb, size_t count, float lr, float beta1,
+                 float beta2, float eps, int step, hipStream_t st) {
+    const float c1 = 1.0f / (1.0f - powf(beta1, float(step)));
+    const float c2 = 1.0f / (1.0f - powf(beta2, float(step)));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_of(count, 256)), dim3(256), 0, st, p, g, m, v, pb, count, lr, beta1, beta2,
+                       eps, c1, c2);
+}
+
+void launch_to_bf16(const float* src, __bf16* dst, size_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_of(count, 256)), dim3(256), 0, st, src, dst, count);
+}
+
+void launch_features(const FrameParams& P, const GBufferSoA& gb, const float4* shade, const float* lum_var,
+                     float* feat, float* rad, hipStream_t st) {
+    const size_t npx = size_t(P.width) * P.height;
+    hipLaunchKernelGGL(k_features, dim3(blocks_of(npx, 256)), dim3(256), 0, st, P, gb, shade, lum_var, feat, rad);
+}
+
+void launch_gather(const __bf16* feat_all, const float* rad_all, const float* ref_all, const int32_t* idx, int batch,
+                   int px_per_image, __bf16* feat, float* rad, float* ref, hipStream_t st) {
+    const size_t n = size_t(batch) * px_per_image;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_of(n, 256)), dim3(256), 0, st, feat_all, rad_all, ref_all, idx, batch,
+                       px_per_image, feat, rad, ref);
+}
+
+}  // namespace rmh

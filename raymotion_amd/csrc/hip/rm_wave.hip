@@ -56,6 +56,7 @@ struct PassIO {
     ReservoirSoA res;
     float4* hdr;
     uint8_t* rgb;
+    float* var;  // restir_shade: per-pixel sample variance of luminance (may be null)
 };
 
 template <int S, bool SHADE>
@@ -72,8 +73,18 @@ struct PathEngine {
     const ReservoirSoA& rm;
     float4* hdr;
     uint8_t* rgb;
+    float* lum_var;
     __device__ PathEngine(Tracer<S>& t, const FrameParams& p, const PassIO& io)
-        : pt(t), P(p), gb(io.gb), rm(io.res), hdr(io.hdr), rgb(io.rgb) {}
+        : pt(t), P(p), gb(io.gb), rm(io.res), hdr(io.hdr), rgb(io.rgb), lum_var(io.var) {}
+
+    __device__ void write_var(int px, float v) {
+        if (SHADE && lum_var) lum_var[px] = v;
+    }
+    __device__ void add_lum(Lane& L, V3 c) {  // luminance moments for lum_var
+        const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+        L.sl += lum;
+        L.sl2 += lum * lum;
+    }
 
     __device__ void set_ray(Lane& L, int phase, V3 o, V3 dir, float dist) {
         L.phase = phase;
@@ -183,6 +194,8 @@ struct PathEngine {
             V3 c{L.sr / fn, L.sg / fn, L.sb / fn};
             if (L.kind == 3) c = clamped(c.x, c.y, c.z);
             hdr[L.pixel] = make_float4(c.x, c.y, c.z, 0.0f);
+            const float n = float(L.n);
+            write_var(L.pixel, L.n > 1 ? fmaxf(0.0f, (L.sl2 - L.sl * L.sl / n) / (n - 1.0f)) : 0.0f);
         }
         L.phase = kIdle;
     }
@@ -195,9 +208,11 @@ struct PathEngine {
             if (L.kind == 1) {
                 const V3 full = clamped(L.dr + L.ar, L.dg + L.ag, L.db + L.ab);
                 L.sr += full.x; L.sg += full.y; L.sb += full.z;
+                if (SHADE && lum_var) add_lum(L, full);
             } else {
                 const V3 c = clamped(L.ar, L.ag, L.ab);
                 L.sr += c.x; L.sg += c.y; L.sb += c.z;
+                if (SHADE && lum_var) add_lum(L, c);
                 if (!SHADE && P.adapt_min > 0) {
                     const float lum = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
                     L.sl += lum;
@@ -237,6 +252,7 @@ struct PathEngine {
             L.kind = ibits(g0.w);
             if (L.kind == 0) {
                 hdr[px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                write_var(px, 0.0f);
                 L.phase = kIdle;
                 return kPixelDone;
             }
@@ -309,6 +325,7 @@ struct PathEngine {
         if (P.bounces <= 1 || P.spp <= 0) {
             const V3 c = clamped(L.dr, L.dg, L.db);
             hdr[px] = make_float4(c.x, c.y, c.z, 0.0f);
+            write_var(px, 0.0f);
             L.phase = kIdle;
             return false;
         }
@@ -752,9 +769,9 @@ void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& i
 }  // namespace
 
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
-                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, unsigned int* queue, unsigned long long* cnt,
-                 hipStream_t st) {
-    const PassIO io{gb, rm, hdr, rgb};
+                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
+                 unsigned long long* cnt, hipStream_t st) {
+    const PassIO io{gb, rm, hdr, rgb, lum_var};
     if (stack <= 32) {
         if (shade) launch_persistent<32, PathEngine<32, true>>(sc, P, io, queue, cnt, st);
         else launch_persistent<32, PathEngine<32, false>>(sc, P, io, queue, cnt, st);
@@ -766,7 +783,7 @@ void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P
 
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                      unsigned int* queue, unsigned long long* cnt, hipStream_t st) {
-    const PassIO io{gb, ri, nullptr, nullptr};
+    const PassIO io{gb, ri, nullptr, nullptr, nullptr};
     if (stack <= 32) launch_persistent<32, GenEngine<32>>(sc, P, io, queue, cnt, st);
     else launch_persistent<64, GenEngine<64>>(sc, P, io, queue, cnt, st);
 }

@@ -206,6 +206,60 @@ class Backend:
                                             ptr(grad)), "rm_render_adjoint")
         return grad
 
+    # ---- learned denoiser
+    def kpn_set_weights(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        self._check(lib().rm_kpn_set_weights(self._h, ptr(p), p.size), "rm_kpn_set_weights")
+
+    def kpn_get_weights(self) -> np.ndarray:
+        out = np.zeros(lib().rm_kpn_param_count(), np.float32)
+        self._check(lib().rm_kpn_get_weights(self._h, ptr(out), out.size), "rm_kpn_get_weights")
+        return out
+
+    def render_features(self, frame: Frame, seed: int):
+        """(features (H, W, 16), noisy radiance (H, W, 3)) of a ReSTIR frame."""
+        px = frame.width * frame.height
+        feat = np.zeros((frame.height, frame.width, 16), np.float32)
+        noisy = np.zeros((frame.height, frame.width, 3), np.float32)
+        self._check(lib().rm_render_features(self._h, ctypes.byref(frame), seed, ptr(feat), ptr(noisy)),
+                    "rm_render_features")
+        assert feat.size == px * 16
+        return feat, noisy
+
+    def kpn_dataset_clear(self):
+        self._check(lib().rm_kpn_dataset_clear(self._h), "rm_kpn_dataset_clear")
+
+    def kpn_dataset_add(self, feat, noisy, reference):
+        f = np.ascontiguousarray(feat, np.float32)
+        h, w = f.shape[:2]
+        n = np.ascontiguousarray(noisy, np.float32)
+        r = np.ascontiguousarray(reference, np.float32)
+        self._check(lib().rm_kpn_dataset_add(self._h, ptr(f), ptr(n), ptr(r), w, h), "rm_kpn_dataset_add")
+
+    def kpn_train(self, steps: int, batch: int, lr: float, seed: int) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(lib().rm_kpn_train(self._h, steps, batch, lr, seed, ptr(losses)), "rm_kpn_train")
+        return losses[:steps]
+
+    def kpn_denoise(self, feat, noisy) -> np.ndarray:
+        f = np.ascontiguousarray(feat, np.float32)
+        h, w = f.shape[:2]
+        out = np.zeros((h, w, 3), np.float32)
+        self._check(lib().rm_kpn_denoise(self._h, ptr(f), ptr(np.ascontiguousarray(noisy, np.float32)), w, h,
+                                         ptr(out)), "rm_kpn_denoise")
+        return out
+
+    def kpn_gradient(self, feat, noisy, reference):
+        """(loss, gradient) of the current weights on images (N, H, W, C)."""
+        f = np.ascontiguousarray(feat, np.float32)
+        n, h, w = f.shape[:3]
+        grad = np.zeros(lib().rm_kpn_param_count(), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(lib().rm_kpn_gradient(self._h, ptr(f), ptr(np.ascontiguousarray(noisy, np.float32)),
+                                          ptr(np.ascontiguousarray(reference, np.float32)), n, w, h, ptr(grad),
+                                          ptr(loss)), "rm_kpn_gradient")
+        return float(loss[0]), grad
+
     def read_restir(self, width: int, height: int):
         px = width * height
         g = np.zeros(px, GBUFFER_DTYPE)

@@ -1,0 +1,125 @@
+"""GPU kernel-predicting denoiser against the PyTorch fp32 reference of the same
+network (bf16 rounding of features, weights and activations emulated), and
+its use inside the frame.
+
+Bars: denoised output within 5e-3 relative at most and 2e-4 on average (an
+activation whose FP32 sum straddles a bf16 rounding boundary moves by one bf16
+ulp, 2^-8); parameter gradient within 2e-2 relative in norm and
+cosine similarity above 0.999 per layer (the GPU keeps the backward's
+activation gradients in bf16, the reference in fp32).
+"""
+import numpy as np
+import pytest
+
+import kpn_reference as ref
+from helpers import frame
+from raymotion_amd import denoise, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(rng, n, h, w):
+    feat = np.zeros((n, h, w, 16), np.float32)
+    feat[..., :13] = rng.normal(0.0, 1.0, (n, h, w, 13))
+    noisy = rng.gamma(1.0, 0.5, (n, h, w, 3)).astype(np.float32)
+    target = (noisy * rng.uniform(0.5, 1.5, (n, h, w, 3))).astype(np.float32)
+    return feat, noisy, target
+
+
+def _params(seed):
+    p = denoise.init_params(seed)
+    # a head with visible weights, so the softmax is not near uniform
+    L = denoise.layout()[-1]
+    rng = np.random.default_rng(seed + 1)
+    w = p[L.w_off:L.w_off + L.cout * L.kp].reshape(L.cout, L.kp)
+    w[:25, :9 * L.cin] = rng.normal(0, 0.05, (25, 9 * L.cin))
+    p[L.b_off:L.b_off + 25] = rng.normal(0, 0.5, 25)
+    return p
+
+
+def test_forward_matches_torch(backend):
+    rng = np.random.default_rng(0)
+    feat, noisy, _ = _inputs(rng, 1, 37, 53)  # not a multiple of the tiles
+    p = _params(3)
+    backend.kpn_set_weights(p)
+    g = backend.kpn_denoise(feat[0], noisy[0])
+    r = ref.forward(p, feat, noisy).detach().numpy()[0]
+    err = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
+    print(f"kpn forward: max rel err {err.max():.3g}, mean {err.mean():.3g}")
+    # An activation whose FP32 sums (GPU vs torch order) straddle a bf16 rounding
+    # boundary differs by one bf16 ulp (2^-8 relative) downstream.
+    assert err.max() <= 5e-3
+    assert err.mean() <= 2e-4
+
+
+def test_gradient_matches_torch(backend):
+    rng = np.random.default_rng(1)
+    feat, noisy, target = _inputs(rng, 2, 24, 40)
+    p = _params(5)
+    backend.kpn_set_weights(p)
+    loss, g = backend.kpn_gradient(feat, noisy, target)
+    rloss, rg = ref.gradient(p, feat, noisy, target)
+    assert abs(loss - rloss) <= 1e-4 * abs(rloss)
+    for i, L in enumerate(denoise.layout()):
+        for name, a, b in (("w", L.w_off, L.w_off + L.cout * L.kp), ("b", L.b_off, L.b_off + L.cout)):
+            x, y = g[a:b].astype(np.float64), rg[a:b].astype(np.float64)
+            rel = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
+            cos = float(x @ y / max(np.linalg.norm(x) * np.linalg.norm(y), 1e-30))
+            print(f"layer {i} {name}: rel {rel:.3g} cos {cos:.6f}")
+            assert rel <= 2e-2 and cos >= 0.999, (i, name)
+
+
+def test_training_step_reduces_loss(backend):
+    rng = np.random.default_rng(2)
+    feat, noisy, target = _inputs(rng, 4, 32, 32)
+    backend.kpn_dataset_clear()
+    for i in range(4):
+        backend.kpn_dataset_add(feat[i], noisy[i], target[i])
+    backend.kpn_set_weights(denoise.init_params(7))
+    losses = backend.kpn_train(60, 2, 3e-3, 11)
+    print(f"loss {losses[:5].mean():.5f} -> {losses[-5:].mean():.5f}")
+    assert np.all(np.isfinite(losses))
+    assert losses[-5:].mean() < losses[:5].mean()
+
+
+def test_learned_denoise_in_frame(backend):
+    """denoise = 2 inside rm_render equals the network applied to the frame's
+    exported features."""
+    s = scenes.mixed_spheres(seed=3, n_spheres=12, subdivisions=2)
+    s.build_bvh()
+    backend.set_scene(s)
+    backend.kpn_set_weights(_params(9))
+    cam = scenes.mixed_camera(2)
+    f = denoise.restir_frame(96, 64, 4, cam, denoise=0, temporal=0)
+    feat, noisy = backend.render_features(f, 77)
+    expect = backend.kpn_denoise(feat, noisy)
+    f2 = denoise.restir_frame(96, 64, 4, cam, denoise=2, temporal=0)
+    _, hdr = backend.render(f2, 77)
+    assert backend.stats().path == 1
+    np.testing.assert_array_equal(hdr.reshape(64, 96, 3), expect)
+    # features: radiance channels are log(1 + noisy), normals unit or zero
+    np.testing.assert_allclose(feat[..., :3], np.log1p(noisy), rtol=1e-6, atol=1e-6)
+    nrm = np.linalg.norm(feat[..., 6:9], axis=-1)
+    assert np.all((np.abs(nrm - 1) < 1e-3) | (nrm == 0))
+
+
+WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent.parent / "raymotion_amd" / "data" / "kpn_mixed.npy"
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="trained weights not committed")
+def test_trained_network_beats_atrous(backend):
+    """On a held-out scene and view, the trained network's single-frame output
+    is closer to a converged render than the a-trous filter's (same 4-spp
+    ReSTIR input, no temporal history) and than the noisy input."""
+    s = scenes.mixed_spheres(seed=987654321, subdivisions=3)
+    s.build_bvh()
+    backend.set_scene(s)
+    backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
+    cam = scenes.orbit_camera((0.0, 0.0, 0.0), 9.0, 2.5, 123.0, 45.0)
+    _, ref = backend.render(denoise.reference_frame(256, 256, 2048, cam), 5)
+    err = {}
+    for name, mode in (("noisy", 0), ("atrous", 1), ("kpn", 2)):
+        _, hdr = backend.render(denoise.restir_frame(256, 256, 4, cam, denoise=mode, temporal=0), 77)
+        err[name] = denoise.tonemapped_rmse(hdr, ref)
+    print("held-out tonemapped RMSE:", {k: round(v, 5) for k, v in err.items()})
+    assert err["kpn"] < err["atrous"] < err["noisy"]
