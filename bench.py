@@ -61,6 +61,32 @@ PIXEL_BYTES = {  # per pixel and launch: reads + writes of the pass's own buffer
 }
 
 
+# HBM bytes per launch come from rocprofv3 PMC runs of this same command
+# (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings), committed as
+# a table in this summary; bench.py cannot run the profiler itself.
+PROFILE = ROOT / "profiles" / "restir_shade_c1_720p16spp.md"
+PROFILE_KERNEL = {"restir_shade": "k_persistent<32, PathEngine<32, true> >",
+                  "restir_generate": "k_persistent<32, GenEngine<32> >", "restir_spatial": "k_spatial<32>"}
+
+
+def measured_traffic(name: str):
+    """HBM bytes (read + write) per launch of pass `name` from the committed PMC
+    table, or None."""
+    if not PROFILE.exists() or name not in PROFILE_KERNEL:
+        return None
+    rows = PROFILE.read_text().splitlines()
+    start = next((i for i, l in enumerate(rows) if "FETCH KiB" in l), None)
+    if start is None:
+        return None
+    for line in rows[start + 2:]:
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) != 5:
+            break
+        if cells[0] == PROFILE_KERNEL[name]:
+            return int((float(cells[3]) + float(cells[4])) * 1e6)
+    return None
+
+
 def frame_settings(frame_index: int) -> rm.Frame:
     f = rm.Frame()
     f.width, f.height, f.spp, f.bounces = 1280, 720, 16, 8
@@ -306,14 +332,14 @@ def run_denoiser_train(args) -> dict:
 
 
 def denoiser_extras() -> dict | None:
-    """Equal-RMSE comparison with the committed trained weights (one evaluated
-    frame of the mixed-scene orbit)."""
+    """Equal-RMSE comparison with the committed trained weights (frames 4 and 20
+    of the mixed-scene orbit, 4096-spp references)."""
     from raymotion_amd import denoise
     if not WEIGHTS.exists():
         return None
     backend = rm.Backend(0)
     backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
-    ev = denoise.evaluate(backend, frames=(4,), svgf_spp=(4, 8, 16, 32))
+    ev = denoise.evaluate(backend, frames=(4, 20), svgf_spp=(4, 8, 16, 32, 64))
     backend.close()
     return ev
 
@@ -468,7 +494,7 @@ def run_mixed(args) -> dict:
                    "pass_tris_per_ray": {tracing[i]: round(float(tr[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "gpu_frame_ms": round(float(pm[7]), 3)},
         "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                      "bytes_per_launch": int(alg_bytes), "kernel_ms": round(float(pm[dom]), 3)},
     }
     if not args.no_cpu_baseline:

@@ -173,23 +173,25 @@ def evaluate(backend, frames=(4,), ref_spp: int = 4096, history: int = 4, svgf_s
     svgf = [run(spp, 1) for spp in svgf_spp]
     cnn = [run(spp, 2) for spp in cnn_spp]
     out = {"frames": list(frames), "reference_spp": ref_spp, "svgf": svgf, "cnn": cnn}
-    # a-trous cost at the learned path's RMSE: log-log interpolation over spp
+    # a-trous cost at the learned path's error: log-log interpolation over spp,
+    # for linear and for tonemapped RMSE
     c = cnn[0]
-    e = np.array([r["rmse_tonemapped"] for r in svgf])
     t = np.array([r["ms"] for r in svgf])
-    ms_eq, bound = None, None
-    if c["rmse_tonemapped"] >= e[0]:
-        ms_eq = float(t[0])  # already matched at the lowest sample count swept
-    for i in range(len(svgf) - 1):
-        if ms_eq is None and e[i] >= c["rmse_tonemapped"] >= e[i + 1]:
-            a = (math.log(c["rmse_tonemapped"]) - math.log(e[i])) / (math.log(e[i + 1]) - math.log(e[i]))
-            ms_eq = float(math.exp(math.log(t[i]) + a * (math.log(t[i + 1]) - math.log(t[i]))))
-    if ms_eq is None:
-        bound = float(t[-1])  # not reached within the sweep: its cost is a lower bound
-    out["svgf_ms_at_cnn_rmse"] = ms_eq
-    out["speedup_at_equal_rmse"] = None if ms_eq is None else round(ms_eq / c["ms"], 3)
-    out["svgf_ms_lower_bound"] = bound
-    out["speedup_lower_bound"] = None if bound is None else round(bound / c["ms"], 3)
+    out["equal_rmse"] = {}
+    for key in ("rmse", "rmse_tonemapped"):
+        e = np.array([r[key] for r in svgf])
+        ms_eq, bound = None, None
+        if c[key] >= e[0]:
+            ms_eq = float(t[0])  # already matched at the lowest sample count swept
+        for i in range(len(svgf) - 1):
+            if ms_eq is None and e[i] >= c[key] >= e[i + 1]:
+                a = (math.log(c[key]) - math.log(e[i])) / (math.log(e[i + 1]) - math.log(e[i]))
+                ms_eq = float(math.exp(math.log(t[i]) + a * (math.log(t[i + 1]) - math.log(t[i]))))
+        if ms_eq is None:
+            bound = float(t[-1])  # not reached within the sweep: its cost is a lower bound
+        ms = ms_eq if ms_eq is not None else bound
+        out["equal_rmse"][key] = {"svgf_ms": round(ms, 3), "cnn_ms": c["ms"], "speedup": round(ms / c["ms"], 3),
+                                  "lower_bound": ms_eq is None}
     return out
 
 

@@ -194,7 +194,7 @@ def optimize(renderer: Renderer, target, p0, steps: int = 200, lr: float = 1e-2,
         opt.step()
         with torch.no_grad():
             p.copy_(torch.as_tensor(renderer.scene.clamp(p.detach().numpy()), dtype=torch.float32))
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
         if log_every and (it % log_every == 0 or it == steps - 1):
             print(f"iter {it}: loss {losses[-1]:.6g}", flush=True)
     return p.detach().numpy().astype(np.float64), np.array(losses)

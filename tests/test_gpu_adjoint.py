@@ -121,7 +121,7 @@ def test_inverse_rendering_recovers_parameters(backend):
     target = hi.forward(gt, 1)
     rng = np.random.default_rng(2)
     p0 = r.scene.clamp(gt * rng.uniform(0.7, 1.3, gt.shape))
-    p, losses = optimize(r, target, p0, steps=80, lr=1e-2)
+    p, losses = optimize(r, target, p0, steps=80, lr=1e-2, final_lr_fraction=1.0)  # constant lr for a short run
 
     def image_error(q):
         return float(np.mean((_blocks(hi.forward(q, 2)) - _blocks(target)) ** 2))
