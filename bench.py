@@ -395,6 +395,8 @@ def main() -> int:
     ap.add_argument("--save-weights", default=None, help="denoiser-train: write the trained weights (.npy)")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint cost measurement in the mixed line")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
+    ap.add_argument("--schedule", type=int, choices=(0, 1, 2), default=0,
+                    help="path passes: 0 persistent lanes, 1 wavefront + persistent tail, 2 wavefront only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy kernel (scales HBM counter readings)")
     args = ap.parse_args()
@@ -438,16 +440,18 @@ def run_mixed(args) -> dict:
     backend = rm.Backend(0)
     backend.set_scene(exported)
     backend.set_profiling(True)
+    backend.set_schedule(args.schedule)
 
     for k in range(args.warmup):
         backend.render(frame_settings(k), 12345 + k, want_hdr=False)
-    wall, rays, pass_ms, nodes, tris, prays = [], [], [], [], [], []
+    wall, rays, pass_ms, nodes, tris, prays, waves = [], [], [], [], [], [], []
     for k in range(args.warmup, args.warmup + args.steps):
         f = frame_settings(k)
         t0 = time.perf_counter()
         backend.render(f, 12345 + k, want_hdr=False)
         wall.append((time.perf_counter() - t0) * 1e3)
         st = backend.stats()
+        waves.append((st.wave_iters, st.wave_tail))
         rays.append(st.rays[5])
         pass_ms.append([st.ms[i] for i in range(8)])
         nodes.append([st.node_visits[i] for i in range(6)])
@@ -492,7 +496,10 @@ def run_mixed(args) -> dict:
                    "pass_rays": {tracing[i]: int(ry[i]) for i in tracing},
                    "pass_nodes_per_ray": {tracing[i]: round(float(nd[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "pass_tris_per_ray": {tracing[i]: round(float(tr[i] / max(ry[i], 1.0)), 2) for i in tracing},
-                   "gpu_frame_ms": round(float(pm[7]), 3)},
+                   "gpu_frame_ms": round(float(pm[7]), 3),
+                   "schedule": {0: "persistent", 1: "wavefront", 2: "wavefront-only"}[args.schedule],
+                   "wavefront_iterations": round(float(np.mean([w[0] for w in waves])), 1),
+                   "wavefront_tail_pixels": int(np.mean([w[1] for w in waves]))},
         "roofline": {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                      "bytes_per_launch": int(alg_bytes), "kernel_ms": round(float(pm[dom]), 3)},

@@ -115,6 +115,8 @@ typedef struct {
     int32_t bvh_nodes;    /* 4-wide nodes resident */
     int32_t stack_depth;  /* traversal stack entries the resident tree needs */
     float scene_ms[2];    /* last rm_set_scene, GPU time: host-to-device copies, refit kernels */
+    int32_t wave_iters;   /* wavefront schedule: trace + step iterations of the path pass */
+    uint32_t wave_tail;   /* active pixels then finished by the persistent loop */
 } rm_stats;
 
 typedef struct rm_ctx rm_ctx;
@@ -191,6 +193,13 @@ int rm_kpn_denoise(rm_ctx* ctx, const float* features, const float* noisy, int w
 /* Loss and parameter gradient of the current weights on `count` images. */
 int rm_kpn_gradient(rm_ctx* ctx, const float* features, const float* noisy, const float* reference, int count,
                     int width, int height, float* grad, float* loss);
+
+/* Schedule of the path passes (restir_shade, render_paths with one lane per
+ * pixel): 0 (default) persistent lanes; 1 wavefront kernels over path states
+ * kept in device memory, the last pixels finished by persistent lanes; 2
+ * wavefront kernels to the end. All give identical images. No counterpart in
+ * the reference, whose passes run one thread per pixel. */
+int rm_set_schedule(rm_ctx* ctx, int schedule);
 
 /* Per-pass GPU timing with HIP events (off by default). */
 int rm_set_profiling(rm_ctx* ctx, int enable);

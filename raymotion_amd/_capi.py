@@ -79,7 +79,7 @@ class Stats(ctypes.Structure):
         ("refit", ctypes.c_int32),
         ("rays", ctypes.c_uint64 * 6), ("node_visits", ctypes.c_uint64 * 6), ("tri_tests", ctypes.c_uint64 * 6),
         ("ms", ctypes.c_float * 8), ("bvh_nodes", ctypes.c_int32), ("stack_depth", ctypes.c_int32),
-        ("scene_ms", ctypes.c_float * 2),
+        ("scene_ms", ctypes.c_float * 2), ("wave_iters", ctypes.c_int32), ("wave_tail", ctypes.c_uint32),
     ]
 
 
@@ -106,6 +106,7 @@ _SIGNATURES = [
     ("rm_kpn_denoise", _I, [_P, _P, _P, _I, _I, _P]),
     ("rm_kpn_gradient", _I, [_P, _P, _P, _P, _I, _I, _I, _P, _P]),
     ("rm_set_profiling", _I, [_P, _I]),
+    ("rm_set_schedule", _I, [_P, _I]),
     ("rm_get_stats", _I, [_P, ctypes.POINTER(Stats)]),
     ("rm_scene_create", _P, []),
     ("rm_scene_destroy", None, [_P]),

@@ -83,6 +83,7 @@ struct rm_ctx {
     std::mutex mu;
     std::string err;
     bool profiling = false;
+    int schedule = 0;  // path passes: 0 persistent lanes, 1 wavefront, 2 wavefront without the persistent tail
     hipEvent_t ev[16] = {};
 
     // Resident scene.
@@ -106,7 +107,14 @@ struct rm_ctx {
     DevBuf shade, den_a, hist_color;  // float4 per pixel
     DevBuf rgb8, hdr3, mega_hdr;
     DevBuf counters;
-    DevBuf queue;  // work counters of the persistent passes: [0] generate, [1] shade / megakernel
+    DevBuf queue;  // work counters of the persistent passes: [0] generate, [1] shade / megakernel, [2..3] shade order
+    DevBuf shade_order;  // restir_shade work order, one pixel index per entry
+    // Wavefront schedule: two slot-state buffers, hits, control words, and a
+    // pinned read-back of the active count with its events.
+    DevBuf wave_state[2], wave_hits, wave_ctl;
+    unsigned* wave_host = nullptr;
+    hipEvent_t wave_ev[2] = {};
+    size_t wave_cap = 0;
     DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
     DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
     std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
@@ -214,6 +222,64 @@ bool upload(rm_ctx* c, DevBuf& b, const std::vector<T>& v) {
     return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;
 }
 
+// Wavefront buffers for `pixels` slots.
+bool wave_ensure(rm_ctx* c, size_t pixels, WaveBufs& wb) {
+    const size_t bytes = wave_slot_words(true) * pixels * sizeof(uint32_t);
+    if (!c->wave_state[0].ensure(bytes) || !c->wave_state[1].ensure(bytes) ||
+        !c->wave_hits.ensure(pixels * sizeof(uint4)) || !c->wave_ctl.ensure(4 * sizeof(unsigned)))
+        return false;
+    if (!c->wave_host) {
+        void* h = nullptr;
+        if (hipHostMalloc(&h, 2 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess) return false;
+        c->wave_host = static_cast<unsigned*>(h);
+    }
+    for (auto& e : c->wave_ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            return false;
+        }
+    wb.state[0] = c->wave_state[0].as<uint32_t>();
+    wb.state[1] = c->wave_state[1].as<uint32_t>();
+    wb.hits = c->wave_hits.as<uint4>();
+    wb.ctl = c->wave_ctl.as<unsigned>();
+    wb.host = c->wave_host;
+    wb.ev[0] = c->wave_ev[0];
+    wb.ev[1] = c->wave_ev[1];
+    wb.stride = pixels;
+    wb.tail = c->schedule != 2;
+    return true;
+}
+
+// Path engine pass (restir_shade or render_paths with one lane per pixel) on
+// the context's schedule.
+int path_pass(rm_ctx* c, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
+              const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned long long* cnt,
+              rm_stats& st) {
+    if (c->schedule == 0) {
+        unsigned* order = nullptr;
+        if (shade) {
+            if (!c->shade_order.ensure(size_t(P.width) * size_t(P.height) * sizeof(unsigned))) {
+                c->drop_history();
+                return c->fail(RM_ERR_ALLOC, "HIP work-order buffer allocation failed");
+            }
+            order = c->shade_order.as<unsigned>();
+        }
+        launch_wave(c->stack_need, shade, sc, P, gb, rm, hdr, rgb, lum_var, c->queue.as<unsigned>() + 1, order, cnt,
+                    c->stream);
+        return RM_OK;
+    }
+    WaveBufs wb{};
+    if (!wave_ensure(c, size_t(P.width) * size_t(P.height), wb)) {
+        c->drop_history();
+        return c->fail(RM_ERR_ALLOC, "HIP wavefront buffer allocation failed");
+    }
+    WaveRun wr{};
+    const int e = run_wave(c->stack_need, shade, sc, P, gb, rm, hdr, rgb, lum_var, wb, cnt, c->stream, &wr);
+    st.wave_iters = wr.iters;
+    st.wave_tail = wr.tail;
+    return check(c, hipError_t(e), shade ? "restir_shade" : "render_paths");
+}
+
 // Activation buffers for a forward (and backward) pass over `pixels` pixels.
 bool kpn_ensure(rm_ctx* c, size_t pixels, bool train) {
     const KpnNet net = kpn_net();
@@ -309,7 +375,7 @@ int rm_create(rm_ctx** out, int hip_device) {
             c->profiling = false;
         }
     }
-    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(2 * sizeof(unsigned))) {
+    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(4 * sizeof(unsigned))) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         g_create_error = "HIP counter allocation failed";
@@ -326,8 +392,8 @@ void rm_destroy(rm_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     DevBuf* bufs[] = {&c->nodes, &c->geo, &c->nrm, &c->nn, &c->matid, &c->mats, &c->uv, &c->lights, &c->cdf,
                       &c->texels, &c->level_nodes, &c->shade, &c->den_a, &c->hist_color, &c->rgb8, &c->hdr3,
-                      &c->mega_hdr, &c->counters, &c->queue, &c->probe_o, &c->probe_d, &c->probe_t, &c->probe_tri,
-                      &c->probe_tuv, &c->adj_img, &c->adj_tri_group, &c->adj_light_group, &c->adj_base,
+                      &c->mega_hdr, &c->counters, &c->queue, &c->shade_order, &c->probe_o, &c->probe_d,
+                      &c->probe_t, &c->probe_tri, &c->probe_tuv, &c->adj_img, &c->adj_tri_group, &c->adj_light_group, &c->adj_base,
                       &c->adj_grad, &c->adj_bad};
     for (DevBuf* b : bufs) b->release();
     DevBuf* kbufs[] = {&c->kpn_p, &c->kpn_pb, &c->kpn_g, &c->kpn_m, &c->kpn_v, &c->kpn_wd, &c->kpn_logits,
@@ -343,6 +409,12 @@ void rm_destroy(rm_ctx* c) {
     c->hist_res.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& b : c->wave_state) b.release();
+    c->wave_hits.release();
+    c->wave_ctl.release();
+    for (auto& e : c->wave_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->wave_host) (void)hipHostFree(c->wave_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -358,6 +430,13 @@ int rm_set_profiling(rm_ctx* c, int enable) {
     for (auto& e : c->ev)
         if (enable && !e) return c->fail(RM_ERR_DEVICE, "HIP events unavailable for profiling");
     c->profiling = enable != 0;
+    return RM_OK;
+}
+
+int rm_set_schedule(rm_ctx* c, int schedule) {
+    if (!c || schedule < 0 || schedule > 2) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->schedule = schedule;
     return RM_OK;
 }
 
@@ -509,7 +588,7 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
     unsigned long long* cnt = c->counters.as<unsigned long long>();
     if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
         return rc;
-    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, 2 * sizeof(unsigned), c->stream), "work queue")) return rc;
+    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, 4 * sizeof(unsigned), c->stream), "work queue")) return rc;
     rm_stats st{};
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
@@ -542,8 +621,9 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
             c->drop_history();
             return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
         }
-        launch_wave(c->stack_need, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr,
-                    features ? c->kpn_var.as<float>() : nullptr, c->queue.as<unsigned>() + 1, cnt, c->stream);
+        if (int rc = path_pass(c, true, sc, P, gb, rmg, c->shade.as<float4>(), nullptr,
+                               features ? c->kpn_var.as<float>() : nullptr, cnt, st))
+            return rc;
         mark(3);
         const float4* final_img = c->shade.as<float4>();
         bool final_in_hist = false;
@@ -625,10 +705,11 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
             c->drop_history();
             return c->fail(RM_ERR_ALLOC, "HIP image buffer allocation failed");
         }
-        if (lanes == 1)
-            launch_wave(c->stack_need, false, sc, P, GBufferSoA{}, ReservoirSoA{}, hdr ? c->mega_hdr.as<float4>() : nullptr,
-                        c->rgb8.as<uint8_t>(), nullptr, c->queue.as<unsigned>() + 1, cnt, c->stream);
-        else
+        if (lanes == 1) {
+            if (int rc = path_pass(c, false, sc, P, GBufferSoA{}, ReservoirSoA{},
+                                   hdr ? c->mega_hdr.as<float4>() : nullptr, c->rgb8.as<uint8_t>(), nullptr, cnt, st))
+                return rc;
+        } else
             launch_render_paths(c->stack_need, lanes, sc, P, c->rgb8.as<uint8_t>(),
                                 hdr ? c->mega_hdr.as<float4>() : nullptr, cnt, c->stream);
         mark(1);

@@ -21,11 +21,33 @@ void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const G
                     hipStream_t st);
 // Persistent path engine: restir_shade (shade = true, writes hdr and, if not
 // null, the per-pixel sample variance of luminance) or render_paths with one
-// lane per pixel (writes rgb, and hdr if not null). `queue` must hold 0 when
-// the kernel starts.
+// lane per pixel (writes rgb, and hdr if not null). queue[0] (and, for
+// restir_shade with `order`, queue[1] and queue[2]) must hold 0 when the pass
+// starts. With `order` (width * height words), restir_shade queues pixels by
+// expected chain length first.
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
                  const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
-                 unsigned long long* cnt, hipStream_t st);
+                 unsigned int* order, unsigned long long* cnt, hipStream_t st);
+// Wavefront variant of launch_wave: the same per-pixel results from a lean
+// trace kernel and a state-machine kernel over path states kept in HBM.
+struct WaveBufs {
+    uint32_t* state[2];  // wave_slot_words(shade) * stride words each
+    uint4* hits;         // stride entries
+    unsigned int* ctl;   // 4 control words
+    unsigned int* host;  // 2 words of pinned host memory
+    hipEvent_t ev[2];
+    size_t stride;       // slot capacity, at least width * height
+    bool tail;           // hand the last slots to the persistent loop (else iterate to the end)
+};
+struct WaveRun {
+    int iters;      // trace + step iterations launched
+    unsigned tail;  // active slots (last read-back) handed to the persistent loop
+};
+size_t wave_slot_words(bool shade);
+// Returns a hipError_t code; waits on the stream a few times.
+int run_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
+             const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, const WaveBufs& wb,
+             unsigned long long* cnt, hipStream_t st, WaveRun* run);
 void launch_temporal(const FrameParams& P, const GBufferSoA& gb, const GBufferSoA& hg, const float4* in,
                      const float4* hist, float4* out, hipStream_t st);
 void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in, float4* out, int step, hipStream_t st);

@@ -21,6 +21,18 @@ namespace rmh {
 namespace {
 
 enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2 };
+
+// Position for the calling lane (when keep) in a list grown through *count,
+// one atomic per wave; undefined without keep.
+__device__ __forceinline__ unsigned wave_append_pos(bool keep, unsigned* count) {
+    const unsigned long long m = __ballot(keep);
+    if (!m) return 0u;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (int(__lane_id()) == leader) base = atomicAdd(count, unsigned(__popcll(m)));
+    base = __shfl(base, leader);
+    return base + __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
+}
 enum Step : int { kTrace = 0, kPathEnd = 1, kPixelDone = 2 };
 
 struct PathLane {
@@ -67,6 +79,7 @@ struct PathEngine {
     // per SIMD: a little spilling, measured faster than 2 waves without). A
     // 64-entry stack takes 64 KiB of LDS per block, which allows 2.
     static constexpr int kMinBlocks = S <= 32 ? 3 : 2;
+    static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -697,13 +710,21 @@ struct GenEngine {
 // The persistent loop shared by the engines: every iteration traces one ray
 // per active lane, then advances each lane; idle lanes take new pixels from
 // `queue` (one atomic per wave) until the image is exhausted.
+// With `order`, queue position i holds pixel order[i] for i < order_n[0] and
+// order[npx - 1 - (i - order_n[0])] after that, order_n[0] + order_n[1]
+// pixels in all; without it, pixel i.
 template <int S, class E>
 __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene sc, FrameParams P, PassIO io,
-                                                        unsigned int* __restrict__ queue, unsigned long long* cnt) {
+                                                        unsigned int* __restrict__ queue,
+                                                        const unsigned int* __restrict__ order,
+                                                        const unsigned int* __restrict__ order_n,
+                                                        unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     E eng(pt, P, io);
-    const int npx = P.width * P.height;
+    const int npx_all = P.width * P.height;
+    const unsigned front = order ? order_n[0] : 0u;
+    const int npx = order ? int(front + order_n[1]) : npx_all;
     const unsigned lane = __lane_id();
     typename E::Lane L;
     L.phase = kIdle;
@@ -720,8 +741,9 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                 if (L.phase == kIdle) {
                     const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(need >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo(unsigned(need), 0u));
-                    const unsigned px = base + rank;
-                    if (px < unsigned(npx)) {
+                    const unsigned i = base + rank;
+                    if (i < unsigned(npx)) {
+                        const unsigned px = !order ? i : i < front ? order[i] : order[unsigned(npx_all) - 1u - (i - front)];
                         L.phase = kExt;  // start() sets the real phase
                         if (eng.start(L, int(px)) != kTrace) L.phase = kIdle;
                     }
@@ -756,36 +778,347 @@ int resident_blocks(const void* kernel) {
 // Launches k_persistent<S, E> with one resident wave set (cached per instance).
 template <int S, class E>
 void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& io, unsigned* queue,
-                       unsigned long long* cnt, hipStream_t st) {
+                       const unsigned* order, const unsigned* order_n, unsigned long long* cnt, hipStream_t st) {
     static int resident = 0;
     const void* kernel = reinterpret_cast<const void*>(&k_persistent<S, E>);
     if (resident == 0) resident = resident_blocks<S>(kernel);
     const int npx = P.width * P.height;
     const int need = (npx + kBlock - 1) / kBlock;
     const dim3 grid(unsigned(need < resident ? need : resident)), block(kBlock);
-    hipLaunchKernelGGL((k_persistent<S, E>), grid, block, 0, st, sc, P, io, queue, cnt);
+    hipLaunchKernelGGL((k_persistent<S, E>), grid, block, 0, st, sc, P, io, queue, order, order_n, cnt);
+}
+
+// Work order of restir_shade. Per-pixel chains differ widely in length, and a
+// persistent lane that takes a long one late sets the end of the pass, so
+// pixels whose primary hit is a dielectric (camera paths, the longest chains)
+// are queued first (from the front of `order`), then the ReSTIR pixels (from
+// the back). Pixels without a primary hit get their result here (black, zero
+// variance, as start_pixel writes them) and are not queued. The order only
+// changes which lane runs a pixel, never its result: every pixel's random
+// sequence is seeded from its index.
+__global__ __launch_bounds__(kBlock) void k_shade_order(FrameParams P, GBufferSoA gb, float4* __restrict__ hdr,
+                                                        float* __restrict__ lum_var, unsigned* __restrict__ order,
+                                                        unsigned* __restrict__ order_n) {
+    const int npx = P.width * P.height;
+    const int px = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    int kind = 0;
+    if (px < npx) {
+        kind = ibits(gb.p0[px].w);
+        if (kind == 0) {
+            hdr[px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (lum_var) lum_var[px] = 0.0f;
+        }
+    }
+    const bool first = kind != 0 && kind != 1, last = kind == 1;
+    const unsigned a = wave_append_pos(first, order_n);
+    const unsigned b = wave_append_pos(last, order_n + 1);
+    if (first) order[a] = unsigned(px);
+    if (last) order[unsigned(npx) - 1u - b] = unsigned(px);
+}
+
+// ------------------------------------------------------------------ wavefront
+// The same engines split into kernels by role. Each pixel's lane state lives
+// in HBM between rays, one 32-bit word per array of `stride` slots (word k of
+// slot i at k * stride + i), in two buffers: a trace kernel at full occupancy
+// reads only each slot's pending ray and writes its hit, then a logic kernel
+// loads the whole state, advances the engine by one step (the code the
+// persistent loop runs) and appends the slots that trace again, compacted
+// with one atomic per wave, to the other buffer. Active slots only ever
+// decrease, so the host steers with a count read back a few iterations late;
+// below a tail threshold the persistent loop resumes the remaining slots from
+// their saved state. Every slot runs exactly the per-pixel sequence of the
+// persistent engine, so results are identical to it.
+template <class Lane>
+struct Slot {
+    static_assert(sizeof(Lane) % 4 == 0, "lane state is stored as 32-bit words");
+    static constexpr int kLaneWords = int(sizeof(Lane) / 4);
+    static constexpr int kWords = kLaneWords + 1;  // + random-number state
+    static constexpr int kPhase = int(__builtin_offsetof(Lane, phase) / 4);
+    static constexpr int kRo = int(__builtin_offsetof(Lane, ro) / 4);
+    static constexpr int kRd = int(__builtin_offsetof(Lane, rd) / 4);
+    static constexpr int kTmax = int(__builtin_offsetof(Lane, tmax) / 4);
+
+    static __device__ __forceinline__ void load(const uint32_t* __restrict__ st, size_t stride, size_t i, Lane& L,
+                                                uint32_t& rng) {
+        uint32_t w[kLaneWords];
+#pragma unroll
+        for (int k = 0; k < kLaneWords; ++k) w[k] = st[size_t(k) * stride + i];
+        __builtin_memcpy(&L, w, sizeof(Lane));
+        rng = st[size_t(kLaneWords) * stride + i];
+    }
+    static __device__ __forceinline__ void store(uint32_t* __restrict__ st, size_t stride, size_t i, const Lane& L,
+                                                 uint32_t rng) {
+        uint32_t w[kLaneWords];
+        __builtin_memcpy(w, &L, sizeof(Lane));
+#pragma unroll
+        for (int k = 0; k < kLaneWords; ++k) st[size_t(k) * stride + i] = w[k];
+        st[size_t(kLaneWords) * stride + i] = rng;
+    }
+};
+
+// Control words of the wavefront loop.
+enum : int { kWfCount0 = 0, kWfCount1 = 1, kWfFetchTrace = 2, kWfFetchLogic = 3, kWfCtlWords = 4 };
+
+// Next 64 entries of [0, n) for the calling wave, from the fetch counter.
+__device__ __forceinline__ unsigned wave_fetch(unsigned* fetch) {
+    unsigned base = 0;
+    if (__lane_id() == 0) base = atomicAdd(fetch, 64u);
+    return __shfl(base, 0);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Starts every pixel; pixels that need a ray are appended to buffer 0.
+template <int S, class E>
+__global__ __launch_bounds__(kBlock) void k_wf_start(DevScene sc, FrameParams P, PassIO io, uint32_t* __restrict__ dst,
+                                                     size_t stride, unsigned* __restrict__ ctl) {
+    using W = Slot<typename E::Lane>;
+    Tracer<S> pt(sc, P, nullptr);
+    E eng(pt, P, io);
+    const int px = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    typename E::Lane L;
+    bool keep = false;
+    if (px < P.width * P.height) {
+        L.phase = kExt;  // start() sets the real phase
+        keep = eng.start(L, px) == kTrace;
+    }
+    const unsigned j = wave_append_pos(keep, ctl + kWfCount0);
+    if (keep) W::store(dst, stride, j, L, pt.rng.state);
+}
+
+// One ray per active slot of buffer `cur`: hit (t, u, v, triangle) into hits.
+// Clears the count the next logic kernel appends to and its fetch counter.
+template <int S, class E>
+__global__ __launch_bounds__(kBlock) void k_wf_trace(DevScene sc, FrameParams P, const uint32_t* __restrict__ src,
+                                                     size_t stride, uint4* __restrict__ hits, unsigned* __restrict__ ctl,
+                                                     int cur, unsigned long long* cnt) {
+    using W = Slot<typename E::Lane>;
+    __shared__ uint32_t stack[S * kBlock];
+    Tracer<S> pt(sc, P, stack + threadIdx.x);
+    const unsigned n = ctl[kWfCount0 + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl[kWfCount0 + (cur ^ 1)] = 0u;
+        ctl[kWfFetchLogic] = 0u;
+    }
+    for (;;) {
+        const unsigned base = wave_fetch(ctl + kWfFetchTrace);
+        if (base >= n) break;
+        const unsigned i = base + __lane_id();
+        if (i < n) {
+            const size_t k = i;
+            const int phase = int(src[size_t(W::kPhase) * stride + k]);
+            const V3 ro{__uint_as_float(src[size_t(W::kRo) * stride + k]),
+                        __uint_as_float(src[size_t(W::kRo + 1) * stride + k]),
+                        __uint_as_float(src[size_t(W::kRo + 2) * stride + k])};
+            const V3 rd{__uint_as_float(src[size_t(W::kRd) * stride + k]),
+                        __uint_as_float(src[size_t(W::kRd + 1) * stride + k]),
+                        __uint_as_float(src[size_t(W::kRd + 2) * stride + k])};
+            const float tmax = __uint_as_float(src[size_t(W::kTmax) * stride + k]);
+            const Hit h = pt.intersect(ro, rd, tmax, phase == kShadow && sc.opaque);
+            hits[k] = make_uint4(__float_as_uint(h.t), __float_as_uint(h.u), __float_as_uint(h.v), uint32_t(h.tri));
+        }
+    }
+    if (cnt) {
+        const uint32_t rays = wave_sum(pt.tl.rays), nodes = wave_sum(pt.tl.nodes), tris = wave_sum(pt.tl.tris);
+        const unsigned long long over = __ballot(pt.tl.overflow != 0);
+        if (__lane_id() == 0) {
+            if (rays) atomicAdd(cnt + 3 * E::kPass, (unsigned long long)rays);
+            if (nodes) atomicAdd(cnt + 3 * E::kPass + 1, (unsigned long long)nodes);
+            if (tris) atomicAdd(cnt + 3 * E::kPass + 2, (unsigned long long)tris);
+            if (over) atomicAdd(cnt + kCntOverflow, (unsigned long long)__popcll(over));
+        }
+    }
+}
+
+// Advances every slot of buffer `cur` by its traced hit; slots that trace
+// again go to buffer cur ^ 1. Clears the trace fetch counter.
+template <int S, class E>
+__global__ __launch_bounds__(kBlock, E::kLogicBlocks) void k_wf_logic(DevScene sc, FrameParams P, PassIO io,
+                                                                     const uint32_t* __restrict__ src,
+                                                                     uint32_t* __restrict__ dst, size_t stride,
+                                                                     const uint4* __restrict__ hits,
+                                                                     unsigned* __restrict__ ctl, int cur) {
+    using W = Slot<typename E::Lane>;
+    Tracer<S> pt(sc, P, nullptr);
+    E eng(pt, P, io);
+    const unsigned n = ctl[kWfCount0 + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kWfFetchTrace] = 0u;
+    for (;;) {
+        const unsigned base = wave_fetch(ctl + kWfFetchLogic);
+        if (base >= n) break;
+        const unsigned i = base + __lane_id();
+        typename E::Lane L;
+        bool keep = false;
+        if (i < n) {
+            W::load(src, stride, i, L, pt.rng.state);
+            const uint4 hw = hits[i];
+            const Hit h{__uint_as_float(hw.x), __uint_as_float(hw.y), __uint_as_float(hw.z), int(hw.w)};
+            keep = eng.step(L, h) == kTrace;
+        }
+        const unsigned j = wave_append_pos(keep, ctl + kWfCount0 + (cur ^ 1));
+        if (keep) W::store(dst, stride, j, L, pt.rng.state);
+    }
+}
+
+// The persistent loop over the slots left in buffer `cur`: lanes resume saved
+// states (from the trace fetch counter) instead of starting pixels.
+template <int S, class E>
+__global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_wf_finish(DevScene sc, FrameParams P, PassIO io,
+                                                                     const uint32_t* __restrict__ src, size_t stride,
+                                                                     unsigned* __restrict__ ctl, int cur,
+                                                                     unsigned long long* cnt) {
+    using W = Slot<typename E::Lane>;
+    __shared__ uint32_t stack[S * kBlock];
+    Tracer<S> pt(sc, P, stack + threadIdx.x);
+    E eng(pt, P, io);
+    const unsigned n = ctl[kWfCount0 + cur];
+    unsigned* fetch = ctl + kWfFetchTrace;
+    const unsigned lane = __lane_id();
+    typename E::Lane L;
+    L.phase = kIdle;
+    bool exhausted = false;
+    for (;;) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(L.phase == kIdle);
+            if (need) {
+                const int count = __popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned base = 0;
+                if (int(lane) == leader) base = atomicAdd(fetch, unsigned(count));
+                base = __shfl(base, leader);
+                if (L.phase == kIdle) {
+                    const unsigned i = base + __builtin_amdgcn_mbcnt_hi(unsigned(need >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo(unsigned(need), 0u));
+                    if (i < n) W::load(src, stride, i, L, pt.rng.state);
+                }
+                if (base + unsigned(count) >= n) exhausted = true;
+            }
+        }
+        if (__ballot(L.phase != kIdle) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (L.phase != kIdle) {
+            const Hit h = pt.intersect(L.ro, L.rd, L.tmax, eng.any_hit(L));
+            if (eng.step(L, h) != kTrace) L.phase = kIdle;
+        }
+    }
+    pt.flush(cnt, E::kPass);
+}
+
+#ifndef RM_WAVE_BATCH
+#define RM_WAVE_BATCH 4  // iterations between count read-backs
+#endif
+#ifndef RM_WAVE_TAIL8
+#define RM_WAVE_TAIL8 8  // tail threshold in eighths of the persistent loop's resident lanes
+#endif
+#ifndef RM_WAVE_MAX_ITERS
+#define RM_WAVE_MAX_ITERS 4096
+#endif
+
+template <int S, class E>
+int run_wavefront(const DevScene& sc, const FrameParams& P, const PassIO& io, const WaveBufs& wb,
+                  unsigned long long* cnt, hipStream_t st, WaveRun* run) {
+    static int trace_blocks = 0, logic_blocks = 0, finish_blocks = 0;
+    if (trace_blocks == 0) {
+        trace_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_trace<S, E>));
+        logic_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_logic<S, E>));
+        finish_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_finish<S, E>));
+    }
+    const unsigned npx = unsigned(P.width) * unsigned(P.height);
+    if (size_t(npx) > wb.stride) return int(hipErrorInvalidValue);
+    const size_t stride = wb.stride;
+    hipError_t e = hipMemsetAsync(wb.ctl, 0, kWfCtlWords * sizeof(unsigned), st);
+    if (e != hipSuccess) return int(e);
+    const dim3 block(kBlock);
+    hipLaunchKernelGGL((k_wf_start<S, E>), dim3((npx + kBlock - 1) / kBlock), block, 0, st, sc, P, io, wb.state[0],
+                       stride, wb.ctl);
+    // Grids: the resident set, or fewer blocks when the last known count
+    // (an upper bound of the current one) needs fewer.
+    unsigned known = npx;
+    auto grid_for = [&](int resident) {
+        const unsigned need = known > 0u ? (known + kBlock - 1u) / kBlock : 1u;
+        return dim3(need < unsigned(resident) ? need : unsigned(resident));
+    };
+    const unsigned tail = wb.tail ? unsigned(size_t(finish_blocks) * kBlock * RM_WAVE_TAIL8 / 8) : 0u;
+    int cur = 0, pending = -1, batch = 0, it = 0;
+    while (it < RM_WAVE_MAX_ITERS) {
+        for (int b = 0; b < RM_WAVE_BATCH; ++b, ++it) {
+            hipLaunchKernelGGL((k_wf_trace<S, E>), grid_for(trace_blocks), block, 0, st, sc, P, wb.state[cur], stride,
+                               wb.hits, wb.ctl, cur, cnt);
+            hipLaunchKernelGGL((k_wf_logic<S, E>), grid_for(logic_blocks), block, 0, st, sc, P, io, wb.state[cur],
+                               wb.state[cur ^ 1], stride, wb.hits, wb.ctl, cur);
+            cur ^= 1;
+        }
+        const int slot = batch & 1;
+        e = hipMemcpyAsync(wb.host + slot, wb.ctl + kWfCount0 + cur, sizeof(unsigned), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(wb.ev[slot], st);
+        if (e != hipSuccess) return int(e);
+        if (pending >= 0) {
+            // The count of two read-backs ago; the batch launched since keeps the GPU busy meanwhile.
+            e = hipEventSynchronize(wb.ev[pending]);
+            if (e != hipSuccess) return int(e);
+            known = wb.host[pending];
+            if (known <= tail || known == 0u) break;
+        }
+        pending = slot;
+        ++batch;
+    }
+    hipLaunchKernelGGL((k_wf_finish<S, E>), grid_for(finish_blocks), block, 0, st, sc, P, io, wb.state[cur], stride,
+                       wb.ctl, cur, cnt);
+    if (run) {
+        run->iters = it;
+        run->tail = known;
+    }
+    return int(hipGetLastError());
 }
 
 }  // namespace
 
-void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
-                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
-                 unsigned long long* cnt, hipStream_t st) {
+size_t wave_slot_words(bool) { return size_t(Slot<PathLane>::kWords); }
+
+int run_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
+             const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, const WaveBufs& wb,
+             unsigned long long* cnt, hipStream_t st, WaveRun* run) {
     const PassIO io{gb, rm, hdr, rgb, lum_var};
     if (stack <= 32) {
-        if (shade) launch_persistent<32, PathEngine<32, true>>(sc, P, io, queue, cnt, st);
-        else launch_persistent<32, PathEngine<32, false>>(sc, P, io, queue, cnt, st);
+        return shade ? run_wavefront<32, PathEngine<32, true>>(sc, P, io, wb, cnt, st, run)
+                     : run_wavefront<32, PathEngine<32, false>>(sc, P, io, wb, cnt, st, run);
+    }
+    return shade ? run_wavefront<64, PathEngine<64, true>>(sc, P, io, wb, cnt, st, run)
+                 : run_wavefront<64, PathEngine<64, false>>(sc, P, io, wb, cnt, st, run);
+}
+
+void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
+                 const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
+                 unsigned int* order, unsigned long long* cnt, hipStream_t st) {
+    const PassIO io{gb, rm, hdr, rgb, lum_var};
+    unsigned* order_n = queue + 1;
+    if (shade && order) {
+        const int npx = P.width * P.height;
+        hipLaunchKernelGGL(k_shade_order, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, gb, hdr, lum_var,
+                           order, order_n);
     } else {
-        if (shade) launch_persistent<64, PathEngine<64, true>>(sc, P, io, queue, cnt, st);
-        else launch_persistent<64, PathEngine<64, false>>(sc, P, io, queue, cnt, st);
+        order = nullptr;
+    }
+    if (stack <= 32) {
+        if (shade) launch_persistent<32, PathEngine<32, true>>(sc, P, io, queue, order, order_n, cnt, st);
+        else launch_persistent<32, PathEngine<32, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+    } else {
+        if (shade) launch_persistent<64, PathEngine<64, true>>(sc, P, io, queue, order, order_n, cnt, st);
+        else launch_persistent<64, PathEngine<64, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
     }
 }
 
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                      unsigned int* queue, unsigned long long* cnt, hipStream_t st) {
     const PassIO io{gb, ri, nullptr, nullptr, nullptr};
-    if (stack <= 32) launch_persistent<32, GenEngine<32>>(sc, P, io, queue, cnt, st);
-    else launch_persistent<64, GenEngine<64>>(sc, P, io, queue, cnt, st);
+    if (stack <= 32) launch_persistent<32, GenEngine<32>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+    else launch_persistent<64, GenEngine<64>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
 }
 
 }  // namespace rmh

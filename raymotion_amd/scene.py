@@ -272,6 +272,11 @@ class Backend:
     def set_profiling(self, on: bool):
         self._check(lib().rm_set_profiling(self._h, int(on)), "rm_set_profiling")
 
+    def set_schedule(self, schedule: int):
+        """Path passes: 0 persistent lanes (default), 1 wavefront kernels with
+        a persistent tail, 2 wavefront kernels only."""
+        self._check(lib().rm_set_schedule(self._h, int(schedule)), "rm_set_schedule")
+
     def stats(self) -> Stats:
         s = Stats()
         self._check(lib().rm_get_stats(self._h, ctypes.byref(s)), "rm_get_stats")
