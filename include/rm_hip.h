@@ -104,7 +104,8 @@ typedef struct {
     int32_t path;         /* 0 megakernel (render_paths), 1 ReSTIR + denoise */
     int32_t lanes;        /* megakernel lanes per pixel: 1, 8 or 32 */
     int32_t has_history;  /* temporal reuse was active */
-    int32_t refit;        /* last rm_set_scene's tree: 0 unchanged, 1 refit, 2 rebuild */
+    int32_t refit;        /* last scene upload's tree: 0 unchanged, 1 refit, 2 rebuild from
+                             the given nodes, 3 built on the GPU (rm_set_scene_gpu_tree) */
     /* Per pass: megakernel, generate, spatial, shade, probe, and the total. */
     uint64_t rays[6];        /* rays traced */
     uint64_t node_visits[6]; /* 4-wide BVH nodes fetched */
@@ -135,6 +136,17 @@ const char* rm_last_error(const rm_ctx* ctx);
  * resident 4-wide tree is refit on the GPU, at most 31 times in a row;
  * otherwise it is rebuilt from the given nodes. */
 int rm_set_scene(rm_ctx* ctx, const rm_scene_desc* scene);
+
+/* As rm_set_scene, but the 4-wide tree is built on the GPU from the triangles
+ * (a linear BVH over Morton codes of the centroids, collapsed to 4-wide nodes
+ * with leaves of at most 4 triangles); scene->nodes and scene->bvh_tri are not
+ * used and may be empty. The tree is rebuilt when `rebuild` is nonzero, when
+ * the triangle count changed or after 31 refits in a row, and otherwise refit
+ * on the GPU when positions moved. Images equal those of rm_set_scene with any
+ * valid tree: traversal returns the closest hit whatever the tree. No
+ * counterpart in the reference, which builds its tree on the host (the
+ * rebuild every 32nd geometry update of its runtime.hpp:119-130). */
+int rm_set_scene_gpu_tree(rm_ctx* ctx, const rm_scene_desc* scene, int rebuild);
 
 /* One frame (render_image_metal, metal.mm:95-414): rgb8 receives W*H*3 bytes,
  * row-major, top row first; hdr (may be NULL) receives the W*H*3 linear values

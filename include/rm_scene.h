@@ -8,6 +8,7 @@
  *   rm_scene_refit_bvh      refit_bvh               (src/engine/bvh.cpp:250-272)
  *   rm_scene_export         the marshalling of      (src/engine/metal.mm:109-145)
  *   rm_scene_upload         marshalling + rm_set_scene
+ *   rm_scene_upload_gpu_tree  marshalling + rm_set_scene_gpu_tree (no host tree needed)
  * Return codes are the RM_* codes of rm_hip.h.
  */
 #ifndef RM_SCENE_H
@@ -60,6 +61,8 @@ int rm_scene_export(const rm_scene* s, rm_triangle* tris, rm_bvh_node* nodes, in
                     float* light_cdf, float* light_total);
 /* Marshals and calls rm_set_scene. */
 int rm_scene_upload(rm_ctx* ctx, const rm_scene* s);
+/* Marshals (the scene needs no BVH) and calls rm_set_scene_gpu_tree. */
+int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild);
 
 /* Camera::set (camera.cpp:3-22): position, Euler angles in degrees and field
  * of view (focal = 240 / tan(fov/2) pixels, independent of resolution). */

@@ -120,6 +120,8 @@ _SIGNATURES = [
     ("rm_scene_precompute", _I, [_P, _P, _P, _P, _P, _P]),
     ("rm_scene_export", _I, [_P, _P, _P, _P, _P, _P, _P]),
     ("rm_scene_upload", _I, [_P, _P]),
+    ("rm_scene_upload_gpu_tree", _I, [_P, _P, _I]),
+    ("rm_set_scene_gpu_tree", _I, [_P, _P, _I]),
     ("rm_default_frame", None, [ctypes.POINTER(Frame), _I, _I, _I, _I]),
     ("rm_camera_set", None, [_P, _P, ctypes.c_double, ctypes.POINTER(Camera)]),
     ("rm_camera_orbit", None, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,

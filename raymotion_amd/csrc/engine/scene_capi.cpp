@@ -159,6 +159,14 @@ int rm_scene_upload(rm_ctx* ctx, const rm_scene* s) {
     return rm_set_scene(ctx, &d);
 }
 
+int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild) {
+    if (!ctx || !s) return RM_ERR_INVALID;
+    raymotion::MarshalledScene m;
+    raymotion::marshal_scene(s->scene, m);
+    const rm_scene_desc d = m.desc(s->scene);
+    return rm_set_scene_gpu_tree(ctx, &d, rebuild);
+}
+
 static rm_camera to_c(const raymotion::Camera& c) {
     return {float(c.x), float(c.y), float(c.z), float(c.focal), float(c.m0), float(c.m1), float(c.m2),
             float(c.m3), float(c.m4), float(c.m5), float(c.m6), float(c.m7), float(c.m8)};

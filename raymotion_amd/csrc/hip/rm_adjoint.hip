@@ -151,8 +151,9 @@ void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams&
                     unsigned long long* cnt, hipStream_t st) {
     const size_t threads = size_t(P.width) * size_t(P.height) * size_t(lanes);
     const dim3 grid(unsigned((threads + kBlock - 1) / kBlock)), block(kBlock);
-    if (stack <= 32) hipLaunchKernelGGL((k_adjoint<32>), grid, block, 0, st, sc, P, A, lanes, cnt);
-    else hipLaunchKernelGGL((k_adjoint<64>), grid, block, 0, st, sc, P, A, lanes, cnt);
+    with_stack(stack, [&](auto cap) {
+        hipLaunchKernelGGL((k_adjoint<decltype(cap)::value>), grid, block, 0, st, sc, P, A, lanes, cnt);
+    });
 }
 
 }  // namespace rmh

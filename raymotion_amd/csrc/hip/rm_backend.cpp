@@ -90,6 +90,11 @@ struct rm_ctx {
     bool has_scene = false;
     DevBuf nodes, geo, nrm, nn, matid, mats, uv, lights, cdf, texels;
     DevBuf level_nodes;
+    // GPU-built tree: triangle records in scene order, the leaf-order
+    // permutation, build scratch and a pinned read-back of level sizes.
+    bool gpu_tree = false;
+    DevBuf geo_scene, perm, build_scratch;
+    uint32_t* build_host = nullptr;
     std::vector<int32_t> level_start;  // refit launch ranges into level_nodes, deepest level first
     int32_t nnodes = 0, nlights = 0, opaque = 1, stack_need = 0, ntri = 0;
     float light_total = 0.0f;
@@ -409,6 +414,10 @@ void rm_destroy(rm_ctx* c) {
     c->hist_res.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    c->geo_scene.release();
+    c->perm.release();
+    c->build_scratch.release();
+    if (c->build_host) (void)hipHostFree(c->build_host);
     for (auto& b : c->wave_state) b.release();
     c->wave_hits.release();
     c->wave_ctl.release();
@@ -455,13 +464,13 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     if (!validate_scene(*s, false, false, false, perr)) return c->fail(RM_ERR_INVALID, perr);
     const SceneKeys keys = scene_keys(*s);
     const bool had = c->has_scene;
-    if (had && keys == c->keys) {
+    if (had && keys == c->keys && !c->gpu_tree) {
         c->last_refit = 0;
         c->stats.refit = 0;
         return RM_OK;  // identical scene already resident
     }
     const bool same_count = had && keys.ntri == c->keys.ntri;
-    const bool topo_same = same_count && same_topology(*s, c->prev_bvh_tri, c->prev_topo);
+    const bool topo_same = same_count && !c->gpu_tree && same_topology(*s, c->prev_bvh_tri, c->prev_topo);
     const bool moved = !same_count || keys.positions != c->keys.positions;
     const bool normals = !same_count || keys.normals != c->keys.normals;
     const bool materials = !same_count || keys.material != c->keys.material || keys.ntexel != c->keys.ntexel;
@@ -531,6 +540,7 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
         c->nnodes = int32_t(ps.depth.size());
         c->stack_need = ps.stack_need;
         c->refits = 0;
+        c->gpu_tree = false;
         topology_of(*s, c->prev_bvh_tri, c->prev_topo);
     }
     if (refit) ++c->refits;
@@ -543,6 +553,134 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     c->hist_key = history_key(keys, s->light_total);
     c->has_scene = true;
     c->last_refit = refit ? 1 : rebuild ? 2 : 0;
+    c->stats.refit = c->last_refit;
+    c->stats.bvh_nodes = c->nnodes;
+    c->stats.stack_depth = c->stack_need;
+    c->stats.scene_ms[0] = c->scene_ms[0];
+    c->stats.scene_ms[1] = c->scene_ms[1];
+    return RM_OK;
+}
+
+int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_request) {
+    if (!c || !s_in) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    rm_scene_desc sd = *s_in;  // the caller's tree, if any, is not used
+    sd.nodes = nullptr;
+    sd.nnode = 0;
+    sd.bvh_tri = nullptr;
+    const rm_scene_desc* s = &sd;
+    std::string perr;
+    if (!validate_scene(*s, false, false, false, perr)) return c->fail(RM_ERR_INVALID, perr);
+    if (s->ntri == 0 || s->ntri > size_t(INT32_MAX / 4)) return c->fail(RM_ERR_INVALID, "GPU tree build needs 1 to 2^29 triangles");
+    const SceneKeys keys = scene_keys(*s);
+    const bool had = c->has_scene && c->gpu_tree;
+    if (had && keys == c->keys && !rebuild_request) {
+        c->last_refit = 0;
+        c->stats.refit = 0;
+        return RM_OK;
+    }
+    const bool same_count = had && keys.ntri == c->keys.ntri;
+    const bool moved = !same_count || keys.positions != c->keys.positions;
+    const bool normals = !same_count || keys.normals != c->keys.normals;
+    const bool materials = !same_count || keys.material != c->keys.material || keys.ntexel != c->keys.ntexel;
+    // Rebuild on request, when the triangle count changed, or after 31
+    // refits in a row (the policy rm_set_scene applies to host trees).
+    const bool rebuild = rebuild_request || !same_count || (moved && c->refits >= 31);
+    const bool refit = !rebuild && moved;
+    if (!validate_scene(*s, false, materials, true, perr)) return c->fail(RM_ERR_INVALID, perr);
+
+    PackedScene& ps = c->pack;
+    if (moved || normals) pack_geometry(*s, ps);  // no tree: records in scene order
+    pack_lights(*s, ps);
+    if (materials) pack_materials(*s, ps);
+    const int n = int(s->ntri);
+    const size_t max_nodes = size_t(std::max(n - 1, 1));
+
+    c->has_scene = false;
+    const bool prof = c->profiling;
+    if (prof) (void)hipEventRecord(c->ev[8], c->stream);
+    bool ok = upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
+    if (ok && moved) ok = upload(c, c->geo_scene, ps.geo);
+    if (ok && (moved || normals)) ok = upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn);
+    if (ok && materials)
+        ok = upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
+             upload(c, c->texels, ps.texels);
+    if (ok && (rebuild || refit)) ok = c->geo.ensure(size_t(n) * kTriGeoWords * sizeof(float4));
+    if (ok && rebuild)
+        ok = c->perm.ensure(size_t(n) * sizeof(uint32_t)) && c->nodes.ensure(max_nodes * kNodeWords * sizeof(float4)) &&
+             c->build_scratch.ensure(gpu_build_bytes(n));
+    if (ok && rebuild && !c->build_host) {
+        void* h = nullptr;
+        ok = hipHostMalloc(&h, 2 * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+        c->build_host = static_cast<uint32_t*>(h);
+    }
+    if (!ok) {
+        c->drop_history();
+        return c->fail(RM_ERR_ALLOC, "HIP scene buffer allocation failed");
+    }
+    if (prof) (void)hipEventRecord(c->ev[9], c->stream);
+    GpuBuildResult br;
+    if (rebuild) {
+        GpuBuildArgs a{};
+        a.geo_scene = c->geo_scene.as<float4>();
+        a.geo = c->geo.as<float4>();
+        a.perm = c->perm.as<uint32_t>();
+        a.nodes = c->nodes.as<float4>();
+        a.max_nodes = max_nodes;
+        a.scratch = c->build_scratch.ptr;
+        a.scratch_bytes = c->build_scratch.cap;
+        a.host_words = c->build_host;
+        a.n = n;
+        a.leaf_tris = kGpuLeafTris;
+        if (int rc = check(c, hipError_t(gpu_build(a, br, c->stream)), "GPU tree build")) return rc;
+        if (br.stack_need > kMaxStack)
+            return c->fail(RM_ERR_INVALID, "BVH too deep: the traversal stack would need " +
+                                               std::to_string(br.stack_need) + " entries (at most " +
+                                               std::to_string(kMaxStack) + ")");
+        // Refit order: the levels' contiguous node ranges, deepest level first.
+        const size_t nl = br.level_count.size();
+        std::vector<int32_t> first(nl, 0), order;
+        for (size_t l = 1; l < nl; ++l) first[l] = first[l - 1] + br.level_count[l - 1];
+        order.reserve(size_t(br.nodes));
+        c->level_start.assign(1, 0);
+        for (size_t l = nl; l-- > 0;) {
+            for (int k = 0; k < br.level_count[l]; ++k) order.push_back(first[l] + k);
+            c->level_start.push_back(int32_t(order.size()));
+        }
+        if (!upload(c, c->level_nodes, order)) return c->fail(RM_ERR_ALLOC, "HIP scene buffer allocation failed");
+    } else if (refit) {
+        launch_gather_tris(c->geo_scene.as<float4>(), c->perm.as<uint32_t>(), n, c->geo.as<float4>(), c->stream);
+        for (size_t l = 0; l + 1 < c->level_start.size(); ++l)
+            launch_refit_level(c->nodes.as<float4>(), c->geo.as<float4>(),
+                               c->level_nodes.as<int32_t>() + c->level_start[l],
+                               c->level_start[l + 1] - c->level_start[l], c->stream);
+    }
+    if (prof) (void)hipEventRecord(c->ev[10], c->stream);
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "scene upload")) return rc;
+    c->scene_ms[0] = c->scene_ms[1] = 0.0f;
+    if (prof) {
+        if (hipEventElapsedTime(&c->scene_ms[0], c->ev[8], c->ev[9]) != hipSuccess) c->scene_ms[0] = 0.0f;
+        if (hipEventElapsedTime(&c->scene_ms[1], c->ev[9], c->ev[10]) != hipSuccess) c->scene_ms[1] = 0.0f;
+    }
+    if (rebuild) {
+        c->nnodes = br.nodes;
+        c->stack_need = br.stack_need;
+        c->refits = 0;
+        c->gpu_tree = true;
+        c->prev_bvh_tri.clear();
+        c->prev_topo.clear();
+    }
+    if (refit) ++c->refits;
+    if (materials) c->opaque = ps.opaque;
+    c->nlights = int32_t(s->nlight);
+    c->light_tri.assign(s->light_tri, s->light_tri + s->nlight);
+    c->light_total = ps.light_total;
+    c->ntri = n;
+    c->keys = keys;
+    c->hist_key = history_key(keys, s->light_total);
+    c->has_scene = true;
+    c->last_refit = refit ? 1 : rebuild ? 3 : 0;
     c->stats.refit = c->last_refit;
     c->stats.bvh_nodes = c->nnodes;
     c->stats.stack_depth = c->stack_need;

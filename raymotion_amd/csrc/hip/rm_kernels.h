@@ -2,13 +2,25 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+#include <vector>
+
 #include "rm_layout.h"
 
 namespace rmh {
 
 // `stack` is the traversal stack depth the resident tree needs; it selects the
-// kernel instance with a large enough LDS stack (32 or 64 entries per lane).
+// kernel instance with a large enough LDS stack (32, 48 or 64 entries per lane).
 constexpr int kMaxStack = 64;
+
+// Calls f(std::integral_constant<int, S>{}) with the smallest stack capacity
+// S in {32, 48, 64} that holds `stack` entries.
+template <class F>
+inline auto with_stack(int stack, F&& f) {
+    if (stack <= 32) return f(std::integral_constant<int, 32>{});
+    if (stack <= 48) return f(std::integral_constant<int, 48>{});
+    return f(std::integral_constant<int, 64>{});
+}
 
 // render_paths_simd with 8 or 32 lanes per pixel.
 void launch_render_paths(int stack, int lanes, const DevScene& sc, const FrameParams& P, uint8_t* out, float4* hdr,
@@ -69,6 +81,32 @@ struct AdjointArgs {
 };
 void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams& P, const AdjointArgs& A,
                     unsigned long long* cnt, hipStream_t st);
+// GPU tree build (rm_build.hip): linear BVH over the triangle centroids,
+// collapsed to 4-wide nodes with leaves of at most leaf_tris triangles.
+constexpr int kGpuLeafTris = 4;
+struct GpuBuildArgs {
+    const float4* geo_scene;  // n triangle records, scene order (input)
+    float4* geo;              // n triangle records, leaf order (output)
+    uint32_t* perm;           // n: scene index of leaf-order triangle k (output)
+    float4* nodes;            // room for max_nodes 4-wide nodes (output)
+    size_t max_nodes;         // at least max(n - 1, 1)
+    void* scratch;            // gpu_build_bytes(n) bytes
+    size_t scratch_bytes;
+    uint32_t* host_words;     // 2 words of pinned host memory
+    int n;
+    int leaf_tris;
+};
+struct GpuBuildResult {
+    int nodes = 0;
+    int stack_need = 0;
+    std::vector<int> level_count;  // 4-wide nodes per level, root level first; each level is contiguous
+};
+size_t gpu_build_bytes(int n);
+// Returns a hipError_t code; waits on the stream once per output level.
+int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st);
+// dst[k] = src[perm[k]] for n triangle records.
+void launch_gather_tris(const float4* src, const uint32_t* perm, int n, float4* dst, hipStream_t st);
+
 // Bottom-up refit of the nodes listed in `order` (one tree level, deepest first).
 void launch_refit_level(float4* nodes, const float4* geo, const int32_t* level_nodes, int count, hipStream_t st);
 

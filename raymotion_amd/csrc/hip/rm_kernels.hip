@@ -354,6 +354,9 @@ inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock
         if ((stack) <= 32) {             \
             constexpr int S = 32;        \
             launch;                      \
+        } else if ((stack) <= 48) {      \
+            constexpr int S = 48;        \
+            launch;                      \
         } else {                         \
             constexpr int S = 64;        \
             launch;                      \

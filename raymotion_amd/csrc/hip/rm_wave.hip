@@ -76,9 +76,9 @@ struct PathEngine {
     using Lane = PathLane;
     static constexpr int kPass = SHADE ? kCntShade : kCntMega;
     // Resident 256-lane blocks per CU the register allocation targets (3 waves
-    // per SIMD: a little spilling, measured faster than 2 waves without). A
-    // 64-entry stack takes 64 KiB of LDS per block, which allows 2.
-    static constexpr int kMinBlocks = S <= 32 ? 3 : 2;
+    // per SIMD: a little spilling, measured faster than 2 waves without). The
+    // stack takes S KiB of LDS per block: 160 KiB hold 3 blocks up to S = 48, 2 at 64.
+    static constexpr int kMinBlocks = S <= 48 ? 3 : 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     Tracer<S>& pt;
     const FrameParams& P;
@@ -526,7 +526,8 @@ template <int S>
 struct GenEngine {
     using Lane = GenLane;
     static constexpr int kPass = kCntGenerate;
-    static constexpr int kMinBlocks = S <= 32 ? 4 : 2;  // shadow rays: latency hiding wins over the spills
+    // shadow rays: latency hiding wins over the spills (LDS allows 4 blocks at S = 32)
+    static constexpr int kMinBlocks = S <= 32 ? 4 : S <= 48 ? 3 : 2;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -1085,12 +1086,11 @@ int run_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, co
              const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, const WaveBufs& wb,
              unsigned long long* cnt, hipStream_t st, WaveRun* run) {
     const PassIO io{gb, rm, hdr, rgb, lum_var};
-    if (stack <= 32) {
-        return shade ? run_wavefront<32, PathEngine<32, true>>(sc, P, io, wb, cnt, st, run)
-                     : run_wavefront<32, PathEngine<32, false>>(sc, P, io, wb, cnt, st, run);
-    }
-    return shade ? run_wavefront<64, PathEngine<64, true>>(sc, P, io, wb, cnt, st, run)
-                 : run_wavefront<64, PathEngine<64, false>>(sc, P, io, wb, cnt, st, run);
+    return with_stack(stack, [&](auto cap) {
+        constexpr int S = decltype(cap)::value;
+        return shade ? run_wavefront<S, PathEngine<S, true>>(sc, P, io, wb, cnt, st, run)
+                     : run_wavefront<S, PathEngine<S, false>>(sc, P, io, wb, cnt, st, run);
+    });
 }
 
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
@@ -1105,20 +1105,20 @@ void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P
     } else {
         order = nullptr;
     }
-    if (stack <= 32) {
-        if (shade) launch_persistent<32, PathEngine<32, true>>(sc, P, io, queue, order, order_n, cnt, st);
-        else launch_persistent<32, PathEngine<32, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
-    } else {
-        if (shade) launch_persistent<64, PathEngine<64, true>>(sc, P, io, queue, order, order_n, cnt, st);
-        else launch_persistent<64, PathEngine<64, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
-    }
+    with_stack(stack, [&](auto cap) {
+        constexpr int S = decltype(cap)::value;
+        if (shade) launch_persistent<S, PathEngine<S, true>>(sc, P, io, queue, order, order_n, cnt, st);
+        else launch_persistent<S, PathEngine<S, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+    });
 }
 
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                      unsigned int* queue, unsigned long long* cnt, hipStream_t st) {
     const PassIO io{gb, ri, nullptr, nullptr, nullptr};
-    if (stack <= 32) launch_persistent<32, GenEngine<32>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
-    else launch_persistent<64, GenEngine<64>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+    with_stack(stack, [&](auto cap) {
+        constexpr int S = decltype(cap)::value;
+        launch_persistent<S, GenEngine<S>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+    });
 }
 
 }  // namespace rmh

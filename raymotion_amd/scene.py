@@ -168,12 +168,23 @@ class Backend:
         if rc:
             raise BackendError(rc, f"{what}: {lib().rm_last_error(self._h).decode()}")
 
-    def set_scene(self, scene: Scene | Exported):
+    def set_scene(self, scene: Scene | Exported, gpu_tree: bool = False, rebuild: bool = False):
+        """Uploads a scene. With gpu_tree the backend builds its traversal tree
+        on the GPU (the scene needs no BVH; `rebuild` forces a new build,
+        otherwise a resident GPU tree is refit while the triangle count stays)."""
         if isinstance(scene, Scene):
-            self._check(lib().rm_scene_upload(self._h, scene.handle), "rm_scene_upload")
+            if gpu_tree:
+                self._check(lib().rm_scene_upload_gpu_tree(self._h, scene.handle, int(rebuild)),
+                            "rm_scene_upload_gpu_tree")
+            else:
+                self._check(lib().rm_scene_upload(self._h, scene.handle), "rm_scene_upload")
         else:
             d = scene.desc()
-            self._check(lib().rm_set_scene(self._h, ctypes.byref(d)), "rm_set_scene")
+            if gpu_tree:
+                self._check(lib().rm_set_scene_gpu_tree(self._h, ctypes.byref(d), int(rebuild)),
+                            "rm_set_scene_gpu_tree")
+            else:
+                self._check(lib().rm_set_scene(self._h, ctypes.byref(d)), "rm_set_scene")
 
     def render(self, frame: Frame, seed: int, want_hdr: bool = True):
         px = frame.width * frame.height
