@@ -13,6 +13,8 @@ namespace raymotion {
 // render switches RAYMOTION_RESTIR, RAYMOTION_DENOISE, RAYMOTION_RESTIR_TEMPORAL,
 // RAYMOTION_RESTIR_CANDIDATES, RAYMOTION_RESTIR_SPATIAL, UOW2_NOADAPT and
 // UOW2_NOMIS like the reference, plus RAYMOTION_HIP_DEVICE (device index).
+// config.gpu_tree > 0 has the tree built (2) or refit (1) on the GPU instead of
+// taken from scene.nodes; the scene runtime sets it unless RAYMOTION_GPU_BVH=0.
 bool render_image_hip(const Scene& scene, const Camera& camera, const RenderConfig& config, uint64_t seed,
                       std::vector<uint8_t>& output, std::string& error);
 // Same, also returning the linear image that was tonemapped (W*H*3 floats).

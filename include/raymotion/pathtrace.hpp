@@ -68,6 +68,11 @@ struct RenderConfig {
     double restir_radius = 16;
     int restir_mcap = 128;
     int denoise_iterations = 2;
+    // Traversal tree of the HIP backend: 0 the scene's host-built tree
+    // (scene.nodes); 1 a tree built on the GPU and refit there while the
+    // triangle count stays (scene.nodes may be empty); 2 the same, rebuilt for
+    // this frame. Not in the reference, whose tree is always host-built.
+    int gpu_tree = 0;
 };
 
 }  // namespace raymotion

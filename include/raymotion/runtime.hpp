@@ -3,7 +3,9 @@
 // main.cpp files call (interface of include/raymotion/runtime.hpp:12-148 in the
 // reference). World geometry is rebuilt only when the pushed instances
 // change; with unchanged topology the BVH is refit, and every 32nd geometry
-// update is a full rebuild (runtime.hpp:119-130). Frame seeds are 12345+frame.
+// update is a full rebuild (runtime.hpp:119-130), on the GPU unless
+// RAYMOTION_GPU_BVH=0. Frame seeds are 12345+frame.
+#include <cstdlib>
 #include <filesystem>
 #include <memory>
 #include <optional>
@@ -47,6 +49,13 @@ class Objects {
     int framerate;
     int max_frames;
     double timer = 0;
+
+    // RAYMOTION_GPU_BVH=0 keeps the reference's host-built tree; otherwise the
+    // HIP backend builds and refits the tree on the GPU (same images).
+    static bool gpu_bvh() {
+        const char* v = std::getenv("RAYMOTION_GPU_BVH");
+        return !(v && v[0] == '0' && v[1] == '\0');
+    }
 
     void append(const Instance& in) {
         topology.push_back(in.model.get());
@@ -123,8 +132,16 @@ public:
             renderer.scene.clear_triangles();
             topology.clear();
             for (const Instance& i : instances) append(i);
-            if (topology == previous_topology && geometry_updates % 32 != 0) refit_bvh(renderer.scene);
-            else renderer.build();
+            const bool rebuild = !(topology == previous_topology && geometry_updates % 32 != 0);
+            if (gpu_bvh()) {
+                // The backend builds (rebuild) or refits the tree on the GPU.
+                renderer.scene.clear_bvh();
+                renderer.config.gpu_tree = rebuild ? 2 : 1;
+            } else {
+                renderer.config.gpu_tree = 0;
+                if (rebuild) renderer.build();
+                else refit_bvh(renderer.scene);
+            }
             ++geometry_updates;
             previous_topology = topology;
             previous_instances = instances;
@@ -145,6 +162,7 @@ public:
             ok = renderer.render_to_png(output.string());
         }
         if (!ok) throw std::runtime_error("cannot write rendered frame");
+        if (renderer.config.gpu_tree == 2) renderer.config.gpu_tree = 1;  // rebuilt once
         ++frame;
         timer = double(frame) / framerate;
         instances.clear();

@@ -47,7 +47,9 @@ bool render_image_hip_hdr(const Scene& scene, const Camera& cam, const RenderCon
     static MarshalledScene marshalled;
     marshal_scene(scene, marshalled);
     const rm_scene_desc desc = marshalled.desc(scene);
-    if (rm_set_scene(b.ctx, &desc) != RM_OK) {
+    const int rc = cfg.gpu_tree > 0 ? rm_set_scene_gpu_tree(b.ctx, &desc, cfg.gpu_tree == 2 ? 1 : 0)
+                                    : rm_set_scene(b.ctx, &desc);
+    if (rc != RM_OK) {
         error = rm_last_error(b.ctx);
         return false;
     }

@@ -8,7 +8,8 @@
 #include "raymotion/runtime.hpp"
 
 namespace {
-int builds = 0, frames = 0;
+int builds = 0, frames = 0, gpu_tree = -1;
+bool tree_empty = false;
 std::vector<raymotion::Triangle> captured;
 void expect(bool ok, const char* what) {
     if (!ok) throw std::runtime_error(std::string("runtime check failed: ") + what);
@@ -21,13 +22,20 @@ void Renderer::configure(int w, int h, int spp, int b) {
     config.width = w; config.height = h; config.spp = spp; config.max_bounces = b;
 }
 void Renderer::build(int leaf) { ++builds; build_bvh(scene, leaf); }
-std::vector<uint8_t> Renderer::render(uint64_t, ProgressFn) { captured = scene.tris; ++frames; return std::vector<uint8_t>(3); }
+std::vector<uint8_t> Renderer::render(uint64_t, ProgressFn) {
+    captured = scene.tris;
+    gpu_tree = config.gpu_tree;
+    tree_empty = scene.nodes.empty();
+    ++frames;
+    return std::vector<uint8_t>(3);
+}
 bool Renderer::render_to_ppm(const std::string&, uint64_t, ProgressFn) { return true; }
 bool Renderer::render_to_png(const std::string&, uint64_t, ProgressFn) { return true; }
 }  // namespace raymotion
 
 int main() {
     using namespace raymotion;
+    setenv("RAYMOTION_GPU_BVH", "0", 1);  // the reference's host-tree policy first
     auto model = std::make_shared<Scene>();
     model->add_triangle({0, 0, 0}, {1, 0, 0}, {0, 1, 0}, 0, 0, 0, 0, 0, 0, {0, 0, 1}, {0, 0, 1}, {0, 0, 1}, 1, 1, 1, 0,
                         0, 0, 0, 1, 0.5, 0);
@@ -64,6 +72,25 @@ int main() {
         object.render();
     }
     expect(builds - before == 2, "refit, with a rebuild every 32nd geometry update");
+    expect(gpu_tree == 0, "host tree when RAYMOTION_GPU_BVH=0");
+    // GPU tree (the default): the same policy, carried by config.gpu_tree.
+    setenv("RAYMOTION_GPU_BVH", "1", 1);
+    Objects gpu([](const auto&, int, int, int) {}, 1, 1, 1);
+    const int builds_before = builds;
+    gpu.push(model, {0, 0, 3});
+    gpu.render();
+    expect(gpu_tree == 2 && tree_empty, "first frame: GPU build, no host tree");
+    gpu.push(model, {0, 0, 3});
+    gpu.render();
+    expect(gpu_tree == 1, "camera-only frame keeps the GPU tree");
+    int rebuilds = 0;
+    for (int i = 0; i < 64; ++i) {
+        gpu.push(model, {0, 0, 4.0 + i});  // geometry updates 1..64: rebuilds at 32 and 64
+        gpu.render();
+        rebuilds += gpu_tree == 2;
+        expect(gpu_tree == 1 || gpu_tree == 2, "GPU tree every frame");
+    }
+    expect(rebuilds == 2 && builds == builds_before, "GPU refit, rebuilt every 32nd update, no host build");
     bool threw = false;
     try { object.push(nullptr); } catch (const std::runtime_error&) { threw = true; }
     expect(threw, "null model rejected");
