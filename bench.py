@@ -161,7 +161,7 @@ def run_heightfield(args) -> dict:
                 "boundary": t4 - t1, "rays": st.rays[5], "refit": st.refit, "gpu_ms": st.ms[7],
                 "upload_ms": st.scene_ms[0], "refit_ms": st.scene_ms[1], "pass_ms": [st.ms[i] for i in range(8)],
                 "nodes": [st.node_visits[i] for i in range(6)], "tris": [st.tri_tests[i] for i in range(6)],
-                "prays": [st.rays[i] for i in range(6)], "bvh_nodes": st.bvh_nodes}
+                "prays": [st.rays[i] for i in range(6)], "bvh_nodes": st.bvh_nodes, "stack": st.stack_depth}
 
     for k in range(1, 1 + args.warmup):
         one(k)
@@ -206,7 +206,7 @@ def run_heightfield(args) -> dict:
                                "rebuild every 32", "width": 1920, "height": 1080, "spp": 8, "max_depth": 6,
                    "triangles": ntri, "restir": {"candidates": 8, "spatial": 4, "radius_px": 16, "temporal": True},
                    "atrous_iterations": 2, "rays_per_frame": int(mean["rays"]),
-                   "tree": "GPU linear BVH (rebuilt every 32nd frame, GPU refit otherwise)" if gpu_tree else
+                   "tree": "built on the GPU (rebuilt every 32nd frame, GPU refit otherwise)" if gpu_tree else
                            "host SAH (rebuilt every 32nd frame, FP64 host refit otherwise; GPU refits its copy)",
                    "host_ms": {"scene_update" + ("" if gpu_tree else "_and_fp64_refit"): round(mean["update"] * 1e3, 1),
                                "marshal_and_set_scene": round(mean["upload"] * 1e3, 1),
@@ -215,6 +215,7 @@ def run_heightfield(args) -> dict:
                               "frame_passes": round(mean["gpu_ms"], 3)},
                    "gpu_mrays_per_s": round(mean["rays"] / (mean["gpu_ms"] / 1e3) / 1e6, 2),
                    "refit_gbs": round(refit_bytes / (mean["refit_ms"] / 1e3) / 1e9, 1) if mean["refit_ms"] > 0 else None,
+                   "tree_nodes": rebuild["bvh_nodes"], "tree_stack_depth": rebuild["stack"],
                    "rebuild_frame_ms": round((rebuild["boundary"] + rebuild["update"]) * 1e3, 1),
                    "rebuild_frame_tree": {0: "unchanged", 1: "refit (same topology)", 2: "rebuilt from the host tree",
                                           3: "built on the GPU"}[rebuild["refit"]],
@@ -228,6 +229,8 @@ def run_heightfield(args) -> dict:
                      "bytes_per_launch": int(alg_bytes), "kernel_ms": round(float(pm[dom]), 3)},
     }
     if not args.no_cpu_baseline:
+        if gpu_tree:
+            scene.build_bvh()  # the oracle traverses the host tree
         line["cpu_baseline"] = cpu_baseline(scene.export(), heightfield_frame(32), "heightfield frame 32")
     return line
 
@@ -406,7 +409,8 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint cost measurement in the mixed line")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the CPU oracle timing")
     ap.add_argument("--host-tree", action="store_true",
-                    help="heightfield: host SAH build / FP64 refit instead of the GPU tree build / refit")
+                    help="upload the engine's host SAH tree (and, heightfield, refit it on the host) instead of "
+                         "building the traversal tree on the GPU")
     ap.add_argument("--schedule", type=int, choices=(0, 1, 2), default=0,
                     help="path passes: 0 persistent lanes, 1 wavefront + persistent tail, 2 wavefront only")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -450,7 +454,7 @@ def run_mixed(args) -> dict:
     scene.build_bvh()
     exported = scene.export()
     backend = rm.Backend(0)
-    backend.set_scene(exported)
+    backend.set_scene(exported, gpu_tree=not args.host_tree, rebuild=True)
     backend.set_profiling(True)
     backend.set_schedule(args.schedule)
 
@@ -464,6 +468,7 @@ def run_mixed(args) -> dict:
         wall.append((time.perf_counter() - t0) * 1e3)
         st = backend.stats()
         waves.append((st.wave_iters, st.wave_tail))
+        tree_shape = (st.bvh_nodes, st.stack_depth)
         rays.append(st.rays[5])
         pass_ms.append([st.ms[i] for i in range(8)])
         nodes.append([st.node_visits[i] for i in range(6)])
@@ -509,6 +514,8 @@ def run_mixed(args) -> dict:
                    "pass_nodes_per_ray": {tracing[i]: round(float(nd[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "pass_tris_per_ray": {tracing[i]: round(float(tr[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "gpu_frame_ms": round(float(pm[7]), 3),
+                   "tree": "host SAH (the scene's own)" if args.host_tree else "built on the GPU",
+                   "tree_nodes": tree_shape[0], "tree_stack_depth": tree_shape[1],
                    "schedule": {0: "persistent", 1: "wavefront", 2: "wavefront-only"}[args.schedule],
                    "wavefront_iterations": round(float(np.mean([w[0] for w in waves])), 1),
                    "wavefront_tail_pixels": int(np.mean([w[1] for w in waves]))},

@@ -138,8 +138,8 @@ const char* rm_last_error(const rm_ctx* ctx);
 int rm_set_scene(rm_ctx* ctx, const rm_scene_desc* scene);
 
 /* As rm_set_scene, but the 4-wide tree is built on the GPU from the triangles
- * (a linear BVH over Morton codes of the centroids, collapsed to 4-wide nodes
- * with leaves of at most 4 triangles); scene->nodes and scene->bvh_tri are not
+ * (Morton order, then parallel locally-ordered clustering by surface area,
+ * collapsed to 4-wide nodes with one triangle per leaf slot); scene->nodes and scene->bvh_tri are not
  * used and may be empty. The tree is rebuilt when `rebuild` is nonzero, when
  * the triangle count changed or after 31 refits in a row, and otherwise refit
  * on the GPU when positions moved. Images equal those of rm_set_scene with any

@@ -40,8 +40,11 @@ struct DevBuf {
         if (hipMalloc(&ptr, bytes) != hipSuccess) return false;
         cap = bytes;
         // Fresh buffers start zeroed so history that was never written reads
-        // as "no data" (kind 0, invalid reservoir, black).
-        return hipMemset(ptr, 0, bytes) == hipSuccess;
+        // as "no data" (kind 0, invalid reservoir, black). hipMemset runs on
+        // the null stream, which the context's non-blocking stream does not
+        // wait for: finish it here, or it could land after work the caller
+        // queues next on that stream (an upload into this buffer).
+        return hipMemset(ptr, 0, bytes) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     }
     void release() {
         if (ptr) (void)hipFree(ptr);
@@ -609,7 +612,7 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
     if (ok && (rebuild || refit)) ok = c->geo.ensure(size_t(n) * kTriGeoWords * sizeof(float4));
     if (ok && rebuild)
         ok = c->perm.ensure(size_t(n) * sizeof(uint32_t)) && c->nodes.ensure(max_nodes * kNodeWords * sizeof(float4)) &&
-             c->build_scratch.ensure(gpu_build_bytes(n));
+             c->level_nodes.ensure(max_nodes * sizeof(int32_t)) && c->build_scratch.ensure(gpu_build_bytes(n));
     if (ok && rebuild && !c->build_host) {
         void* h = nullptr;
         ok = hipHostMalloc(&h, 2 * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
@@ -627,28 +630,21 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
         a.geo = c->geo.as<float4>();
         a.perm = c->perm.as<uint32_t>();
         a.nodes = c->nodes.as<float4>();
+        a.level_nodes = c->level_nodes.as<int32_t>();
         a.max_nodes = max_nodes;
         a.scratch = c->build_scratch.ptr;
         a.scratch_bytes = c->build_scratch.cap;
         a.host_words = c->build_host;
         a.n = n;
-        a.leaf_tris = kGpuLeafTris;
         if (int rc = check(c, hipError_t(gpu_build(a, br, c->stream)), "GPU tree build")) return rc;
         if (br.stack_need > kMaxStack)
             return c->fail(RM_ERR_INVALID, "BVH too deep: the traversal stack would need " +
                                                std::to_string(br.stack_need) + " entries (at most " +
                                                std::to_string(kMaxStack) + ")");
-        // Refit order: the levels' contiguous node ranges, deepest level first.
-        const size_t nl = br.level_count.size();
-        std::vector<int32_t> first(nl, 0), order;
-        for (size_t l = 1; l < nl; ++l) first[l] = first[l - 1] + br.level_count[l - 1];
-        order.reserve(size_t(br.nodes));
+        // Refit launch ranges into level_nodes (written by the build), deepest level first.
         c->level_start.assign(1, 0);
-        for (size_t l = nl; l-- > 0;) {
-            for (int k = 0; k < br.level_count[l]; ++k) order.push_back(first[l] + k);
-            c->level_start.push_back(int32_t(order.size()));
-        }
-        if (!upload(c, c->level_nodes, order)) return c->fail(RM_ERR_ALLOC, "HIP scene buffer allocation failed");
+        for (size_t l = br.level_count.size(); l-- > 0;)
+            c->level_start.push_back(c->level_start.back() + br.level_count[l]);
     } else if (refit) {
         launch_gather_tris(c->geo_scene.as<float4>(), c->perm.as<uint32_t>(), n, c->geo.as<float4>(), c->stream);
         for (size_t l = 0; l + 1 < c->level_start.size(); ++l)
@@ -973,6 +969,8 @@ int rm_kpn_get_weights(rm_ctx* c, float* params, size_t n) {
     std::lock_guard<std::mutex> lock(c->mu);
     if (!c->kpn_ready) return c->fail(RM_ERR_INVALID, "no denoiser weights set");
     if (n != kpn_net().nparam) return c->fail(RM_ERR_INVALID, "rm_kpn_get_weights: wrong size");
+    // hipMemcpy runs on the null stream, which does not wait for the context's stream.
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "readback")) return rc;
     if (int rc = check(c, hipMemcpy(params, c->kpn_p.ptr, n * sizeof(float), hipMemcpyDeviceToHost), "readback"))
         return rc;
     return RM_OK;
@@ -1235,6 +1233,8 @@ int rm_read_restir(rm_ctx* c, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoi
     std::lock_guard<std::mutex> lock(c->mu);
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     if (!c->hist_valid) return c->fail(RM_ERR_INVALID, "no ReSTIR frame rendered yet");
+    // hipMemcpy runs on the null stream, which does not wait for the context's stream.
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "readback")) return rc;
     const size_t px = size_t(c->hist_w) * size_t(c->hist_h);
     std::vector<float4> a(px), b(px), d(px), e(px), g(px);
     auto get = [&](const DevBuf& buf, std::vector<float4>& dst) {

@@ -3,18 +3,21 @@
 // topology changes and at every 32nd geometry update (its
 // include/raymotion/runtime.hpp:119-130).
 //
-// Linear BVH: 63-bit Morton codes of the triangle centroids (21 bits per axis
-// over the centroid bounds), a device radix sort of (code, triangle) pairs,
-// the binary radix tree of the sorted codes built with one thread per inner
-// node (Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees,
-// and k-d Trees", HPG 2012; equal codes ordered by position), bounds from the
-// leaves up with one arrival counter per inner node, then a top-down collapse
-// to 4-wide nodes, one launch per output level: each wide node opens the
-// binary children of largest surface area until it has four, children of at
-// most leaf_tris triangles become leaf slots and the others wide nodes of the
-// next level. Nodes of one level are contiguous, which is the order the
-// refit launches need. Traversal returns the closest hit whatever the tree,
-// so images equal those rendered with a host-built tree.
+// 63-bit Morton codes of the triangle centroids (21 bits per axis over the
+// centroid bounds) and a device radix sort put the triangles in spatial order;
+// the binary tree is then built bottom-up by parallel locally-ordered
+// clustering (Meister and Bittner, "Parallel Locally-Ordered Clustering for
+// Bounding Volume Hierarchy Construction", IEEE TVCG 2018): every cluster
+// finds, within kPlocRadius neighbours on either side in that order, the one
+// whose merged box has the least surface area; mutual nearest neighbours merge
+// into a new node, and the surviving clusters are compacted in order, until
+// one is left. A top-down collapse then writes 4-wide nodes, one launch per
+// output level: each wide node opens the binary children of largest surface
+// area until it has four; single triangles become leaf slots, the others wide
+// nodes of the next level. The nodes are then renumbered depth-first (a node
+// next to its first child, as in the host tree's preorder) and listed level by
+// level for the refit launches. Traversal returns the closest hit whatever the
+// tree, so images equal those rendered with a host-built tree.
 #include <hipcub/hipcub.hpp>
 
 #include "rm_kernels.h"
@@ -121,109 +124,141 @@ __global__ __launch_bounds__(kB) void k_gather(const float4* __restrict__ src, c
     dst[w] = src[size_t(perm[k]) * kTriGeoWords + part];
 }
 
-// Common-prefix length of sorted codes i and j (equal codes continue with the
-// positions), -1 outside [0, n).
-__device__ __forceinline__ int delta(const uint64_t* __restrict__ keys, int n, int i, long long j) {
-    if (j < 0 || j >= n) return -1;
-    const uint64_t a = keys[i], b = keys[j];
-    if (a == b) return 64 + __clz(uint32_t(i) ^ uint32_t(j));
-    return __clzll((long long)(a ^ b));
-}
-
-// Inner node i of the radix tree (n >= 2). Node ids: inner 0 .. n-2, leaf k
-// as n - 1 + k. range[i] = first and last sorted triangle below.
-__global__ __launch_bounds__(kB) void k_radix_tree(const uint64_t* __restrict__ keys, int n, int2* __restrict__ child,
-                                                   int* __restrict__ parent, int2* __restrict__ range) {
-    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
-    if (i >= n - 1) return;
-    const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
-    const int dmin = delta(keys, n, i, (long long)i - d);
-    long long lmax = 2;
-    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    long long l = 0;
-    for (long long t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-    const long long j = i + l * d;
-    const int dnode = delta(keys, n, i, j);
-    long long s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = int(i + s * d + (d < 0 ? -1 : 0));
-    const int first = int(min((long long)i, j)), last = int(max((long long)i, j));
-    const int left = first == gamma ? n - 1 + gamma : gamma;
-    const int right = last == gamma + 1 ? n - 1 + gamma + 1 : gamma + 1;
-    child[i] = make_int2(left, right);
-    range[i] = make_int2(first, last);
-    parent[left] = i;
-    parent[right] = i;
-}
-
 __device__ __forceinline__ void store_box(float* __restrict__ box, int node, const TriBox& b) {
-    float* p = box + size_t(node) * 8;
-    for (int a = 0; a < 3; ++a) {
-        __hip_atomic_store(p + a, b.lo[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(p + 4 + a, b.hi[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    float4* p = reinterpret_cast<float4*>(box + size_t(node) * 8);
+    p[0] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+    p[1] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
 }
-__device__ __forceinline__ TriBox load_box(float* __restrict__ box, int node) {
-    const float* p = box + size_t(node) * 8;
+__device__ __forceinline__ TriBox load_box(const float* __restrict__ box, int node) {
+    const float4* p = reinterpret_cast<const float4*>(box + size_t(node) * 8);
+    const float4 l = p[0], h = p[1];
     TriBox b;
-    for (int a = 0; a < 3; ++a) {
-        b.lo[a] = __hip_atomic_load(p + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        b.hi[a] = __hip_atomic_load(p + 4 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    b.lo[0] = l.x; b.lo[1] = l.y; b.lo[2] = l.z;
+    b.hi[0] = h.x; b.hi[1] = h.y; b.hi[2] = h.z;
     return b;
 }
+__device__ __forceinline__ float half_area(const TriBox& b) {
+    const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+    return x * y + y * z + z * x;
+}
 
-// Boxes of the inner nodes, from the leaves up: the second child to arrive
-// at a node computes its box and continues to the parent.
-__global__ __launch_bounds__(kB) void k_tree_bounds(const float4* __restrict__ geo, int n, const int2* __restrict__ child,
-                                                    const int* __restrict__ parent, int* __restrict__ arrivals,
-                                                    float* __restrict__ box) {
+constexpr int kPlocRadius = 16;
+
+// Leaf clusters in sorted order: id n - 1 + k for sorted triangle k (inner
+// nodes take ids 0 .. n - 2 in order of creation).
+__global__ __launch_bounds__(kB) void k_ploc_init(const float4* __restrict__ geo, int n, int* __restrict__ cid,
+                                                  float4* __restrict__ clo, float4* __restrict__ chi) {
     const int k = int(blockIdx.x) * kB + int(threadIdx.x);
     if (k >= n) return;
-    int node = parent[n - 1 + k];
-    while (true) {
-        __threadfence();
-        if (atomicAdd(arrivals + node, 1) == 0) return;  // the other child finishes this node
-        __threadfence();
-        const int2 ch = child[node];
-        const int c2[2] = {ch.x, ch.y};
-        TriBox b;
-        for (int a = 0; a < 3; ++a) {
-            b.lo[a] = __builtin_inff();
-            b.hi[a] = -__builtin_inff();
+    const TriBox b = tri_box(geo + size_t(k) * kTriGeoWords);
+    cid[k] = n - 1 + k;
+    clo[k] = make_float4(b.lo[0], b.lo[1], b.lo[2], 0.0f);
+    chi[k] = make_float4(b.hi[0], b.hi[1], b.hi[2], 0.0f);
+}
+
+// Nearest neighbour of every cluster within kPlocRadius positions: least
+// merged surface area, ties to the pair of smaller (lower, then higher) index,
+// a total order on pairs, so the best pair overall is always mutual.
+__global__ __launch_bounds__(kB) void k_ploc_nn(const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                const unsigned* __restrict__ count, int* __restrict__ nn) {
+    __shared__ float4 slo[kB + 2 * kPlocRadius], shi[kB + 2 * kPlocRadius];
+    const int m = int(*count);
+    const int b0 = int(blockIdx.x) * kB;
+    if (b0 >= m) return;
+    for (int t = int(threadIdx.x); t < kB + 2 * kPlocRadius; t += kB) {
+        const int g = b0 - kPlocRadius + t;
+        if (g >= 0 && g < m) {
+            slo[t] = clo[g];
+            shi[t] = chi[g];
         }
-        for (int c = 0; c < 2; ++c) {
-            const TriBox cb = c2[c] >= n - 1 ? tri_box(geo + size_t(c2[c] - (n - 1)) * kTriGeoWords)
-                                             : load_box(box, c2[c]);
-            for (int a = 0; a < 3; ++a) {
-                b.lo[a] = fminf(b.lo[a], cb.lo[a]);
-                b.hi[a] = fmaxf(b.hi[a], cb.hi[a]);
-            }
-        }
-        store_box(box, node, b);
-        if (node == 0) return;
-        node = parent[node];
     }
+    __syncthreads();
+    const int i = b0 + int(threadIdx.x);
+    if (i >= m) return;
+    const int li = int(threadIdx.x) + kPlocRadius;
+    const float4 al = slo[li], ah = shi[li];
+    float best = __builtin_inff();
+    int bj = -1;
+    const int j0 = max(0, i - kPlocRadius), j1 = min(m - 1, i + kPlocRadius);
+    for (int j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float4 bl = slo[j - b0 + kPlocRadius], bh = shi[j - b0 + kPlocRadius];
+        const float x = fmaxf(ah.x, bh.x) - fminf(al.x, bl.x);
+        const float y = fmaxf(ah.y, bh.y) - fminf(al.y, bl.y);
+        const float z = fmaxf(ah.z, bh.z) - fminf(al.z, bl.z);
+        const float c = x * y + y * z + z * x;
+        // Candidates come in increasing j: for equal cost the first one has the
+        // smaller pair (min(i, j), max(i, j)) as well.
+        if (c < best) {
+            best = c;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// Mutual pairs (i < j) merge into a new inner node held by cluster i; cluster
+// j is dropped (keep = 0).
+__global__ __launch_bounds__(kB) void k_ploc_merge(const unsigned* __restrict__ count, int* __restrict__ cid,
+                                                   float4* __restrict__ clo, float4* __restrict__ chi,
+                                                   const int* __restrict__ nn, int* __restrict__ keep,
+                                                   unsigned* __restrict__ next_id, int2* __restrict__ child,
+                                                   float* __restrict__ box) {
+    const int m = int(*count);
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    if (!mutual) {
+        keep[i] = 1;
+        return;
+    }
+    if (i > j) {
+        keep[i] = 0;
+        return;
+    }
+    const int a = cid[i], b = cid[j];
+    const int node = int(atomicAdd(next_id, 1u));
+    const float4 al = clo[i], ah = chi[i], bl = clo[j], bh = chi[j];
+    TriBox u;
+    u.lo[0] = fminf(al.x, bl.x); u.lo[1] = fminf(al.y, bl.y); u.lo[2] = fminf(al.z, bl.z);
+    u.hi[0] = fmaxf(ah.x, bh.x); u.hi[1] = fmaxf(ah.y, bh.y); u.hi[2] = fmaxf(ah.z, bh.z);
+    child[node] = make_int2(a, b);
+    store_box(box, node, u);
+    cid[i] = node;
+    clo[i] = make_float4(u.lo[0], u.lo[1], u.lo[2], 0.0f);
+    chi[i] = make_float4(u.hi[0], u.hi[1], u.hi[2], 0.0f);
+    keep[i] = 1;
+}
+
+// Surviving clusters to their positions (exclusive sum of keep), order kept;
+// the last thread writes the new count.
+__global__ __launch_bounds__(kB) void k_ploc_compact(const unsigned* __restrict__ count, const int* __restrict__ keep,
+                                                     const int* __restrict__ pos, const int* __restrict__ cid,
+                                                     const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                     int* __restrict__ cid2, float4* __restrict__ clo2,
+                                                     float4* __restrict__ chi2, unsigned* __restrict__ count2) {
+    const int m = int(*count);
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i >= m) return;
+    if (keep[i]) {
+        const int p = pos[i];
+        cid2[p] = cid[i];
+        clo2[p] = clo[i];
+        chi2[p] = chi[i];
+    }
+    if (i == m - 1) *count2 = unsigned(pos[i] + keep[i]);
 }
 
 struct BuildTree {
     const float4* geo;  // leaf order
     int n;
     const int2* child;
-    const int2* range;
-    float* box;
+    const float* box;
 };
 
 __device__ __forceinline__ TriBox node_box(const BuildTree& T, int id) {
     return id >= T.n - 1 ? tri_box(T.geo + size_t(id - (T.n - 1)) * kTriGeoWords) : load_box(T.box, id);
-}
-__device__ __forceinline__ float half_area(const TriBox& b) {
-    const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
-    return x * y + y * z + z * x;
 }
 
 // One output level: item i (binary node in[i].x, traversal stack depth above
@@ -231,8 +266,7 @@ __device__ __forceinline__ float half_area(const TriBox& b) {
 // (their wide index base + count + position) through *out_count.
 __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __restrict__ in, int count, int base,
                                                  int2* __restrict__ out, unsigned* __restrict__ out_count,
-                                                 float4* __restrict__ nodes, int* __restrict__ stack_need,
-                                                 int leaf_tris) {
+                                                 float4* __restrict__ nodes, int* __restrict__ stack_need) {
     const int i = int(blockIdx.x) * kB + int(threadIdx.x);
     if (i >= count) return;
     const int b = in[i].x, stack_here = in[i].y;
@@ -280,11 +314,9 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
             lo[a][s] = bx.lo[a];
             hi[a][s] = bx.hi[a];
         }
-        const int first = id >= T.n - 1 ? id - (T.n - 1) : T.range[id].x;
-        const int tris = id >= T.n - 1 ? 1 : T.range[id].y - T.range[id].x + 1;
-        if (tris <= leaf_tris) {
-            ref[s] = first;
-            kind[s] = tris;
+        if (id >= T.n - 1) {
+            ref[s] = id - (T.n - 1);  // leaf-order triangle
+            kind[s] = 1;
         } else {
             ++inner;
             kind[s] = 0;
@@ -313,7 +345,125 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
                        __int_as_float(kind[3]));
 }
 
+// Depth-first renumbering of the collapsed tree (which comes out level by
+// level): subtree sizes from the deepest level up, then indices from the root
+// down, each inner child's subtree following its parent's earlier children, so
+// a node sits next to its first child as in the host tree's preorder.
+__device__ __forceinline__ void node_links(const float4* __restrict__ node, int ref[4], int kind[4]) {
+    const float4 rf = node[6], kd = node[7];
+    ref[0] = __float_as_int(rf.x); ref[1] = __float_as_int(rf.y); ref[2] = __float_as_int(rf.z);
+    ref[3] = __float_as_int(rf.w);
+    kind[0] = __float_as_int(kd.x); kind[1] = __float_as_int(kd.y); kind[2] = __float_as_int(kd.z);
+    kind[3] = __float_as_int(kd.w);
+}
+
+__global__ __launch_bounds__(kB) void k_subtree_size(const float4* __restrict__ nodes, int first, int count,
+                                                     int* __restrict__ size) {
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i >= count) return;
+    int ref[4], kind[4];
+    node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
+    int s = 1;
+    for (int c = 0; c < 4; ++c)
+        if (kind[c] == 0) s += size[ref[c]];
+    size[first + i] = s;
+}
+
+__global__ __launch_bounds__(kB) void k_dfs_index(const float4* __restrict__ nodes, int first, int count,
+                                                  const int* __restrict__ size, int* __restrict__ index) {
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i >= count) return;
+    int ref[4], kind[4];
+    node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
+    int next = index[first + i] + 1;
+    for (int c = 0; c < 4; ++c)
+        if (kind[c] == 0) {
+            index[ref[c]] = next;
+            next += size[ref[c]];
+        }
+}
+
+// Node i to index[i], inner-child references renumbered.
+__global__ __launch_bounds__(kB) void k_renumber(const float4* __restrict__ src, int total,
+                                                 const int* __restrict__ index, float4* __restrict__ dst) {
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i >= total) return;
+    const float4* a = src + size_t(i) * kNodeWords;
+    float4* b = dst + size_t(index[i]) * kNodeWords;
+    int ref[4], kind[4];
+    node_links(a, ref, kind);
+    for (int c = 0; c < 4; ++c)
+        if (kind[c] == 0) ref[c] = index[ref[c]];
+    for (int w = 0; w < 6; ++w) b[w] = a[w];
+    b[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+    b[7] = a[7];
+}
+
+// Refit order: the new indices of one level's nodes at out[0 .. count).
+__global__ __launch_bounds__(kB) void k_level_order(const int* __restrict__ index, int first, int count,
+                                                    int32_t* __restrict__ out) {
+    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    if (i < count) out[i] = index[first + i];
+}
+
 unsigned blocks(size_t n) { return unsigned((n + kB - 1) / kB); }
+
+// Device scratch layout of one build (256-byte aligned pieces).
+struct Scratch {
+    uint64_t *keys_in, *keys;
+    uint32_t* vals_in;
+    int2* child;
+    float* box;
+    int* cid[2];
+    float4 *clo[2], *chi[2];
+    int *nn, *keep, *pos;
+    int2* lvl[2];
+    float4* wide;        // collapsed nodes in level order, before renumbering
+    int *size, *index;   // per collapsed node
+    uint32_t* ctl;  // 6 centroid-bound words, cluster counts (2), next inner id, level count, stack need
+    void *sort_tmp, *scan_tmp;
+    size_t sort_bytes, scan_bytes, total;
+};
+
+Scratch carve(void* base, int n) {
+    const size_t N = size_t(n > 1 ? n : 2);  // also bounds the 4-wide node count (n - 1, or 1)
+    Scratch s{};
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, s.sort_bytes, static_cast<const uint64_t*>(nullptr),
+                                             static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
+                                             static_cast<uint32_t*>(nullptr), int(N), 0, 63);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, s.scan_bytes, static_cast<const int*>(nullptr),
+                                           static_cast<int*>(nullptr), int(N));
+    size_t off = 0;  // with base == nullptr only the size is wanted
+    auto take = [&](size_t bytes) {
+        void* q = base ? static_cast<void*>(static_cast<char*>(base) + off) : nullptr;
+        off += (bytes + 255) & ~size_t(255);
+        return q;
+    };
+    s.keys_in = static_cast<uint64_t*>(take(N * 8));
+    s.keys = static_cast<uint64_t*>(take(N * 8));
+    s.vals_in = static_cast<uint32_t*>(take(N * 4));
+    s.child = static_cast<int2*>(take(N * 8));
+    s.box = static_cast<float*>(take(N * 32));
+    for (int k = 0; k < 2; ++k) {
+        s.cid[k] = static_cast<int*>(take(N * 4));
+        s.clo[k] = static_cast<float4*>(take(N * 16));
+        s.chi[k] = static_cast<float4*>(take(N * 16));
+        s.lvl[k] = static_cast<int2*>(take(N * 8));
+    }
+    s.nn = static_cast<int*>(take(N * 4));
+    s.keep = static_cast<int*>(take(N * 4));
+    s.pos = static_cast<int*>(take(N * 4));
+    s.wide = static_cast<float4*>(take(N * kNodeWords * sizeof(float4)));
+    s.size = static_cast<int*>(take(N * 4));
+    s.index = static_cast<int*>(take(N * 4));
+    s.ctl = static_cast<uint32_t*>(take(64));
+    s.sort_tmp = take(s.sort_bytes);
+    s.scan_tmp = take(s.scan_bytes);
+    s.total = off;
+    return s;
+}
+
+enum : int { kCtlCount0 = 6, kCtlCount1 = 7, kCtlNextId = 8, kCtlLevel = 9, kCtlStack = 10 };
 
 }  // namespace
 
@@ -322,71 +472,68 @@ void launch_gather_tris(const float4* src, const uint32_t* perm, int n, float4* 
     hipLaunchKernelGGL(k_gather, dim3(blocks(size_t(n) * kTriGeoWords)), dim3(kB), 0, st, src, perm, n, dst);
 }
 
-size_t gpu_build_bytes(int n) {
-    const size_t N = size_t(n > 1 ? n : 2);
-    size_t sort_bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, static_cast<const uint64_t*>(nullptr),
-                                             static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
-                                             static_cast<uint32_t*>(nullptr), int(N), 0, 63);
-    // keys x2, values (the second is the permutation, kept), child, range,
-    // parent, arrivals, boxes, two level lists, bounds, counters, sort scratch
-    return N * (8 * 2 + 4 + 8 + 8 + 4 * 2 + 4 + 32 + 8 * 2) + 16 * 256 + ((sort_bytes + 255) & ~size_t(255));
-}
+size_t gpu_build_bytes(int n) { return carve(nullptr, n).total; }
 
 int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st) {
     const int n = a.n;
     r = GpuBuildResult{};
     if (n <= 0) return int(hipErrorInvalidValue);
-    const size_t N = size_t(n > 1 ? n : 2);
-    char* p = static_cast<char*>(a.scratch);
-    auto take = [&](size_t bytes) {
-        char* q = p;
-        p += (bytes + 255) & ~size_t(255);
-        return static_cast<void*>(q);
-    };
-    uint64_t* keys_in = static_cast<uint64_t*>(take(N * 8));
-    uint64_t* keys = static_cast<uint64_t*>(take(N * 8));
-    uint32_t* vals_in = static_cast<uint32_t*>(take(N * 4));
-    int2* child = static_cast<int2*>(take(N * 8));
-    int2* range = static_cast<int2*>(take(N * 8));
-    int* parent = static_cast<int*>(take(N * 8));
-    int* arrivals = static_cast<int*>(take(N * 4));
-    float* box = static_cast<float*>(take(N * 32));
-    int2* lvl[2] = {static_cast<int2*>(take(N * 8)), static_cast<int2*>(take(N * 8))};
-    uint32_t* ctl = static_cast<uint32_t*>(take(64));  // 6 centroid-bound words, out count, stack need
-    size_t sort_bytes = 0;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys_in, keys, vals_in, a.perm, n, 0, 63, st);
-    if (e != hipSuccess) return int(e);
-    void* sort_tmp = take(sort_bytes);
-    if (size_t(p - static_cast<char*>(a.scratch)) > a.scratch_bytes) return int(hipErrorInvalidValue);
-
-    const uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, 0u, 0u};
-    if ((e = hipMemcpyAsync(ctl, init, sizeof(init), hipMemcpyHostToDevice, st)) != hipSuccess) return int(e);
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, ctl);
-    hipLaunchKernelGGL(k_morton, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, ctl, keys_in, vals_in);
-    if ((e = hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_bytes, keys_in, keys, vals_in, a.perm, n, 0, 63, st)) !=
-        hipSuccess)
+    Scratch s = carve(a.scratch, n);
+    if (s.total > a.scratch_bytes) return int(hipErrorInvalidValue);
+    hipError_t e;
+    uint32_t init[11] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, uint32_t(n), 0u, 0u, 0u, 0u};
+    if ((e = hipMemcpyAsync(s.ctl, init, sizeof(init), hipMemcpyHostToDevice, st)) != hipSuccess) return int(e);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, s.ctl);
+    hipLaunchKernelGGL(k_morton, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, s.ctl, s.keys_in,
+                       s.vals_in);
+    if ((e = hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, s.sort_bytes, s.keys_in, s.keys, s.vals_in, a.perm, n, 0,
+                                                63, st)) != hipSuccess)
         return int(e);
     hipLaunchKernelGGL(k_gather, dim3(blocks(size_t(n) * kTriGeoWords)), dim3(kB), 0, st, a.geo_scene, a.perm, n,
                        a.geo);
+    // Clustering. The host reads the cluster count after every round (it
+    // only sizes the grids and ends the loop; the kernels read it on the GPU).
+    // The collapse starts from the root: the last cluster, or the only leaf.
+    const int2 leaf_root = make_int2(0, 0);
+    if ((e = hipMemcpyAsync(s.lvl[0], &leaf_root, sizeof(leaf_root), hipMemcpyHostToDevice, st)) != hipSuccess)
+        return int(e);
     if (n >= 2) {
-        hipLaunchKernelGGL(k_radix_tree, dim3(blocks(size_t(n - 1))), dim3(kB), 0, st, keys, n, child, parent, range);
-        if ((e = hipMemsetAsync(arrivals, 0, size_t(n) * 4, st)) != hipSuccess) return int(e);
-        hipLaunchKernelGGL(k_tree_bounds, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo, n, child, parent, arrivals,
-                           box);
+        hipLaunchKernelGGL(k_ploc_init, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo, n, s.cid[0], s.clo[0],
+                           s.chi[0]);
+        int m = n, cur = 0, rounds = 0;
+        while (m > 1) {
+            if (++rounds > 100000) return int(hipErrorUnknown);
+            unsigned* cnt = s.ctl + kCtlCount0 + cur;
+            unsigned* cnt2 = s.ctl + kCtlCount0 + (cur ^ 1);
+            const dim3 g(blocks(size_t(m)));
+            hipLaunchKernelGGL(k_ploc_nn, g, dim3(kB), 0, st, s.clo[cur], s.chi[cur], cnt, s.nn);
+            hipLaunchKernelGGL(k_ploc_merge, g, dim3(kB), 0, st, cnt, s.cid[cur], s.clo[cur], s.chi[cur], s.nn,
+                               s.keep, s.ctl + kCtlNextId, s.child, s.box);
+            if ((e = hipcub::DeviceScan::ExclusiveSum(s.scan_tmp, s.scan_bytes, s.keep, s.pos, m, st)) != hipSuccess)
+                return int(e);
+            hipLaunchKernelGGL(k_ploc_compact, g, dim3(kB), 0, st, cnt, s.keep, s.pos, s.cid[cur], s.clo[cur],
+                               s.chi[cur], s.cid[cur ^ 1], s.clo[cur ^ 1], s.chi[cur ^ 1], cnt2);
+            if ((e = hipMemcpyAsync(a.host_words, cnt2, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess)
+                return int(e);
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) return int(e);
+            const int m2 = int(a.host_words[0]);
+            if (m2 >= m || m2 < 1) return int(hipErrorUnknown);  // every round merges at least one pair
+            m = m2;
+            cur ^= 1;
+        }
+        if ((e = hipMemcpyAsync(s.lvl[0], s.cid[cur], sizeof(int), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+            return int(e);
     }
     // Collapse, one level per launch; the host reads each level's size.
-    const int2 root = make_int2(n >= 2 ? 0 : n - 1, 0);
-    if ((e = hipMemcpyAsync(lvl[0], &root, sizeof(root), hipMemcpyHostToDevice, st)) != hipSuccess) return int(e);
-    const BuildTree T{a.geo, n, child, range, box};
+    const BuildTree T{a.geo, n, s.child, s.box};
     int base = 0, count = 1, cur = 0;
     r.level_count.clear();
     while (count > 0) {
         if (size_t(base) + size_t(count) > a.max_nodes) return int(hipErrorInvalidValue);
-        if ((e = hipMemsetAsync(ctl + 6, 0, 4, st)) != hipSuccess) return int(e);
-        hipLaunchKernelGGL(k_collapse, dim3(blocks(size_t(count))), dim3(kB), 0, st, T, lvl[cur], count, base,
-                           lvl[cur ^ 1], ctl + 6, a.nodes, reinterpret_cast<int*>(ctl + 7), a.leaf_tris);
-        if ((e = hipMemcpyAsync(a.host_words, ctl + 6, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
+        if ((e = hipMemsetAsync(s.ctl + kCtlLevel, 0, 4, st)) != hipSuccess) return int(e);
+        hipLaunchKernelGGL(k_collapse, dim3(blocks(size_t(count))), dim3(kB), 0, st, T, s.lvl[cur], count, base,
+                           s.lvl[cur ^ 1], s.ctl + kCtlLevel, s.wide, reinterpret_cast<int*>(s.ctl + kCtlStack));
+        if ((e = hipMemcpyAsync(a.host_words, s.ctl + kCtlLevel, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
             hipSuccess)
             return int(e);
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return int(e);
@@ -397,6 +544,24 @@ int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st) {
     }
     r.nodes = base;
     r.stack_need = int(a.host_words[1]);
+    // Depth-first renumbering, and the refit order (deepest level first).
+    const int levels = int(r.level_count.size());
+    std::vector<int> first(size_t(levels), 0);
+    for (int l = 1; l < levels; ++l) first[size_t(l)] = first[size_t(l - 1)] + r.level_count[size_t(l - 1)];
+    for (int l = levels; l-- > 0;)
+        hipLaunchKernelGGL(k_subtree_size, dim3(blocks(size_t(r.level_count[size_t(l)]))), dim3(kB), 0, st, s.wide,
+                           first[size_t(l)], r.level_count[size_t(l)], s.size);
+    if ((e = hipMemsetAsync(s.index, 0, sizeof(int), st)) != hipSuccess) return int(e);  // the root stays node 0
+    for (int l = 0; l < levels; ++l)
+        hipLaunchKernelGGL(k_dfs_index, dim3(blocks(size_t(r.level_count[size_t(l)]))), dim3(kB), 0, st, s.wide,
+                           first[size_t(l)], r.level_count[size_t(l)], s.size, s.index);
+    hipLaunchKernelGGL(k_renumber, dim3(blocks(size_t(r.nodes))), dim3(kB), 0, st, s.wide, r.nodes, s.index, a.nodes);
+    int at = 0;
+    for (int l = levels; l-- > 0;) {
+        hipLaunchKernelGGL(k_level_order, dim3(blocks(size_t(r.level_count[size_t(l)]))), dim3(kB), 0, st, s.index,
+                           first[size_t(l)], r.level_count[size_t(l)], a.level_nodes + at);
+        at += r.level_count[size_t(l)];
+    }
     return int(hipGetLastError());
 }
 

@@ -81,28 +81,28 @@ struct AdjointArgs {
 };
 void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams& P, const AdjointArgs& A,
                     unsigned long long* cnt, hipStream_t st);
-// GPU tree build (rm_build.hip): linear BVH over the triangle centroids,
-// collapsed to 4-wide nodes with leaves of at most leaf_tris triangles.
-constexpr int kGpuLeafTris = 4;
+// GPU tree build (rm_build.hip): triangles in Morton order, clustered into a
+// binary tree, collapsed to 4-wide nodes with one triangle per leaf slot.
 struct GpuBuildArgs {
     const float4* geo_scene;  // n triangle records, scene order (input)
     float4* geo;              // n triangle records, leaf order (output)
     uint32_t* perm;           // n: scene index of leaf-order triangle k (output)
-    float4* nodes;            // room for max_nodes 4-wide nodes (output)
+    float4* nodes;            // room for max_nodes 4-wide nodes (output), depth-first order
+    int32_t* level_nodes;     // room for max_nodes: node indices level by level, deepest level first (output)
     size_t max_nodes;         // at least max(n - 1, 1)
     void* scratch;            // gpu_build_bytes(n) bytes
     size_t scratch_bytes;
     uint32_t* host_words;     // 2 words of pinned host memory
     int n;
-    int leaf_tris;
 };
 struct GpuBuildResult {
     int nodes = 0;
     int stack_need = 0;
-    std::vector<int> level_count;  // 4-wide nodes per level, root level first; each level is contiguous
+    std::vector<int> level_count;  // 4-wide nodes per level, root level first
 };
 size_t gpu_build_bytes(int n);
-// Returns a hipError_t code; waits on the stream once per output level.
+// Returns a hipError_t code; waits on the stream once per clustering round
+// and once per output level.
 int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st);
 // dst[k] = src[perm[k]] for n triangle records.
 void launch_gather_tris(const float4* src, const uint32_t* perm, int n, float4* dst, hipStream_t st);
