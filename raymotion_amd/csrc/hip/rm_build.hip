@@ -142,7 +142,7 @@ __device__ __forceinline__ float half_area(const TriBox& b) {
     return x * y + y * z + z * x;
 }
 
-constexpr int kPlocRadius = 16;
+constexpr int kPlocRadius = 8;  // measured against 16 and 32: equal or faster frames, faster build
 
 // Leaf clusters in sorted order: id n - 1 + k for sorted triangle k (inner
 // nodes take ids 0 .. n - 2 in order of creation).
