@@ -32,6 +32,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -65,8 +66,9 @@ PIXEL_BYTES = {  # per pixel and launch: reads + writes of the pass's own buffer
 # (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy's readings), committed as
 # a table in this summary; bench.py cannot run the profiler itself.
 PROFILE = ROOT / "profiles" / "restir_shade_c1_720p16spp.md"
-PROFILE_KERNEL = {"restir_shade": "k_persistent<32, PathEngine<32, true> >",
-                  "restir_generate": "k_persistent<32, GenEngine<32> >", "restir_spatial": "k_spatial<32>"}
+# kernel names per pass in that table, any traversal-stack instance
+PROFILE_KERNEL = {"restir_shade": r"k_persistent<\d+, PathEngine<\d+, true> >",
+                  "restir_generate": r"k_persistent<\d+, GenEngine<\d+> >", "restir_spatial": r"k_spatial<\d+>"}
 
 
 def measured_traffic(name: str):
@@ -82,7 +84,7 @@ def measured_traffic(name: str):
         cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
         if len(cells) != 5:
             break
-        if cells[0] == PROFILE_KERNEL[name]:
+        if re.fullmatch(PROFILE_KERNEL[name], cells[0]):
             return int((float(cells[3]) + float(cells[4])) * 1e6)
     return None
 
