@@ -115,7 +115,7 @@ struct rm_ctx {
     DevBuf shade, den_a, hist_color;  // float4 per pixel
     DevBuf rgb8, hdr3, mega_hdr;
     DevBuf counters;
-    DevBuf queue;  // work counters of the persistent passes: [0] generate, [1] shade / megakernel, [2..3] shade order
+    DevBuf queue;  // kQueueWords work counters of the persistent passes (band counters, shade order counts)
     DevBuf shade_order;  // restir_shade work order, one pixel index per entry
     // Wavefront schedule: two slot-state buffers, hits, control words, and a
     // pinned read-back of the active count with its events.
@@ -272,8 +272,8 @@ int path_pass(rm_ctx* c, bool shade, const DevScene& sc, const FrameParams& P, c
             }
             order = c->shade_order.as<unsigned>();
         }
-        launch_wave(c->stack_need, shade, sc, P, gb, rm, hdr, rgb, lum_var, c->queue.as<unsigned>() + 1, order, cnt,
-                    c->stream);
+        launch_wave(c->stack_need, shade, sc, P, gb, rm, hdr, rgb, lum_var, c->queue.as<unsigned>() + kQueuePath,
+                    order, c->queue.as<unsigned>() + kQueueOrder, cnt, c->stream);
         return RM_OK;
     }
     WaveBufs wb{};
@@ -383,7 +383,7 @@ int rm_create(rm_ctx** out, int hip_device) {
             c->profiling = false;
         }
     }
-    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(4 * sizeof(unsigned))) {
+    if (!c->counters.ensure(kNumCounters * sizeof(unsigned long long)) || !c->queue.ensure(kQueueWords * sizeof(unsigned))) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         g_create_error = "HIP counter allocation failed";
@@ -722,7 +722,7 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
     unsigned long long* cnt = c->counters.as<unsigned long long>();
     if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
         return rc;
-    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, 4 * sizeof(unsigned), c->stream), "work queue")) return rc;
+    if (int rc = check(c, hipMemsetAsync(c->queue.ptr, 0, kQueueWords * sizeof(unsigned), c->stream), "work queue")) return rc;
     rm_stats st{};
     st.bvh_nodes = c->nnodes;
     st.stack_depth = c->stack_need;
@@ -746,7 +746,7 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
         }
         const GBufferSoA gb = c->gbuf.soa(), hg = c->hist_gbuf.soa();
         const ReservoirSoA ri = c->rinit.soa(), rmg = c->rmerged.soa(), hr = c->hist_res.soa();
-        launch_generate(c->stack_need, sc, P, gb, ri, c->queue.as<unsigned>(), cnt, c->stream);
+        launch_generate(c->stack_need, sc, P, gb, ri, c->queue.as<unsigned>() + kQueueGenerate, cnt, c->stream);
         mark(1);
         launch_spatial(c->stack_need, sc, P, gb, ri, rmg, hg, hr, cnt, c->stream);
         mark(2);

@@ -20,6 +20,8 @@
 // tree, so images equal those rendered with a host-built tree.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "rm_kernels.h"
 
 namespace rmh {
@@ -67,25 +69,40 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// Centroid bounds into cb[0..2] (min) and cb[3..5] (max), as ordered words.
+// Centroid bounds into cb[0..2] (min) and cb[3..5] (max), as ordered words:
+// a grid-stride pass, wave then block reductions, six atomics per block.
 __global__ __launch_bounds__(kB) void k_centroid_bounds(const float4* __restrict__ geo, int n, uint32_t* cb) {
-    const int i = int(blockIdx.x) * kB + int(threadIdx.x);
+    __shared__ float red[kB / 64][6];
     float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
     float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-    if (i < n) {
+    for (int i = int(blockIdx.x) * kB + int(threadIdx.x); i < n; i += int(gridDim.x) * kB) {
         float c[3];
         centroid(geo + size_t(i) * kTriGeoWords, c);
-        for (int a = 0; a < 3; ++a) lo[a] = hi[a] = c[a];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], c[a]);
+            hi[a] = fmaxf(hi[a], c[a]);
+        }
     }
     for (int a = 0; a < 3; ++a) {
         lo[a] = wave_min(lo[a]);
         hi[a] = wave_max(hi[a]);
     }
-    if (__lane_id() == 0 && i - int(__lane_id()) < n)
+    const int wave = int(threadIdx.x) / 64;
+    if (__lane_id() == 0)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(cb + a, ordered(lo[a]));
-            atomicMax(cb + 3 + a, ordered(hi[a]));
+            red[wave][a] = lo[a];
+            red[wave][3 + a] = hi[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = int(threadIdx.x);
+        float v = red[0][k];
+        for (int w = 1; w < kB / 64; ++w) v = k < 3 ? fminf(v, red[w][k]) : fmaxf(v, red[w][k]);
+        if (v == v && (k < 3 ? v < __builtin_inff() : v > -__builtin_inff())) {  // blocks without triangles skip
+            if (k < 3) atomicMin(cb + k, ordered(v));
+            else atomicMax(cb + k, ordered(v));
+        }
+    }
 }
 
 __device__ __forceinline__ uint64_t spread21(uint32_t v) {
@@ -483,7 +500,8 @@ int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st) {
     hipError_t e;
     uint32_t init[11] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, uint32_t(n), 0u, 0u, 0u, 0u};
     if ((e = hipMemcpyAsync(s.ctl, init, sizeof(init), hipMemcpyHostToDevice, st)) != hipSuccess) return int(e);
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, s.ctl);
+    const unsigned bound_blocks = std::min(blocks(size_t(n)), 1024u);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(bound_blocks), dim3(kB), 0, st, a.geo_scene, n, s.ctl);
     hipLaunchKernelGGL(k_morton, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, s.ctl, s.keys_in,
                        s.vals_in);
     if ((e = hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, s.sort_bytes, s.keys_in, s.keys, s.vals_in, a.perm, n, 0,

@@ -22,10 +22,17 @@ inline auto with_stack(int stack, F&& f) {
     return f(std::integral_constant<int, 64>{});
 }
 
+// Work queues of the persistent passes: kBands bands of image rows, one per
+// XCD (workgroups are dispatched to the 8 XCDs in turn). The queue buffer holds
+// kQueueWords words, zero when a frame starts: band counters of generate, of
+// shade / render_paths, and restir_shade's two order counts per band.
+constexpr int kBands = 8;
+constexpr int kQueueGenerate = 0, kQueuePath = kBands, kQueueOrder = 2 * kBands, kQueueWords = 4 * kBands;
+
 // render_paths_simd with 8 or 32 lanes per pixel.
 void launch_render_paths(int stack, int lanes, const DevScene& sc, const FrameParams& P, uint8_t* out, float4* hdr,
                          unsigned long long* cnt, hipStream_t st);
-// Persistent restir_generate; `queue` must hold 0 when the kernel starts.
+// Persistent restir_generate; `queue` (kBands words) must hold 0 when the kernel starts.
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                      unsigned int* queue, unsigned long long* cnt, hipStream_t st);
 void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
@@ -33,13 +40,13 @@ void launch_spatial(int stack, const DevScene& sc, const FrameParams& P, const G
                     hipStream_t st);
 // Persistent path engine: restir_shade (shade = true, writes hdr and, if not
 // null, the per-pixel sample variance of luminance) or render_paths with one
-// lane per pixel (writes rgb, and hdr if not null). queue[0] (and, for
-// restir_shade with `order`, queue[1] and queue[2]) must hold 0 when the pass
-// starts. With `order` (width * height words), restir_shade queues pixels by
-// expected chain length first.
+// lane per pixel (writes rgb, and hdr if not null). `queue` (kBands words) and,
+// for restir_shade with `order` (width * height words), `order_n` (2 * kBands
+// words) must hold 0 when the pass starts; restir_shade then queues each
+// band's pixels by expected chain length.
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
                  const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
-                 unsigned int* order, unsigned long long* cnt, hipStream_t st);
+                 unsigned int* order, unsigned int* order_n, unsigned long long* cnt, hipStream_t st);
 // Wavefront variant of launch_wave: the same per-pixel results from a lean
 // trace kernel and a state-machine kernel over path states kept in HBM.
 struct WaveBufs {

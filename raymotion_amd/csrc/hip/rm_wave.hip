@@ -708,25 +708,42 @@ struct GenEngine {
     __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
 };
 
+// Image bands of the work queues: band b holds rows [band_row(b), band_row(b + 1)).
+__device__ __forceinline__ int band_row(int b, int height) { return (b * height + kBands - 1) / kBands; }
+
+// Work of a persistent pass, split into kBands bands of image rows. A block
+// starts on band blockIdx.x % kBands: workgroups are dispatched to the XCDs in
+// turn, so each XCD's L2 serves the rays of one part of the image; a wave whose
+// band is empty moves on to the next one. Band b's positions are taken from
+// next[b]. Without `order`, position i of band b is pixel
+// band_row(b) * width + i; with it, i < order_n[2b] reads order[s + i] and the
+// others order[e - 1 - (i - order_n[2b])], for the band's segment [s, e) of
+// `order` (order_n[2b] + order_n[2b + 1] positions).
+struct BandQueue {
+    unsigned int* next;
+    const unsigned int* order;
+    const unsigned int* order_n;
+};
+
 // The persistent loop shared by the engines: every iteration traces one ray
 // per active lane, then advances each lane; idle lanes take new pixels from
-// `queue` (one atomic per wave) until the image is exhausted.
-// With `order`, queue position i holds pixel order[i] for i < order_n[0] and
-// order[npx - 1 - (i - order_n[0])] after that, order_n[0] + order_n[1]
-// pixels in all; without it, pixel i.
+// the queue (one atomic per wave) until every band is exhausted.
 template <int S, class E>
 __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene sc, FrameParams P, PassIO io,
-                                                        unsigned int* __restrict__ queue,
-                                                        const unsigned int* __restrict__ order,
-                                                        const unsigned int* __restrict__ order_n,
-                                                        unsigned long long* cnt) {
+                                                                      BandQueue q, unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     E eng(pt, P, io);
-    const int npx_all = P.width * P.height;
-    const unsigned front = order ? order_n[0] : 0u;
-    const int npx = order ? int(front + order_n[1]) : npx_all;
     const unsigned lane = __lane_id();
+    int band = int(blockIdx.x) % kBands, tried = 0;
+    unsigned seg0 = 0, seg1 = 0, front = 0, size = 0;
+    auto enter = [&](int b) {
+        seg0 = unsigned(band_row(b, P.height)) * unsigned(P.width);
+        seg1 = unsigned(band_row(b + 1, P.height)) * unsigned(P.width);
+        front = q.order ? q.order_n[2 * b] : 0u;
+        size = q.order ? front + q.order_n[2 * b + 1] : seg1 - seg0;
+    };
+    enter(band);
     typename E::Lane L;
     L.phase = kIdle;
     bool exhausted = false;
@@ -737,19 +754,27 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                 const int count = __popcll(need);
                 const int leader = __ffsll((long long)need) - 1;
                 unsigned base = 0;
-                if (int(lane) == leader) base = atomicAdd(queue, unsigned(count));
+                if (int(lane) == leader) base = atomicAdd(q.next + band, unsigned(count));
                 base = __shfl(base, leader);
                 if (L.phase == kIdle) {
                     const unsigned rank = __builtin_amdgcn_mbcnt_hi(unsigned(need >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo(unsigned(need), 0u));
                     const unsigned i = base + rank;
-                    if (i < unsigned(npx)) {
-                        const unsigned px = !order ? i : i < front ? order[i] : order[unsigned(npx_all) - 1u - (i - front)];
+                    if (i < size) {
+                        const unsigned px = !q.order ? seg0 + i : i < front ? q.order[seg0 + i]
+                                                                            : q.order[seg1 - 1u - (i - front)];
                         L.phase = kExt;  // start() sets the real phase
                         if (eng.start(L, int(px)) != kTrace) L.phase = kIdle;
                     }
                 }
-                if (base + unsigned(count) >= unsigned(npx)) exhausted = true;
+                if (base + unsigned(count) >= size) {  // this band is done: the next one
+                    if (++tried == kBands) {
+                        exhausted = true;
+                    } else {
+                        band = band + 1 == kBands ? 0 : band + 1;
+                        enter(band);
+                    }
+                }
             }
         }
         if (__ballot(L.phase != kIdle) == 0) {
@@ -778,43 +803,53 @@ int resident_blocks(const void* kernel) {
 
 // Launches k_persistent<S, E> with one resident wave set (cached per instance).
 template <int S, class E>
-void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& io, unsigned* queue,
-                       const unsigned* order, const unsigned* order_n, unsigned long long* cnt, hipStream_t st) {
+void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& io, const BandQueue& q,
+                       unsigned long long* cnt, hipStream_t st) {
     static int resident = 0;
     const void* kernel = reinterpret_cast<const void*>(&k_persistent<S, E>);
     if (resident == 0) resident = resident_blocks<S>(kernel);
     const int npx = P.width * P.height;
     const int need = (npx + kBlock - 1) / kBlock;
     const dim3 grid(unsigned(need < resident ? need : resident)), block(kBlock);
-    hipLaunchKernelGGL((k_persistent<S, E>), grid, block, 0, st, sc, P, io, queue, order, order_n, cnt);
+    hipLaunchKernelGGL((k_persistent<S, E>), grid, block, 0, st, sc, P, io, q, cnt);
 }
 
 // Work order of restir_shade. Per-pixel chains differ widely in length, and a
 // persistent lane that takes a long one late sets the end of the pass, so
-// pixels whose primary hit is a dielectric (camera paths, the longest chains)
-// are queued first (from the front of `order`), then the ReSTIR pixels (from
-// the back). Pixels without a primary hit get their result here (black, zero
-// variance, as start_pixel writes them) and are not queued. The order only
-// changes which lane runs a pixel, never its result: every pixel's random
-// sequence is seeded from its index.
+// within each band's segment of `order` (the band's rows) pixels whose primary
+// hit is a dielectric (camera paths, the longest chains) are queued first
+// (from the segment's start), then the ReSTIR pixels (from its end). Pixels
+// without a primary hit get their result here (black, zero variance, as
+// start_pixel writes them) and are not queued. The order only changes which
+// lane runs a pixel, never its result: every pixel's random sequence is
+// seeded from its index.
 __global__ __launch_bounds__(kBlock) void k_shade_order(FrameParams P, GBufferSoA gb, float4* __restrict__ hdr,
                                                         float* __restrict__ lum_var, unsigned* __restrict__ order,
                                                         unsigned* __restrict__ order_n) {
     const int npx = P.width * P.height;
     const int px = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    int kind = 0;
+    int kind = 0, band = 0;
     if (px < npx) {
         kind = ibits(gb.p0[px].w);
         if (kind == 0) {
             hdr[px] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (lum_var) lum_var[px] = 0.0f;
         }
+        band = (px / P.width) * kBands / P.height;
     }
     const bool first = kind != 0 && kind != 1, last = kind == 1;
-    const unsigned a = wave_append_pos(first, order_n);
-    const unsigned b = wave_append_pos(last, order_n + 1);
-    if (first) order[a] = unsigned(px);
-    if (last) order[unsigned(npx) - 1u - b] = unsigned(px);
+    bool pending = first || last;
+    for (;;) {  // one round per band present in the wave (one or two)
+        const unsigned long long m = __ballot(pending);
+        if (!m) break;
+        const int lb = __shfl(band, __ffsll((long long)m) - 1);
+        const bool mine = pending && band == lb;
+        const unsigned a = wave_append_pos(mine && first, order_n + 2 * lb);
+        const unsigned c = wave_append_pos(mine && last, order_n + 2 * lb + 1);
+        if (mine && first) order[unsigned(band_row(lb, P.height) * P.width) + a] = unsigned(px);
+        if (mine && last) order[unsigned(band_row(lb + 1, P.height) * P.width) - 1u - c] = unsigned(px);
+        if (mine) pending = false;
+    }
 }
 
 // ------------------------------------------------------------------ wavefront
@@ -1095,29 +1130,30 @@ int run_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, co
 
 void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb,
                  const ReservoirSoA& rm, float4* hdr, uint8_t* rgb, float* lum_var, unsigned int* queue,
-                 unsigned int* order, unsigned long long* cnt, hipStream_t st) {
+                 unsigned int* order, unsigned int* order_n, unsigned long long* cnt, hipStream_t st) {
     const PassIO io{gb, rm, hdr, rgb, lum_var};
-    unsigned* order_n = queue + 1;
+    BandQueue q{queue, nullptr, nullptr};
     if (shade && order) {
         const int npx = P.width * P.height;
         hipLaunchKernelGGL(k_shade_order, dim3((npx + kBlock - 1) / kBlock), dim3(kBlock), 0, st, P, gb, hdr, lum_var,
                            order, order_n);
-    } else {
-        order = nullptr;
+        q.order = order;
+        q.order_n = order_n;
     }
     with_stack(stack, [&](auto cap) {
         constexpr int S = decltype(cap)::value;
-        if (shade) launch_persistent<S, PathEngine<S, true>>(sc, P, io, queue, order, order_n, cnt, st);
-        else launch_persistent<S, PathEngine<S, false>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+        if (shade) launch_persistent<S, PathEngine<S, true>>(sc, P, io, q, cnt, st);
+        else launch_persistent<S, PathEngine<S, false>>(sc, P, io, q, cnt, st);
     });
 }
 
 void launch_generate(int stack, const DevScene& sc, const FrameParams& P, const GBufferSoA& gb, const ReservoirSoA& ri,
                      unsigned int* queue, unsigned long long* cnt, hipStream_t st) {
     const PassIO io{gb, ri, nullptr, nullptr, nullptr};
+    const BandQueue q{queue, nullptr, nullptr};
     with_stack(stack, [&](auto cap) {
         constexpr int S = decltype(cap)::value;
-        launch_persistent<S, GenEngine<S>>(sc, P, io, queue, nullptr, nullptr, cnt, st);
+        launch_persistent<S, GenEngine<S>>(sc, P, io, q, cnt, st);
     });
 }
 
