@@ -1,6 +1,8 @@
 // Scene -> reference-layout arrays handed to the C-ABI (internal header).
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -9,8 +11,30 @@
 
 namespace raymotion {
 
+// Allocator whose resize() leaves new elements uninitialised: the marshal
+// writes every record, and value-initialising hundreds of MB first (on one
+// thread) would cost more than the conversion.
+template <class T>
+struct NoInitAllocator : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAllocator<U>;
+    };
+    NoInitAllocator() = default;
+    template <class U>
+    NoInitAllocator(const NoInitAllocator<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+
 struct MarshalledScene {
-    std::vector<rm_triangle> tris;
+    std::vector<rm_triangle, NoInitAllocator<rm_triangle>> tris;
     std::vector<rm_bvh_node> nodes;
     std::vector<float> cdf;
     std::vector<float> texels;

@@ -9,6 +9,9 @@
 
 struct rm_scene {
     raymotion::Scene scene;
+    // Records of the last marshal, reused by the next one (an animated scene
+    // re-marshals into the same buffers). A handle is used by one thread at a time.
+    mutable raymotion::MarshalledScene marshalled;
 };
 
 using raymotion::Vec3;
@@ -138,7 +141,7 @@ int rm_scene_precompute(const rm_scene* s, double* n, double* nn, double* denom,
 int rm_scene_export(const rm_scene* s, rm_triangle* tris, rm_bvh_node* nodes, int32_t* bvh_tri, int32_t* light_tri,
                     float* light_cdf, float* light_total) {
     if (!s) return RM_ERR_INVALID;
-    raymotion::MarshalledScene m;
+    raymotion::MarshalledScene& m = s->marshalled;
     raymotion::marshal_scene(s->scene, m);
     if (tris && !m.tris.empty()) std::memcpy(tris, m.tris.data(), m.tris.size() * sizeof(rm_triangle));
     if (nodes && !m.nodes.empty()) std::memcpy(nodes, m.nodes.data(), m.nodes.size() * sizeof(rm_bvh_node));
@@ -153,7 +156,7 @@ int rm_scene_export(const rm_scene* s, rm_triangle* tris, rm_bvh_node* nodes, in
 
 int rm_scene_upload(rm_ctx* ctx, const rm_scene* s) {
     if (!ctx || !s) return RM_ERR_INVALID;
-    raymotion::MarshalledScene m;
+    raymotion::MarshalledScene& m = s->marshalled;
     raymotion::marshal_scene(s->scene, m);
     const rm_scene_desc d = m.desc(s->scene);
     return rm_set_scene(ctx, &d);
@@ -161,7 +164,7 @@ int rm_scene_upload(rm_ctx* ctx, const rm_scene* s) {
 
 int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild) {
     if (!ctx || !s) return RM_ERR_INVALID;
-    raymotion::MarshalledScene m;
+    raymotion::MarshalledScene& m = s->marshalled;
     raymotion::marshal_scene(s->scene, m);
     const rm_scene_desc d = m.desc(s->scene);
     return rm_set_scene_gpu_tree(ctx, &d, rebuild);

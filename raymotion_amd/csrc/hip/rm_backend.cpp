@@ -223,8 +223,9 @@ int megakernel_lanes(const FrameParams& P) {
     return lanes;
 }
 
-template <class T>
-bool upload(rm_ctx* c, DevBuf& b, const std::vector<T>& v) {
+template <class V>
+bool upload(rm_ctx* c, DevBuf& b, const V& v) {
+    using T = typename V::value_type;
     if (!b.ensure(v.size() * sizeof(T))) return false;
     if (v.empty()) return true;
     return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;

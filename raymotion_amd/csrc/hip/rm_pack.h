@@ -11,7 +11,11 @@
 // Each part has a content key computed over fixed-size chunks in parallel
 // (deterministic for any thread count).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
+#include <new>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -23,20 +27,50 @@ struct F4 {
     float x, y, z, w;
 };
 
+// Host buffers of the scene upload live in pinned memory (hipHostMalloc), so
+// the copies to the device are direct DMA, asynchronous to the host; resize()
+// leaves new elements uninitialised (every packer writes all of them). The
+// upload waits for its copies before the buffers are written again.
+template <class T>
+struct PinnedAllocator {
+    using value_type = T;
+    PinnedAllocator() = default;
+    template <class U>
+    PinnedAllocator(const PinnedAllocator<U>&) noexcept {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) noexcept { (void)hipHostFree(p); }
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+    friend bool operator==(const PinnedAllocator&, const PinnedAllocator&) { return true; }
+    friend bool operator!=(const PinnedAllocator&, const PinnedAllocator&) { return false; }
+};
+template <class T>
+using PinnedVector = std::vector<T, PinnedAllocator<T>>;
+
 struct PackedScene {
-    std::vector<F4> nodes;        // kNodeWords per 4-wide node, preorder (parents before children)
+    PinnedVector<F4> nodes;        // kNodeWords per 4-wide node, preorder (parents before children)
     std::vector<int32_t> depth;   // tree depth of each 4-wide node
     int32_t stack_need = 0;       // deepest possible traversal stack
-    std::vector<F4> geo;          // kTriGeoWords per triangle, leaf order
-    std::vector<F4> nrm;          // kTriNrmWords per triangle, scene order
-    std::vector<F4> nn;           // 1 per triangle
-    std::vector<F4> lights;       // kLightWords per light, CDF order
-    std::vector<float> cdf;
+    PinnedVector<F4> geo;          // kTriGeoWords per triangle, leaf order
+    PinnedVector<F4> nrm;          // kTriNrmWords per triangle, scene order
+    PinnedVector<F4> nn;           // 1 per triangle
+    PinnedVector<F4> lights;       // kLightWords per light, CDF order
+    PinnedVector<float> cdf;
     float light_total = 0.0f;
-    std::vector<int32_t> matid;   // 1 per triangle
-    std::vector<F4> mats;         // kMatWords per material
-    std::vector<F4> uv;           // 2 per triangle when any triangle is textured, else empty
-    std::vector<float> texels;
+    PinnedVector<int32_t> matid;   // 1 per triangle
+    PinnedVector<F4> mats;         // kMatWords per material
+    PinnedVector<F4> uv;           // 2 per triangle when any triangle is textured, else empty
+    PinnedVector<float> texels;
     int32_t opaque = 1;
 };
 
