@@ -1,8 +1,10 @@
 // C-ABI over the host scene engine (include/rm_scene.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "marshal.hpp"
+#include "rm_parallel.h"
 #include "raymotion/bvh.hpp"
 #include "raymotion/renderer.hpp"
 #include "rm_scene.h"
@@ -54,34 +56,42 @@ int rm_scene_add_triangles(rm_scene* s, size_t n, const double* pos, const doubl
 int rm_scene_set_positions(rm_scene* s, size_t n, const double* pos, const double* normals) {
     if (!s || n != s->scene.tris.size() || (n > 0 && !pos)) return RM_ERR_INVALID;
     raymotion::Scene& sc = s->scene;
+    constexpr size_t kChunk = size_t(1) << 14;
+    const size_t chunks = (n + kChunk - 1) / kChunk;
     // Reject the update before touching the scene if a triangle would become
-    // degenerate (the same test as Scene::add_triangle), so the count stays.
-    for (size_t i = 0; i < n; ++i) {
-        const double* p = pos + 9 * i;
-        const Vec3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
-        const Vec3 e1 = b - a, e2 = c - a;
-        const double g00 = dot(e1, e1), g01 = dot(e1, e2), g11 = dot(e2, e2);
-        if (std::abs(g00 * g11 - g01 * g01) <= 1e-12) return RM_ERR_INVALID;
-    }
-    std::vector<raymotion::Triangle> old;
-    old.swap(sc.tris);
-    sc.clear_triangles();
-    sc.tris.reserve(n);
-    for (size_t i = 0; i < n; ++i) {
-        const raymotion::Triangle& t = old[i];
-        const double* p = pos + 9 * i;
-        const Vec3 a{p[0], p[1], p[2]}, b{p[3], p[4], p[5]}, c{p[6], p[7], p[8]};
-        Vec3 n0 = t.n0, n1 = t.n1, n2 = t.n2;
-        if (normals) {
-            const double* q = normals + 9 * i;
-            n0 = {q[0], q[1], q[2]};
-            n1 = {q[3], q[4], q[5]};
-            n2 = {q[6], q[7], q[8]};
+    // degenerate (the test of Scene::add_triangle), so the count stays.
+    std::vector<char> bad(chunks, 0);
+    rmpar::parallel_chunks(chunks, [&](size_t ch) {
+        const size_t a0 = ch * kChunk, a1 = std::min(n, a0 + kChunk);
+        for (size_t i = a0; i < a1 && !bad[ch]; ++i) {
+            const double* p = pos + 9 * i;
+            const Vec3 e1 = Vec3{p[3], p[4], p[5]} - Vec3{p[0], p[1], p[2]};
+            const Vec3 e2 = Vec3{p[6], p[7], p[8]} - Vec3{p[0], p[1], p[2]};
+            const double g00 = dot(e1, e1), g01 = dot(e1, e2), g11 = dot(e2, e2);
+            if (std::abs(g00 * g11 - g01 * g01) <= 1e-12) bad[ch] = 1;
         }
-        const int idx = sc.add_triangle(a, b, c, t.tu0, t.tv0, t.tu1, t.tv1, t.tu2, t.tv2, n0, n1, n2, t.ar, t.ag,
-                                        t.ab, t.er, t.eg, t.eb, t.metallic, t.ior, t.rough, t.shader, t.alpha);
-        sc.tris[size_t(idx)].texture = t.texture;  // idx >= 0: checked above
-    }
+    });
+    for (char b : bad)
+        if (b) return RM_ERR_INVALID;
+    // New precompute in place (add_triangle's arithmetic), then the lights in
+    // triangle order.
+    rmpar::parallel_chunks(chunks, [&](size_t ch) {
+        const size_t a0 = ch * kChunk, a1 = std::min(n, a0 + kChunk);
+        for (size_t i = a0; i < a1; ++i) {
+            raymotion::Triangle& t = sc.tris[i];
+            const double* p = pos + 9 * i;
+            double gram = 0;
+            (void)raymotion::Scene::precompute_triangle({p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, t,
+                                                        gram);
+            if (normals) {
+                const double* q = normals + 9 * i;
+                t.n0 = {q[0], q[1], q[2]};
+                t.n1 = {q[3], q[4], q[5]};
+                t.n2 = {q[6], q[7], q[8]};
+            }
+        }
+    });
+    sc.rebuild_lights();
     return RM_OK;
 }
 

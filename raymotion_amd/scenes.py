@@ -264,18 +264,24 @@ def heightfield(n: int = 1024, seed: int = 0, t: float = 0.0, scene: Scene | Non
 
 class HeightFieldAnimation:
     """Triangle positions of heightfield(n, seed) at time t (grid, then the
-    static lights), with the noise evaluated once."""
+    static lights), with the noise evaluated once. The grid has n distinct x
+    coordinates, so the animated term is evaluated per column and gathered."""
 
     def __init__(self, n: int = 1024, seed: int = 0):
         self.base = heightfield_positions(n, seed, 0.0)
-        self.x = self.base[:, :, 0].copy()
-        self.h0 = self.base[:, :, 1] - 0.05 * np.sin(self.x)
+        x = self.base[:, :, 0]
+        self.h0 = self.base[:, :, 1] - 0.05 * np.sin(x)
+        self.ux, self.xi = np.unique(x, return_inverse=True)
+        self.xi = self.xi.reshape(x.shape)
         self.lights = _heightfield_extras(n, seed)[1]
+        self.out = np.concatenate([self.base, self.lights])
 
     def positions(self, t: float) -> np.ndarray:
-        p = self.base.copy()
-        p[:, :, 1] = self.h0 + 0.05 * np.sin(2.0 * np.pi * t + self.x)
-        return np.concatenate([p, self.lights])
+        """Positions at time t, in a buffer that the next call overwrites."""
+        y = self.out[:len(self.base), :, 1]
+        np.multiply(np.sin(2.0 * np.pi * t + self.ux)[self.xi], 0.05, out=y)
+        y += self.h0
+        return self.out
 
 
 def heightfield_camera(frame: int = 0) -> Camera:

@@ -133,3 +133,35 @@ int main(int, char** argv) {
     r = subprocess.run([str(exe), str(tmp_path / "model.obj"), str(tmp_path / "m" / "model.mtl")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_set_positions_matches_fresh_scene():
+    """rm_scene_set_positions (parallel precompute, lights rebuilt in order)
+    leaves the same records, light list and CDF as adding the moved triangles
+    to an empty scene."""
+    import numpy as np
+    from raymotion_amd import scenes
+    from raymotion_amd.scene import material_rows
+    n, seed = 48, 3
+    a = scenes.heightfield(n, seed)
+    pos = scenes.HeightFieldAnimation(n, seed).positions(0.37)
+    rng = np.random.default_rng(1)
+    nrm = rng.normal(size=pos.shape)
+    nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+    a.set_positions(pos, nrm)
+    rough, light_tri, light_mat = scenes._heightfield_extras(n, seed)
+    g = len(pos) - len(light_tri)
+    mat = material_rows(g, albedo=(0.7, 0.7, 0.7), metallic=1.0)
+    mat[:, 8] = rough
+    b = rm.Scene()
+    b.add_triangles(pos[:g], mat, normals=nrm[:g])
+    b.add_triangles(pos[g:], light_mat, normals=nrm[g:])
+    ea, eb = a.export(), b.export()
+    assert ea.tris.tobytes() == eb.tris.tobytes()
+    assert np.array_equal(ea.light_tri, eb.light_tri)
+    assert ea.light_cdf.tobytes() == eb.light_cdf.tobytes() and ea.light_total == eb.light_total
+    with pytest.raises(rm.BackendError):  # a degenerate triangle rejects the whole update
+        bad = pos.copy()
+        bad[5, 1] = bad[5, 0]
+        a.set_positions(bad)
+    assert a.export().tris.tobytes() == ea.tris.tobytes()
