@@ -1,4 +1,4 @@
-// GPU build of the 4-wide traversal tree from triangles. No counterpart in the
+// GPU build of the wide traversal tree from triangles. No counterpart in the
 // reference, which builds a binary SAH tree on the host whenever the geometry
 // topology changes and at every 32nd geometry update (its
 // include/raymotion/runtime.hpp:119-130).
@@ -11,9 +11,9 @@
 // finds, within kPlocRadius neighbours on either side in that order, the one
 // whose merged box has the least surface area; mutual nearest neighbours merge
 // into a new node, and the surviving clusters are compacted in order, until
-// one is left. A top-down collapse then writes 4-wide nodes, one launch per
+// one is left. A top-down collapse then writes kWide-wide nodes, one launch per
 // output level: each wide node opens the binary children of largest surface
-// area until it has four; single triangles become leaf slots, the others wide
+// area until all kWide slots are used; single triangles become leaf slots, the others
 // nodes of the next level. The nodes are then renumbered depth-first (a node
 // next to its first child, as in the host tree's preorder) and listed level by
 // level for the refit launches. Traversal returns the closest hit whatever the
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
     const int i = int(blockIdx.x) * kB + int(threadIdx.x);
     if (i >= count) return;
     const int b = in[i].x, stack_here = in[i].y;
-    int slot[4];
+    int slot[kWide];
     int ns = 0;
     if (b >= T.n - 1) {
         slot[ns++] = b;  // a one-triangle tree
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
         slot[ns++] = ch.x;
         slot[ns++] = ch.y;
     }
-    while (ns < 4) {
+    while (ns < kWide) {
         int best = -1;
         float best_area = -1.0f;
         for (int s = 0; s < ns; ++s) {
@@ -312,10 +312,10 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
         slot[best] = ch.x;
         slot[ns++] = ch.y;
     }
-    float lo[3][4], hi[3][4];
-    int ref[4], kind[4];
+    float lo[3][kWide], hi[3][kWide];
+    int ref[kWide], kind[kWide];
     int inner = 0;
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < kWide; ++s) {
         if (s >= ns) {
             for (int a = 0; a < 3; ++a) {
                 lo[a][s] = __builtin_inff();
@@ -352,36 +352,24 @@ __global__ __launch_bounds__(kB) void k_collapse(BuildTree T, const int2* __rest
                 ++k;
             }
     }
-    float4* w = nodes + size_t(base + i) * kNodeWords;
-    for (int a = 0; a < 3; ++a) {
-        w[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-        w[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-    }
-    w[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
-    w[7] = make_float4(__int_as_float(kind[0]), __int_as_float(kind[1]), __int_as_float(kind[2]),
-                       __int_as_float(kind[3]));
+    float4 w[kNodeWords];
+    write_node(w, lo, hi, ref, kind);
+    float4* dst = nodes + size_t(base + i) * kNodeWords;
+    for (int k = 0; k < kNodeWords; ++k) dst[k] = w[k];
 }
 
 // Depth-first renumbering of the collapsed tree (which comes out level by
 // level): subtree sizes from the deepest level up, then indices from the root
 // down, each inner child's subtree following its parent's earlier children, so
 // a node sits next to its first child as in the host tree's preorder.
-__device__ __forceinline__ void node_links(const float4* __restrict__ node, int ref[4], int kind[4]) {
-    const float4 rf = node[6], kd = node[7];
-    ref[0] = __float_as_int(rf.x); ref[1] = __float_as_int(rf.y); ref[2] = __float_as_int(rf.z);
-    ref[3] = __float_as_int(rf.w);
-    kind[0] = __float_as_int(kd.x); kind[1] = __float_as_int(kd.y); kind[2] = __float_as_int(kd.z);
-    kind[3] = __float_as_int(kd.w);
-}
-
 __global__ __launch_bounds__(kB) void k_subtree_size(const float4* __restrict__ nodes, int first, int count,
                                                      int* __restrict__ size) {
     const int i = int(blockIdx.x) * kB + int(threadIdx.x);
     if (i >= count) return;
-    int ref[4], kind[4];
-    node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
+    int ref[kWide], kind[kWide];
+    read_node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
     int s = 1;
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kWide; ++c)
         if (kind[c] == 0) s += size[ref[c]];
     size[first + i] = s;
 }
@@ -390,10 +378,10 @@ __global__ __launch_bounds__(kB) void k_dfs_index(const float4* __restrict__ nod
                                                   const int* __restrict__ size, int* __restrict__ index) {
     const int i = int(blockIdx.x) * kB + int(threadIdx.x);
     if (i >= count) return;
-    int ref[4], kind[4];
-    node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
+    int ref[kWide], kind[kWide];
+    read_node_links(nodes + size_t(first + i) * kNodeWords, ref, kind);
     int next = index[first + i] + 1;
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kWide; ++c)
         if (kind[c] == 0) {
             index[ref[c]] = next;
             next += size[ref[c]];
@@ -405,15 +393,16 @@ __global__ __launch_bounds__(kB) void k_renumber(const float4* __restrict__ src,
                                                  const int* __restrict__ index, float4* __restrict__ dst) {
     const int i = int(blockIdx.x) * kB + int(threadIdx.x);
     if (i >= total) return;
+    float4 w[kNodeWords];
     const float4* a = src + size_t(i) * kNodeWords;
-    float4* b = dst + size_t(index[i]) * kNodeWords;
-    int ref[4], kind[4];
-    node_links(a, ref, kind);
-    for (int c = 0; c < 4; ++c)
+    for (int k = 0; k < kNodeWords; ++k) w[k] = a[k];
+    int ref[kWide], kind[kWide];
+    read_node_links(w, ref, kind);
+    for (int c = 0; c < kWide; ++c)
         if (kind[c] == 0) ref[c] = index[ref[c]];
-    for (int w = 0; w < 6; ++w) b[w] = a[w];
-    b[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
-    b[7] = a[7];
+    set_node_refs(w, ref);
+    float4* b = dst + size_t(index[i]) * kNodeWords;
+    for (int k = 0; k < kNodeWords; ++k) b[k] = w[k];
 }
 
 // Refit order: the new indices of one level's nodes at out[0 .. count).
@@ -443,7 +432,7 @@ struct Scratch {
 };
 
 Scratch carve(void* base, int n) {
-    const size_t N = size_t(n > 1 ? n : 2);  // also bounds the 4-wide node count (n - 1, or 1)
+    const size_t N = size_t(n > 1 ? n : 2);  // also bounds the wide node count (n - 1, or 1)
     Scratch s{};
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, s.sort_bytes, static_cast<const uint64_t*>(nullptr),
                                              static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),

@@ -138,43 +138,57 @@ struct Tracer {
         int node = 0;
         for (;;) {
             const float4* n = sc.nodes + size_t(node) * kNodeWords;
-            const float4 lox = n[0], hix = n[1], loy = n[2], hiy = n[3], loz = n[4], hiz = n[5];
-            const float4 rf = n[6], kd = n[7];
+            float4 w[kNodeWords];
+#pragma unroll
+            for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
             ++tl.nodes;
             const float tc = h.t + h.t * (1.0f / 4096.0f);
-            float tn[4];
-            int ref[4], kind[4];
-            {
+            float tn[kWide];
+            int ref[kWide], kind[kWide];
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                const float4 lox = w[node_lo_word(0, g)], hix = w[node_hi_word(0, g)];
+                const float4 loy = w[node_lo_word(1, g)], hiy = w[node_hi_word(1, g)];
+                const float4 loz = w[node_lo_word(2, g)], hiz = w[node_hi_word(2, g)];
+                const float4 rf = w[kNodeRefWord + g], kd = w[kNodeKindWord + g];
                 const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
                 const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
                 const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
                 const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
                 const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const float ax = __builtin_fmaf(lx[c], ix, ox), bx = __builtin_fmaf(hx[c], ix, ox);
-                    const float ay = __builtin_fmaf(ly[c], iy, oy), by = __builtin_fmaf(hy[c], iy, oy);
-                    const float az = __builtin_fmaf(lz[c], iz, oz), bz = __builtin_fmaf(hz[c], iz, oz);
+                for (int j = 0; j < 4; ++j) {
+                    const int c = 4 * g + j;
+                    const float ax = __builtin_fmaf(lx[j], ix, ox), bx = __builtin_fmaf(hx[j], ix, ox);
+                    const float ay = __builtin_fmaf(ly[j], iy, oy), by = __builtin_fmaf(hy[j], iy, oy);
+                    const float az = __builtin_fmaf(lz[j], iz, oz), bz = __builtin_fmaf(hz[j], iz, oz);
                     const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
                     const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
-                    const bool ok = k4[c] >= 0 && nr <= fr;
+                    const bool ok = k4[j] >= 0 && nr <= fr;
                     tn[c] = ok ? nr : __builtin_inff();
-                    ref[c] = r4[c];
-                    kind[c] = ok ? k4[c] : -1;
+                    ref[c] = r4[j];
+                    kind[c] = ok ? k4[j] : -1;
                 }
             }
-            // Sort the four slots by entry distance (misses last).
+            // Sort the slots by entry distance (misses last): the optimal
+            // 5- and 19-comparator networks for four and eight inputs.
 #define RM_CX(a, b)                                                   \
     if (tn[b] < tn[a]) {                                              \
         const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
         const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
         const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
     }
-            RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
+            if constexpr (kWide == 4) {
+                RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
+            } else {
+                RM_CX(0, 2) RM_CX(1, 3) RM_CX(4, 6) RM_CX(5, 7) RM_CX(0, 4) RM_CX(1, 5) RM_CX(2, 6) RM_CX(3, 7)
+                RM_CX(0, 1) RM_CX(2, 3) RM_CX(4, 5) RM_CX(6, 7) RM_CX(2, 4) RM_CX(3, 5) RM_CX(1, 4) RM_CX(3, 6)
+                RM_CX(1, 2) RM_CX(3, 4) RM_CX(5, 6)
+            }
 #undef RM_CX
             // Leaves first, nearest first.
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < kWide; ++c) {
                 if (kind[c] > 0 && tn[c] <= h.t + h.t * (1.0f / 4096.0f)) {
                     const int first = ref[c], count = kind[c];
                     for (int k = 0; k < count; ++k) {
@@ -207,7 +221,7 @@ struct Tracer {
             // Inner children: continue with the nearest, stack the rest.
             int next = -1;
 #pragma unroll
-            for (int c = 3; c >= 0; --c) {
+            for (int c = kWide - 1; c >= 0; --c) {
                 if (kind[c] == 0) {
                     if (next >= 0) {
                         if (sp < S) {

@@ -32,18 +32,18 @@ struct BinTree {  // binary tree recovered from the preorder array
     }
 };
 
-// Collapses the binary subtree under `b` into 4-wide node `dst` (already
+// Collapses the binary subtree under `b` into wide node `dst` (already
 // allocated). Children are gathered by repeatedly opening the inner child with
-// the largest surface area until four slots are used.
+// the largest surface area until all kWide slots are used.
 void collapse(const BinTree& t, int b, int dst, int depth, int stack_here, PackedScene& out) {
-    int kids[4];
+    int kids[kWide];
     int nk = 0;
     if (t.leaf(b)) {
         kids[nk++] = b;  // a leaf root becomes the single child of the root node
     } else {
         kids[nk++] = t.left(b);
         kids[nk++] = t.right(b);
-        while (nk < 4) {
+        while (nk < kWide) {
             int best = -1;
             float best_area = -1.0f;
             for (int k = 0; k < nk; ++k)
@@ -57,12 +57,11 @@ void collapse(const BinTree& t, int b, int dst, int depth, int stack_here, Packe
             kids[nk++] = t.right(open);
         }
     }
-    F4* w = &out.nodes[size_t(dst) * kNodeWords];
     const float inf = std::numeric_limits<float>::infinity();
-    float lo[3][4], hi[3][4];
-    int32_t ref[4], kind[4];
+    float lo[3][kWide], hi[3][kWide];
+    int ref[kWide], kind[kWide];
     int inner = 0;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < kWide; ++c) {
         if (c >= nk) {
             for (int a = 0; a < 3; ++a) { lo[a][c] = inf; hi[a][c] = -inf; }
             ref[c] = 0;
@@ -91,14 +90,8 @@ void collapse(const BinTree& t, int b, int dst, int depth, int stack_here, Packe
         out.depth.push_back(depth + 1);
         ref[c] = child;
         collapse(t, kids[c], child, depth + 1, stack_below, out);
-        w = &out.nodes[size_t(dst) * kNodeWords];  // storage may have moved
     }
-    for (int a = 0; a < 3; ++a) {
-        w[2 * a] = {lo[a][0], lo[a][1], lo[a][2], lo[a][3]};
-        w[2 * a + 1] = {hi[a][0], hi[a][1], hi[a][2], hi[a][3]};
-    }
-    w[6] = {ibf(ref[0]), ibf(ref[1]), ibf(ref[2]), ibf(ref[3])};
-    w[7] = {ibf(kind[0]), ibf(kind[1]), ibf(kind[2]), ibf(kind[3])};
+    write_node(&out.nodes[size_t(dst) * kNodeWords], lo, hi, ref, kind);
 }
 
 using rmpar::parallel_chunks;

@@ -12,35 +12,27 @@ __global__ __launch_bounds__(256) void k_refit(float4* __restrict__ nodes, const
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     float4* n = nodes + size_t(level_nodes[i]) * kNodeWords;
-    const float4 rf = n[6], kd = n[7];
-    const int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
-    const int kind[4] = {__float_as_int(kd.x), __float_as_int(kd.y), __float_as_int(kd.z), __float_as_int(kd.w)};
-    float lo[3][4], hi[3][4];
-    {
-        const float4 w0 = n[0], w1 = n[1], w2 = n[2], w3 = n[3], w4 = n[4], w5 = n[5];
-        const float4 w[6] = {w0, w1, w2, w3, w4, w5};
-        for (int a = 0; a < 3; ++a) {
-            lo[a][0] = w[2 * a].x; lo[a][1] = w[2 * a].y; lo[a][2] = w[2 * a].z; lo[a][3] = w[2 * a].w;
-            hi[a][0] = w[2 * a + 1].x; hi[a][1] = w[2 * a + 1].y; hi[a][2] = w[2 * a + 1].z; hi[a][3] = w[2 * a + 1].w;
-        }
-    }
-    for (int c = 0; c < 4; ++c) {
+    int ref[kWide], kind[kWide];
+    read_node_links(n, ref, kind);
+    float lo[3][kWide], hi[3][kWide];
+    read_node_boxes(n, lo, hi);
+    for (int c = 0; c < kWide; ++c) {
         if (kind[c] < 0) continue;
         float bl[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
         float bh[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
         if (kind[c] == 0) {
+            // An inner child: the union of its (already refit) child boxes.
             const float4* m = nodes + size_t(ref[c]) * kNodeWords;
-            const float4 mk = m[7];
-            const int mkind[4] = {__float_as_int(mk.x), __float_as_int(mk.y), __float_as_int(mk.z), __float_as_int(mk.w)};
-            for (int a = 0; a < 3; ++a) {
-                const float4 l = m[2 * a], h = m[2 * a + 1];
-                const float lv[4] = {l.x, l.y, l.z, l.w}, hv[4] = {h.x, h.y, h.z, h.w};
-                for (int k = 0; k < 4; ++k)
-                    if (mkind[k] >= 0) {
-                        bl[a] = fminf(bl[a], lv[k]);
-                        bh[a] = fmaxf(bh[a], hv[k]);
+            int mref[kWide], mkind[kWide];
+            read_node_links(m, mref, mkind);
+            float ml[3][kWide], mh[3][kWide];
+            read_node_boxes(m, ml, mh);
+            for (int k = 0; k < kWide; ++k)
+                if (mkind[k] >= 0)
+                    for (int a = 0; a < 3; ++a) {
+                        bl[a] = fminf(bl[a], ml[a][k]);
+                        bh[a] = fmaxf(bh[a], mh[a][k]);
                     }
-            }
         } else {
             for (int t = 0; t < kind[c]; ++t) {
                 const float4* g = geo + size_t(ref[c] + t) * kTriGeoWords;
@@ -58,10 +50,10 @@ __global__ __launch_bounds__(256) void k_refit(float4* __restrict__ nodes, const
             hi[a][c] = bh[a];
         }
     }
-    for (int a = 0; a < 3; ++a) {
-        n[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-        n[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
-    }
+    float4 w[kNodeWords];
+    for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
+    write_node(w, lo, hi, ref, kind);
+    for (int k = 0; k < kNodeWords; ++k) n[k] = w[k];
 }
 
 }  // namespace
