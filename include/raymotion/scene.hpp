@@ -77,6 +77,12 @@ struct Scene {
                      const Vec3& n1, const Vec3& n2, double ar, double ag, double ab,
                      double er, double eg, double eb, double metallic, double ior,
                      double rough, int shader, double alpha = 1);
+    // add_triangle's per-triangle precompute into t (no light bookkeeping);
+    // false when the triangle is degenerate. `gram` = |e1 x e2|^2.
+    static bool precompute_triangle(const Vec3& p0, const Vec3& p1, const Vec3& p2, Triangle& t, double& gram);
+    // Recomputes light_tri, light_cdf and light_total from the triangles in
+    // order, with add_triangle's arithmetic.
+    void rebuild_lights();
 };
 
 }  // namespace raymotion
