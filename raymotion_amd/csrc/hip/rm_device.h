@@ -98,6 +98,12 @@ struct NoAdjoint {
     __device__ __forceinline__ void dielectric(const T&...) {}
 };
 
+// LDS copy of the light table (kLightWords float4 per light) and its CDF, for
+// the kernels that stage it (Tracer::stage_lights). Accessed by name so the
+// reads stay LDS reads.
+__shared__ float4 lds_light_rec[kLdsLights * kLightWords];
+__shared__ float lds_light_cdf[kLdsLights];
+
 // Per-lane tracer state. S is the capacity of the lane's LDS traversal stack;
 // stk points at this lane's column of a [S][kBlock] shared array.
 template <int S>
@@ -108,11 +114,45 @@ struct Tracer {
     Rng rng;
     Tally tl;
 
+    // The light table and its CDF, copied into LDS by stage_lights when the
+    // scene has at most kLdsLights lights (lds_on), else read from the scene.
+    // The light sampling loops read them up to 32 times per shading point,
+    // each read a dependent chain (CDF search, then the light's record).
+    bool lds_on = false;
+
     __device__ Tracer(const DevScene& s, const FrameParams& p, uint32_t* stack) : sc(s), P(p), stk(stack) {
         rng.state = 0;
         tl = {0, 0, 0, 0};
     }
     __device__ __forceinline__ float urand() { return rng.next(); }
+
+    // Block-wide: every thread of the block calls it once, before any other
+    // use of the tracer.
+    __device__ void stage_lights() {
+        lds_on = sc.nlights > 0 && sc.nlights <= kLdsLights;
+        if (lds_on) {
+            for (int i = int(threadIdx.x); i < sc.nlights * kLightWords; i += int(blockDim.x))
+                lds_light_rec[i] = sc.lights[i];
+            for (int i = int(threadIdx.x); i < sc.nlights; i += int(blockDim.x)) lds_light_cdf[i] = sc.cdf[i];
+        }
+        __syncthreads();
+    }
+
+    struct LightRec {
+        float4 a, b, c, d;  // the kLightWords words of the light's record
+    };
+    __device__ __forceinline__ LightRec light(int li) const {
+        if (lds_on) {
+            const int w = li * kLightWords;
+            const LightRec r{lds_light_rec[w], lds_light_rec[w + 1], lds_light_rec[w + 2], lds_light_rec[w + 3]};
+            // Keeps the optimizer from merging these reads with the global
+            // branch's into one flat load through a selected pointer.
+            __builtin_amdgcn_sched_barrier(0);
+            return r;
+        }
+        const float4* L = sc.lights + size_t(li) * kLightWords;
+        return LightRec{L[0], L[1], L[2], L[3]};
+    }
 
     // ------------------------------------------------------------ traversal
     // Closest hit (or, with any, the first hit) along o + t d for t in
@@ -478,12 +518,21 @@ struct Tracer {
         return out;
     }
 
-    __device__ int pick_light(float pick) {  // CDF binary search (pathtrace.metal:323-330)
+    // CDF binary search (pathtrace.metal:323-330), over the LDS copy when staged.
+    __device__ int pick_light(float pick) const {
         int lo = 0, hi = sc.nlights - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) / 2;
-            if (sc.cdf[mid] >= pick) hi = mid;
-            else lo = mid + 1;
+        if (lds_on) {
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (lds_light_cdf[mid] >= pick) hi = mid;
+                else lo = mid + 1;
+            }
+        } else {
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (sc.cdf[mid] >= pick) hi = mid;
+                else lo = mid + 1;
+            }
         }
         return lo;
     }
@@ -530,8 +579,8 @@ struct Tracer {
         if (Mc <= 1) {
             const float pick = urand() * total;
             const int li = pick_light(pick);
-            const float4* L = sc.lights + size_t(li) * kLightWords;
-            const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
+            const LightRec lr = light(li);
+            const float4 l0 = lr.a, l1 = lr.b, l2 = lr.c, l3 = lr.d;
             float lu = urand(), lv = urand();
             if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
             const float lw = 1.0f - lu - lv;
@@ -577,19 +626,21 @@ struct Tracer {
         float sel_lx = 0.0f, sel_ly = 0.0f, sel_lz = 0.0f, sel_Fr = 0.0f, sel_Fg = 0.0f, sel_Fb = 0.0f;
         float sel_phat = 0.0f, sel_epdf = 0.0f;
         bool have = false;
-        for (int j = 0; j < Mc; ++j) {
-            const float pick = urand() * total;
-            const float4* L = sc.lights + size_t(pick_light(pick)) * kLightWords;
-            const float4 l0 = L[0], l1 = L[1], l2 = L[2], l3 = L[3];
-            float lu = urand(), lv = urand();
+        // One candidate's point, RIS weight and contribution.
+        struct Cand {
+            float qx, qy, qz, w, ph, epdf, Fr, Fg, Fb;
+        };
+        auto eval_cand = [&](const LightRec& lr, float lu, float lv) {
+            const float4 l0 = lr.a, l1 = lr.b, l2 = lr.c, l3 = lr.d;
             if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
             const float lw = 1.0f - lu - lv;
-            const float qx = lw * l0.x + (lu * l0.w + lv * l1.z);
-            const float qy = lw * l0.y + (lu * l1.x + lv * l1.w);
-            const float qz = lw * l0.z + (lu * l1.y + lv * l2.x);
-            const float ldx = qx - p_x, ldy = qy - p_y, ldz = qz - p_z;
+            Cand c;
+            c.qx = lw * l0.x + (lu * l0.w + lv * l1.z);
+            c.qy = lw * l0.y + (lu * l1.x + lv * l1.w);
+            c.qz = lw * l0.z + (lu * l1.y + lv * l2.x);
+            c.w = c.ph = c.epdf = c.Fr = c.Fg = c.Fb = 0.0f;
+            const float ldx = c.qx - p_x, ldy = c.qy - p_y, ldz = c.qz - p_z;
             const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
-            float w = 0.0f, ph = 0.0f, epdf = 0.0f, Fr = 0.0f, Fg = 0.0f, Fb = 0.0f;
             if (dist > 0.003f) {
                 const float inv = 1.0f / dist;
                 const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
@@ -602,23 +653,46 @@ struct Tracer {
                     if (q > 1e-12f) {
                         const Bsdf e = eval_bsdf(dxn, dyn, dzn, wo_x, wo_y, wo_z, n_x, n_y, n_z, alpha2, metallic,
                                                  spec_prob, ar, ag, ab);
-                        epdf = e.pdf;
-                        Fr = l3.x * e.r * scos * l3.w;
-                        Fg = l3.y * e.g * scos * l3.w;
-                        Fb = l3.z * e.b * scos * l3.w;
-                        ph = lum3(Fr, Fg, Fb);
-                        if (ph > 0.0f) w = ph / q;
+                        c.epdf = e.pdf;
+                        c.Fr = l3.x * e.r * scos * l3.w;
+                        c.Fg = l3.y * e.g * scos * l3.w;
+                        c.Fb = l3.z * e.b * scos * l3.w;
+                        c.ph = lum3(c.Fr, c.Fg, c.Fb);
+                        if (c.ph > 0.0f) c.w = c.ph / q;
                     }
                 }
             }
-            sumW += w;
-            const float r = urand();
-            if (w > 0.0f && r * sumW < w) {
+            return c;
+        };
+        auto take = [&](const Cand& c, float r) {
+            sumW += c.w;
+            if (c.w > 0.0f && r * sumW < c.w) {
                 have = true;
-                sel_lx = qx; sel_ly = qy; sel_lz = qz;
-                sel_Fr = Fr; sel_Fg = Fg; sel_Fb = Fb;
-                sel_phat = ph; sel_epdf = epdf;
+                sel_lx = c.qx; sel_ly = c.qy; sel_lz = c.qz;
+                sel_Fr = c.Fr; sel_Fg = c.Fg; sel_Fb = c.Fb;
+                sel_phat = c.ph; sel_epdf = c.epdf;
             }
+        };
+        // Candidates in pairs, so the two light searches and record reads
+        // overlap; each candidate's draws (pick, lu, lv, then the reservoir's
+        // r) and the reservoir updates keep their order.
+        int j = 0;
+        for (; j + 1 < Mc; j += 2) {
+            const float pa = urand() * total;
+            const float ua = urand(), va = urand(), ra = urand();
+            const float pb = urand() * total;
+            const float ub = urand(), vb = urand(), rb = urand();
+            const LightRec A = light(pick_light(pa));
+            const LightRec B = light(pick_light(pb));
+            const Cand ca = eval_cand(A, ua, va);
+            const Cand cb = eval_cand(B, ub, vb);
+            take(ca, ra);
+            take(cb, rb);
+        }
+        if (j < Mc) {
+            const float pa = urand() * total;
+            const float ua = urand(), va = urand(), ra = urand();
+            take(eval_cand(light(pick_light(pa)), ua, va), ra);
         }
         ns.sumW = sumW;
         ns.M = uint32_t(Mc);

@@ -94,6 +94,7 @@ constexpr int kTriGeoWords = 3;   // [v0.xyz v1.x] [v1.yz v2.xy] [v2.z id - -]
 constexpr int kTriNrmWords = 3;   // [n0.xyz n.x] [n1.xyz n.y] [n2.xyz n.z]
 constexpr int kMatWords = 4;      // [ar ag ab alpha] [er eg eb ior] [rough metallic shader texoff] [texw texh - -]
 constexpr int kLightWords = 4;    // [v0.xyz v1.x] [v1.yz v2.xy] [v2.z nn.xyz] [er eg eb alpha]
+constexpr int kLdsLights = 64;    // scenes with at most this many lights sample them from LDS
 constexpr int kBlock = 256;       // threads per block of every pixel pass
 
 // Per-pass counters: counters[3 * pass + {0 rays, 1 node visits, 2 triangle

@@ -571,8 +571,8 @@ struct GenEngine {
     }
 
     // Uniform point of light `li` at folded barycentrics (lu, lv).
-    __device__ V3 light_point(const float4* Lt, float lu, float lv) const {
-        const float4 l0 = Lt[0], l1 = Lt[1], l2 = Lt[2];
+    __device__ V3 light_point(const typename Tracer<S>::LightRec& Lt, float lu, float lv) const {
+        const float4 l0 = Lt.a, l1 = Lt.b, l2 = Lt.c;
         const float lw = 1.0f - lu - lv;
         return V3{lw * l0.x + (lu * l0.w + lv * l1.z), lw * l0.y + (lu * l1.x + lv * l1.w),
                   lw * l0.z + (lu * l1.y + lv * l2.x)};
@@ -584,14 +584,14 @@ struct GenEngine {
         while (L.j < L.Mc) {
             const float pick = pt.urand() * sc.light_total;
             const int li = pt.pick_light(pick);
-            const float4* Lt = sc.lights + size_t(li) * kLightWords;
+            const typename Tracer<S>::LightRec Lt = pt.light(li);
             float lu = pt.urand(), lv = pt.urand();
             if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
             const V3 q = light_point(Lt, lu, lv);
             const float ldx = q.x - L.p.x, ldy = q.y - L.p.y, ldz = q.z - L.p.z;
             const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
             if (dist > 0.003f) {
-                const float4 l2 = Lt[2], l3 = Lt[3];
+                const float4 l2 = Lt.c, l3 = Lt.d;
                 const float inv = 1.0f / dist;
                 const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
                 const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
@@ -625,8 +625,8 @@ struct GenEngine {
         }
         if (L.have && L.sumWs > 0.0f && L.sumWu > 0.0f && L.sel_ph > 0.0f) {
             const int px = L.pixel;
-            const float4* Lt = sc.lights + size_t(L.sel_light) * kLightWords;
-            const float4 l2 = Lt[2], l3 = Lt[3];
+            const typename Tracer<S>::LightRec Lt = pt.light(L.sel_light);
+            const float4 l2 = Lt.c, l3 = Lt.d;
             const V3 q = light_point(Lt, L.sel_lu, L.sel_lv);
             ri.r0[px] = make_float4(q.x, q.y, q.z, L.sel_ph);
             ri.r1[px] = make_float4(l2.y, l2.z, l2.w, l3.w);
@@ -733,6 +733,7 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                                                                       BandQueue q, unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
+    pt.stage_lights();
     E eng(pt, P, io);
     const unsigned lane = __lane_id();
     int band = int(blockIdx.x) % kBands, tried = 0;
