@@ -20,7 +20,9 @@
 namespace rmh {
 namespace {
 
-enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2 };
+// kNee: the lane stands at a shading point and waits for k_persistent to run
+// its light sampling together with the other waiting lanes of its wave.
+enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2, kNee = 3 };
 
 // Position for the calling lane (when keep) in a list grown through *count,
 // one atomic per wave; undefined without keep.
@@ -33,7 +35,7 @@ __device__ __forceinline__ unsigned wave_append_pos(bool keep, unsigned* count) 
     base = __shfl(base, leader);
     return base + __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
 }
-enum Step : int { kTrace = 0, kPathEnd = 1, kPixelDone = 2 };
+enum Step : int { kTrace = 0, kPathEnd = 1, kPixelDone = 2, kWait = 3 };
 
 struct PathLane {
     int phase;
@@ -80,6 +82,10 @@ struct PathEngine {
     // stack takes S KiB of LDS per block: 160 KiB hold 3 blocks up to S = 48, 2 at 64.
     static constexpr int kMinBlocks = S <= 48 ? 3 : 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
+    static constexpr bool kDeferNee = true;
+    // Set by k_persistent: shade_hit leaves the light sampling to the loop
+    // (phase kNee, step result kWait) instead of running it at once.
+    bool defer_nee = false;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -454,13 +460,23 @@ struct PathEngine {
         float spec_prob = 0.05f + 0.95f * metallic;
         if (spec_prob > 0.98f) spec_prob = 0.98f;
         if (metallic >= 0.999f) spec_prob = 1.0f;
-        const typename Tracer<S>::NeeSample ns = pt.nee_prepare(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy,
-                                                                woz, rough, metallic, spec_prob, hit.ar, hit.ag,
-                                                                hit.ab);
         L.p = hit.p;
         L.pn = V3{pnx, pny, pnz};
         L.h_ar = hit.ar; L.h_ag = hit.ag; L.h_ab = hit.ab;
         L.h_rough = rough; L.h_alpha2 = alpha2; L.h_met = metallic; L.h_spec = spec_prob;
+        if (defer_nee) {
+            L.phase = kNee;
+            return kWait;
+        }
+        return nee(L);
+    }
+
+    // Light sampling at the vertex kept in L (the shading point, its normal,
+    // the material, and wo = -L.d), then its shadow ray.
+    __device__ int nee(Lane& L) {
+        const typename Tracer<S>::NeeSample ns =
+            pt.nee_prepare(L.p.x, L.p.y, L.p.z, L.pn.x, L.pn.y, L.pn.z, -L.d.x, -L.d.y, -L.d.z, L.h_rough, L.h_met,
+                           L.h_spec, L.h_ar, L.h_ag, L.h_ab);
         L.nr = ns.r; L.ng = ns.g; L.nb = ns.b;
         L.nsumW = ns.sumW;
         L.nM = ns.M;
@@ -498,6 +514,11 @@ struct PathEngine {
         const int r = on_trace(L, h);
         return r == kPathEnd ? next_sample(L) : r;
     }
+    // Runs the deferred light sampling of a lane in phase kNee.
+    __device__ int run_nee(Lane& L) {
+        const int r = nee(L);
+        return r == kPathEnd ? next_sample(L) : r;
+    }
     __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
 };
 
@@ -528,6 +549,7 @@ struct GenEngine {
     static constexpr int kPass = kCntGenerate;
     // shadow rays: latency hiding wins over the spills (LDS allows 4 blocks at S = 32)
     static constexpr int kMinBlocks = S <= 32 ? 4 : S <= 48 ? 3 : 2;
+    static constexpr bool kDeferNee = false;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -725,6 +747,15 @@ struct BandQueue {
     const unsigned int* order_n;
 };
 
+#ifndef RM_NEE_MAX_AGE
+#define RM_NEE_MAX_AGE 2
+#endif
+#ifndef RM_NEE_FRAC8
+#define RM_NEE_FRAC8 6
+#endif
+constexpr int kNeeMaxAge = RM_NEE_MAX_AGE;
+constexpr int kNeeFrac8 = RM_NEE_FRAC8;  // waiting share of the active lanes, in eighths, that runs the batch
+
 // The persistent loop shared by the engines: every iteration traces one ray
 // per active lane, then advances each lane; idle lanes take new pixels from
 // the queue (one atomic per wave) until every band is exhausted.
@@ -735,6 +766,8 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     pt.stage_lights();
     E eng(pt, P, io);
+    if constexpr (E::kDeferNee) eng.defer_nee = true;
+    int nee_age = 0;  // iterations the wave's waiting lanes have waited
     const unsigned lane = __lane_id();
     int band = int(blockIdx.x) % kBands, tried = 0;
     unsigned seg0 = 0, seg1 = 0, front = 0, size = 0;
@@ -778,13 +811,31 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                 }
             }
         }
-        if (__ballot(L.phase != kIdle) == 0) {
+        const unsigned long long act = __ballot(L.phase != kIdle);
+        if (act == 0) {
             if (exhausted) break;
             continue;
         }
-        if (L.phase != kIdle) {
+        if constexpr (E::kDeferNee) {
+            // The light sampling loop costs a wave the same whether one lane
+            // or all run it, so lanes reaching a shading point wait until
+            // three quarters of the active lanes do (which includes the case
+            // that no active lane has a ray to trace) or until they have
+            // waited kNeeMaxAge iterations. A lane's own sequence of
+            // operations is unchanged.
+            const unsigned long long wait = __ballot(L.phase == kNee);
+            if (wait) {
+                const int nw = __popcll(wait), na = __popcll(act);
+                if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge) {
+                    nee_age = 0;
+                    if (L.phase == kNee && eng.run_nee(L) == kPixelDone) L.phase = kIdle;
+                }
+            }
+        }
+        if (L.phase == kExt || L.phase == kShadow) {
             const Hit h = pt.intersect(L.ro, L.rd, L.tmax, eng.any_hit(L));
-            if (eng.step(L, h) != kTrace) L.phase = kIdle;
+            const int r = eng.step(L, h);
+            if (r != kTrace && r != kWait) L.phase = kIdle;
         }
     }
     pt.flush(cnt, E::kPass);
