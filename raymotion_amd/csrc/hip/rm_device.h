@@ -46,6 +46,41 @@ struct Rng {
     }
 };
 
+// Multiplier a and increment c of n steps of Rng's state recurrence
+// (s -> 747796405 s + 2891336453 mod 2^32): n steps map s to a s + c.
+__device__ __forceinline__ void lcg_jump(uint32_t n, uint32_t& a, uint32_t& c) {
+    uint32_t acc_a = 1u, acc_c = 0u, cur_a = 747796405u, cur_c = 2891336453u;
+    while (n) {
+        if (n & 1u) {
+            acc_a *= cur_a;
+            acc_c = acc_c * cur_a + cur_c;
+        }
+        cur_c = (cur_a + 1u) * cur_c;
+        cur_a *= cur_a;
+        n >>= 1;
+    }
+    a = acc_a;
+    c = acc_c;
+}
+
+// Position of the set bit of m with rank r (0: the lowest); r < popcount(m).
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long lo = m & ((1ull << w) - 1ull);
+        const int c = __popcll(lo);
+        if (r >= c) {
+            r -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = lo;
+        }
+    }
+    return pos;
+}
+
 __device__ __forceinline__ uint64_t pixel_seed(uint64_t seed, uint32_t x, uint32_t y) {
     return seed ^ (uint64_t(y + 1) * 0x9E3779B97F4A7C15ULL) ^ (uint64_t(x + 1) * 0xBF58476D1CE4E5B9ULL);
 }
@@ -553,6 +588,87 @@ struct Tracer {
         int light;
         float ler, leg, leb, er, eg, eb, k;
     };
+    // direct_lighting's RIS (pathtrace.metal:299-469): the shading point,
+    // one candidate, and the running reservoir of candidates taken so far.
+    struct RisIn {
+        float p_x, p_y, p_z, n_x, n_y, n_z, wo_x, wo_y, wo_z, alpha2, metallic, spec_prob, ar, ag, ab;
+    };
+    struct RisCand {
+        float qx, qy, qz, w, ph, epdf, Fr, Fg, Fb;
+    };
+    struct RisPick {
+        float sumW = 0.0f;
+        float lx = 0.0f, ly = 0.0f, lz = 0.0f, Fr = 0.0f, Fg = 0.0f, Fb = 0.0f, phat = 0.0f, epdf = 0.0f;
+        bool have = false;
+    };
+    // Candidate at folded barycentrics (lu, lv) of light record lr.
+    __device__ __forceinline__ RisCand ris_candidate(const LightRec& lr, float lu, float lv, const RisIn& in) const {
+        const float4 l0 = lr.a, l1 = lr.b, l2 = lr.c, l3 = lr.d;
+        if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
+        const float lw = 1.0f - lu - lv;
+        RisCand c;
+        c.qx = lw * l0.x + (lu * l0.w + lv * l1.z);
+        c.qy = lw * l0.y + (lu * l1.x + lv * l1.w);
+        c.qz = lw * l0.z + (lu * l1.y + lv * l2.x);
+        c.w = c.ph = c.epdf = c.Fr = c.Fg = c.Fb = 0.0f;
+        const float ldx = c.qx - in.p_x, ldy = c.qy - in.p_y, ldz = c.qz - in.p_z;
+        const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
+        if (dist > 0.003f) {
+            const float inv = 1.0f / dist;
+            const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
+            const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
+            const float scos = in.n_x * dxn + (in.n_y * dyn + in.n_z * dzn);
+            if (lcos > 1e-8f && scos > 1e-8f) {
+                float lt_lum = lum3(l3.x, l3.y, l3.z);
+                if (lt_lum < 1e-6f) lt_lum = 1e-6f;
+                const float q = dist * dist * lt_lum / (lcos * sc.light_total);
+                if (q > 1e-12f) {
+                    const Bsdf e = eval_bsdf(dxn, dyn, dzn, in.wo_x, in.wo_y, in.wo_z, in.n_x, in.n_y, in.n_z,
+                                             in.alpha2, in.metallic, in.spec_prob, in.ar, in.ag, in.ab);
+                    c.epdf = e.pdf;
+                    c.Fr = l3.x * e.r * scos * l3.w;
+                    c.Fg = l3.y * e.g * scos * l3.w;
+                    c.Fb = l3.z * e.b * scos * l3.w;
+                    c.ph = lum3(c.Fr, c.Fg, c.Fb);
+                    if (c.ph > 0.0f) c.w = c.ph / q;
+                }
+            }
+        }
+        return c;
+    }
+    // Reservoir update with the candidate's draw r.
+    static __device__ __forceinline__ void ris_take(RisPick& s, const RisCand& c, float r) {
+        s.sumW += c.w;
+        if (c.w > 0.0f && r * s.sumW < c.w) {
+            s.have = true;
+            s.lx = c.qx; s.ly = c.qy; s.lz = c.qz;
+            s.Fr = c.Fr; s.Fg = c.Fg; s.Fb = c.Fb;
+            s.phat = c.ph; s.epdf = c.epdf;
+        }
+    }
+    // The light sample of a finished reservoir: its shadow ray and weight.
+    __device__ void ris_finish(NeeSample& ns, const RisPick& s, int Mc, const RisIn& in) const {
+        ns.sumW = s.sumW;
+        ns.M = uint32_t(Mc);
+        if (!s.have || !(s.sumW > 0.0f) || !(s.phat > 0.0f)) return;
+        const float pdf_ris = float(Mc) * s.phat / s.sumW;
+        float wnee = 1.0f;
+        if (P.nomis == 0 && s.epdf > 1e-20f) wnee = pdf_ris / (pdf_ris + s.epdf);
+        float vdx = s.lx - in.p_x, vdy = s.ly - in.p_y, vdz = s.lz - in.p_z;
+        const float vdist = sqrtf(vdx * vdx + vdy * vdy + vdz * vdz);
+        if (!(vdist > 0.003f)) return;
+        const float vinv = 1.0f / vdist;
+        vdx *= vinv; vdy *= vinv; vdz *= vinv;
+        const float W = s.sumW / (float(Mc) * s.phat);
+        ns.shadow = true;
+        ns.o = V3{in.p_x + in.n_x * 0.002f, in.p_y + in.n_y * 0.002f, in.p_z + in.n_z * 0.002f};
+        ns.d = V3{vdx, vdy, vdz};
+        ns.dist = vdist - 0.003f;
+        ns.r = s.Fr * W * wnee;
+        ns.g = s.Fg * W * wnee;
+        ns.b = s.Fb * W * wnee;
+    }
+
     __device__ __forceinline__ NeeSample nee_prepare(float p_x, float p_y, float p_z, float n_x, float n_y, float n_z, float wo_x,
                                      float wo_y, float wo_z, float rough, float metallic, float spec_prob, float ar,
                                      float ag, float ab) {
@@ -622,57 +738,8 @@ struct Tracer {
             }
             return ns;
         }
-        float sumW = 0.0f;
-        float sel_lx = 0.0f, sel_ly = 0.0f, sel_lz = 0.0f, sel_Fr = 0.0f, sel_Fg = 0.0f, sel_Fb = 0.0f;
-        float sel_phat = 0.0f, sel_epdf = 0.0f;
-        bool have = false;
-        // One candidate's point, RIS weight and contribution.
-        struct Cand {
-            float qx, qy, qz, w, ph, epdf, Fr, Fg, Fb;
-        };
-        auto eval_cand = [&](const LightRec& lr, float lu, float lv) {
-            const float4 l0 = lr.a, l1 = lr.b, l2 = lr.c, l3 = lr.d;
-            if (lu + lv > 1.0f) { lu = 1.0f - lu; lv = 1.0f - lv; }
-            const float lw = 1.0f - lu - lv;
-            Cand c;
-            c.qx = lw * l0.x + (lu * l0.w + lv * l1.z);
-            c.qy = lw * l0.y + (lu * l1.x + lv * l1.w);
-            c.qz = lw * l0.z + (lu * l1.y + lv * l2.x);
-            c.w = c.ph = c.epdf = c.Fr = c.Fg = c.Fb = 0.0f;
-            const float ldx = c.qx - p_x, ldy = c.qy - p_y, ldz = c.qz - p_z;
-            const float dist = sqrtf(ldx * ldx + ldy * ldy + ldz * ldz);
-            if (dist > 0.003f) {
-                const float inv = 1.0f / dist;
-                const float dxn = ldx * inv, dyn = ldy * inv, dzn = ldz * inv;
-                const float lcos = -(l2.y * dxn + (l2.z * dyn + l2.w * dzn));
-                const float scos = n_x * dxn + (n_y * dyn + n_z * dzn);
-                if (lcos > 1e-8f && scos > 1e-8f) {
-                    float lt_lum = lum3(l3.x, l3.y, l3.z);
-                    if (lt_lum < 1e-6f) lt_lum = 1e-6f;
-                    const float q = dist * dist * lt_lum / (lcos * total);
-                    if (q > 1e-12f) {
-                        const Bsdf e = eval_bsdf(dxn, dyn, dzn, wo_x, wo_y, wo_z, n_x, n_y, n_z, alpha2, metallic,
-                                                 spec_prob, ar, ag, ab);
-                        c.epdf = e.pdf;
-                        c.Fr = l3.x * e.r * scos * l3.w;
-                        c.Fg = l3.y * e.g * scos * l3.w;
-                        c.Fb = l3.z * e.b * scos * l3.w;
-                        c.ph = lum3(c.Fr, c.Fg, c.Fb);
-                        if (c.ph > 0.0f) c.w = c.ph / q;
-                    }
-                }
-            }
-            return c;
-        };
-        auto take = [&](const Cand& c, float r) {
-            sumW += c.w;
-            if (c.w > 0.0f && r * sumW < c.w) {
-                have = true;
-                sel_lx = c.qx; sel_ly = c.qy; sel_lz = c.qz;
-                sel_Fr = c.Fr; sel_Fg = c.Fg; sel_Fb = c.Fb;
-                sel_phat = c.ph; sel_epdf = c.epdf;
-            }
-        };
+        const RisIn in{p_x, p_y, p_z, n_x, n_y, n_z, wo_x, wo_y, wo_z, alpha2, metallic, spec_prob, ar, ag, ab};
+        RisPick pk;
         // Candidates in pairs, so the two light searches and record reads
         // overlap; each candidate's draws (pick, lu, lv, then the reservoir's
         // r) and the reservoir updates keep their order.
@@ -684,35 +751,118 @@ struct Tracer {
             const float ub = urand(), vb = urand(), rb = urand();
             const LightRec A = light(pick_light(pa));
             const LightRec B = light(pick_light(pb));
-            const Cand ca = eval_cand(A, ua, va);
-            const Cand cb = eval_cand(B, ub, vb);
-            take(ca, ra);
-            take(cb, rb);
+            const RisCand ca = ris_candidate(A, ua, va, in);
+            const RisCand cb = ris_candidate(B, ub, vb, in);
+            ris_take(pk, ca, ra);
+            ris_take(pk, cb, rb);
         }
         if (j < Mc) {
             const float pa = urand() * total;
             const float ua = urand(), va = urand(), ra = urand();
-            take(eval_cand(light(pick_light(pa)), ua, va), ra);
+            ris_take(pk, ris_candidate(light(pick_light(pa)), ua, va, in), ra);
         }
-        ns.sumW = sumW;
-        ns.M = uint32_t(Mc);
-        if (!have || !(sumW > 0.0f) || !(sel_phat > 0.0f)) return ns;
-        const float pdf_ris = float(Mc) * sel_phat / sumW;
-        float wnee = 1.0f;
-        if (!nomis && sel_epdf > 1e-20f) wnee = pdf_ris / (pdf_ris + sel_epdf);
-        float vdx = sel_lx - p_x, vdy = sel_ly - p_y, vdz = sel_lz - p_z;
-        const float vdist = sqrtf(vdx * vdx + vdy * vdy + vdz * vdz);
-        if (!(vdist > 0.003f)) return ns;
-        const float vinv = 1.0f / vdist;
-        vdx *= vinv; vdy *= vinv; vdz *= vinv;
-        const float W = sumW / (float(Mc) * sel_phat);
-        ns.shadow = true;
-        ns.o = so;
-        ns.d = V3{vdx, vdy, vdz};
-        ns.dist = vdist - 0.003f;
-        ns.r = sel_Fr * W * wnee;
-        ns.g = sel_Fg * W * wnee;
-        ns.b = sel_Fb * W * wnee;
+        ris_finish(ns, pk, Mc, in);
+        return ns;
+    }
+
+    // direct_lighting's RIS for all lanes of the wave in `W` at once (each
+    // with its own shading point, in the arguments): every lane of the wave
+    // evaluates candidates of one waiting lane (group lane % |W|). A candidate
+    // takes exactly four draws (pick, lu, lv, r), so candidate j of a waiting
+    // lane starts 4j steps into its random sequence, reached by jumping its
+    // LCG state; each waiting lane then folds its candidates' weights in
+    // candidate order, so its sums, selection and final random state are those
+    // of nee_prepare bit for bit. The cost is ceil(Mc |W| / 64) candidate
+    // evaluations per lane instead of Mc. Called by every lane of the wave,
+    // with the same W, and only when nee_prepare would take its RIS branch
+    // (ris_coop_applies).
+    __device__ bool ris_coop_applies() const {
+        return sc.nlights != 0 && sc.light_total > 1e-12f && P.restir_candidates > 1;
+    }
+    __device__ NeeSample nee_prepare_coop(unsigned long long W, float p_x, float p_y, float p_z, float n_x, float n_y,
+                                          float n_z, float wo_x, float wo_y, float wo_z, float rough, float metallic,
+                                          float spec_prob, float ar, float ag, float ab) {
+        NeeSample ns;
+        ns.shadow = false;
+        ns.o = ns.d = V3{0.0f, 0.0f, 0.0f};
+        ns.dist = 0.0f;
+        ns.r = ns.g = ns.b = ns.sumW = 0.0f;
+        ns.M = 0;
+        ns.light = -1;
+        ns.ler = ns.leg = ns.leb = ns.er = ns.eg = ns.eb = ns.k = 0.0f;
+        int Mc = P.restir_candidates;
+        if (Mc > 32) Mc = 32;
+        float r2 = rough;
+        if (r2 < 0.0f) r2 = 0.0f;
+        if (r2 > 1.0f) r2 = 1.0f;
+        if (r2 < 0.02f) r2 = 0.02f;
+        const float alpha2 = (r2 * r2) * (r2 * r2);
+        const float total = sc.light_total;
+        const int lane = int(__lane_id());
+        const bool waiting = (W >> lane) & 1ull;
+        const int nw = __popcll(W);
+        const int grp = lane % nw, sub = lane / nw;
+        const int gsize = (64 - grp + nw - 1) / nw;       // lanes of group grp
+        const int lmin = 64 / nw, lmax = (64 + nw - 1) / nw;
+        const int owner = nth_set_bit(W, grp);           // the waiting lane this lane works for
+        const int mine = int(__builtin_amdgcn_mbcnt_hi(unsigned(W >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(W), 0u)));
+        const int msize = (64 - mine + nw - 1) / nw;      // size of this lane's own group, when waiting
+        const RisIn in{__shfl(p_x, owner), __shfl(p_y, owner), __shfl(p_z, owner), __shfl(n_x, owner),
+                       __shfl(n_y, owner), __shfl(n_z, owner), __shfl(wo_x, owner), __shfl(wo_y, owner),
+                       __shfl(wo_z, owner), __shfl(alpha2, owner), __shfl(metallic, owner),
+                       __shfl(spec_prob, owner), __shfl(ar, owner), __shfl(ag, owner), __shfl(ab, owner)};
+        const uint32_t s0 = uint32_t(__shfl(int(rng.state), owner));
+        uint32_t ja, jc, sa, sc_;
+        lcg_jump(4u * uint32_t(sub), ja, jc);     // to this lane's first candidate
+        lcg_jump(4u * uint32_t(gsize), sa, sc_);  // from one of its candidates to the next
+        uint32_t st = ja * s0 + jc;
+        RisPick pk;
+        int sel_lane = -1;
+        RisCand keep{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const int rounds = (Mc + lmin - 1) / lmin;
+        const int steps = lmax < Mc ? lmax : Mc;
+        for (int k = 0; k < rounds; ++k) {
+            const int jc0 = k * gsize + sub;  // this lane's candidate in round k
+            RisCand c{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            float r = 0.0f;
+            if (jc0 < Mc) {
+                Rng g;
+                g.state = st;
+                const float pick = g.next() * total;
+                const float lu = g.next(), lv = g.next();
+                r = g.next();
+                c = ris_candidate(light(pick_light(pick)), lu, lv, in);
+            }
+            st = sa * st + sc_;
+            // Waiting lanes fold their group's candidates of this round in order.
+            int round_sel = -1;
+            for (int t = 0; t < steps; ++t) {
+                const bool mine_t = waiting && t < msize && k * msize + t < Mc;
+                const int src = mine_t ? mine + t * nw : lane;
+                const float wt = __shfl(c.w, src), rt = __shfl(r, src);
+                if (mine_t) {
+                    pk.sumW += wt;
+                    if (wt > 0.0f && rt * pk.sumW < wt) {
+                        pk.have = true;
+                        round_sel = src;
+                    }
+                }
+            }
+            // A lane keeps its candidate while its owner's selection is it.
+            if (__shfl(round_sel, owner) == lane) keep = c;
+            if (round_sel >= 0) sel_lane = round_sel;
+        }
+        const int from = sel_lane >= 0 ? sel_lane : lane;
+        pk.lx = __shfl(keep.qx, from); pk.ly = __shfl(keep.qy, from); pk.lz = __shfl(keep.qz, from);
+        pk.Fr = __shfl(keep.Fr, from); pk.Fg = __shfl(keep.Fg, from); pk.Fb = __shfl(keep.Fb, from);
+        pk.phat = __shfl(keep.ph, from); pk.epdf = __shfl(keep.epdf, from);
+        if (waiting) {
+            uint32_t ma, mcn;
+            lcg_jump(4u * uint32_t(Mc), ma, mcn);
+            rng.state = ma * rng.state + mcn;
+            const RisIn own{p_x, p_y, p_z, n_x, n_y, n_z, wo_x, wo_y, wo_z, alpha2, metallic, spec_prob, ar, ag, ab};
+            ris_finish(ns, pk, Mc, own);
+        }
         return ns;
     }
 

@@ -474,9 +474,10 @@ struct PathEngine {
     // Light sampling at the vertex kept in L (the shading point, its normal,
     // the material, and wo = -L.d), then its shadow ray.
     __device__ int nee(Lane& L) {
-        const typename Tracer<S>::NeeSample ns =
-            pt.nee_prepare(L.p.x, L.p.y, L.p.z, L.pn.x, L.pn.y, L.pn.z, -L.d.x, -L.d.y, -L.d.z, L.h_rough, L.h_met,
-                           L.h_spec, L.h_ar, L.h_ag, L.h_ab);
+        return finish_nee(L, pt.nee_prepare(L.p.x, L.p.y, L.p.z, L.pn.x, L.pn.y, L.pn.z, -L.d.x, -L.d.y, -L.d.z,
+                                            L.h_rough, L.h_met, L.h_spec, L.h_ar, L.h_ag, L.h_ab));
+    }
+    __device__ int finish_nee(Lane& L, const typename Tracer<S>::NeeSample& ns) {
         L.nr = ns.r; L.ng = ns.g; L.nb = ns.b;
         L.nsumW = ns.sumW;
         L.nM = ns.M;
@@ -517,6 +518,17 @@ struct PathEngine {
     // Runs the deferred light sampling of a lane in phase kNee.
     __device__ int run_nee(Lane& L) {
         const int r = nee(L);
+        return r == kPathEnd ? next_sample(L) : r;
+    }
+    // The same for the waiting lanes W of the wave together (Tracer::
+    // nee_prepare_coop); every lane of the wave calls it. Returns the waiting
+    // lane's step result (kTrace for the others).
+    __device__ int run_nee_batch(Lane& L, unsigned long long W) {
+        const typename Tracer<S>::NeeSample ns =
+            pt.nee_prepare_coop(W, L.p.x, L.p.y, L.p.z, L.pn.x, L.pn.y, L.pn.z, -L.d.x, -L.d.y, -L.d.z, L.h_rough,
+                                L.h_met, L.h_spec, L.h_ar, L.h_ag, L.h_ab);
+        if (L.phase != kNee) return kTrace;
+        const int r = finish_nee(L, ns);
         return r == kPathEnd ? next_sample(L) : r;
     }
     __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
@@ -817,18 +829,25 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
             continue;
         }
         if constexpr (E::kDeferNee) {
-            // The light sampling loop costs a wave the same whether one lane
-            // or all run it, so lanes reaching a shading point wait until
-            // three quarters of the active lanes do (which includes the case
-            // that no active lane has a ray to trace) or until they have
-            // waited kNeeMaxAge iterations. A lane's own sequence of
-            // operations is unchanged.
+            // Lanes reaching a shading point wait until three quarters of
+            // the active lanes do (which includes the case that no active lane
+            // has a ray to trace) or until they have waited kNeeMaxAge
+            // iterations; then the wave samples lights for all of them at once
+            // (Tracer::nee_prepare_coop: ceil(Mc * waiting / 64) candidate
+            // evaluations per lane and an Mc-step fold, against Mc evaluations
+            // whenever any lane samples alone). A lane's own sequence of
+            // operations and random draws is unchanged.
             const unsigned long long wait = __ballot(L.phase == kNee);
             if (wait) {
                 const int nw = __popcll(wait), na = __popcll(act);
                 if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge) {
                     nee_age = 0;
-                    if (L.phase == kNee && eng.run_nee(L) == kPixelDone) L.phase = kIdle;
+                    if (pt.ris_coop_applies()) {
+                        const bool waiting = L.phase == kNee;
+                        if (eng.run_nee_batch(L, wait) == kPixelDone && waiting) L.phase = kIdle;
+                    } else if (L.phase == kNee && eng.run_nee(L) == kPixelDone) {
+                        L.phase = kIdle;
+                    }
                 }
             }
         }
