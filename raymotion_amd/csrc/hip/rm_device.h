@@ -133,6 +133,15 @@ struct NoAdjoint {
     __device__ __forceinline__ void dielectric(const T&...) {}
 };
 
+#ifndef RM_LEAF_MAX_AGE
+#define RM_LEAF_MAX_AGE 3
+#endif
+constexpr int kLeafMaxAge = RM_LEAF_MAX_AGE;  // traversal iterations a lane waits for a leaf batch at most
+#ifndef RM_LEAF_FRAC8
+#define RM_LEAF_FRAC8 4
+#endif
+constexpr int kLeafFrac8 = RM_LEAF_FRAC8;  // waiting share of the traversing lanes, in eighths, that runs it
+
 // LDS copy of the light table (kLightWords float4 per light) and its CDF, for
 // the kernels that stage it (Tracer::stage_lights). Accessed by name so the
 // reads stay LDS reads.
@@ -211,110 +220,137 @@ struct Tracer {
         const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
         int sp = 0;
         int node = 0;
+        // A lane whose node has hit leaf slots waits (pend) until the wave
+        // intersects leaves for all waiting lanes together: when kLeafFrac8
+        // eighths of the traversing lanes wait (so also when none can
+        // traverse) or they have waited kLeafMaxAge iterations. The leaf code then runs for most lanes at
+        // once instead of whenever any one lane needs it. Each lane's own
+        // sequence (the node's boxes, its leaves with the bound of that moment,
+        // then the next node) is unchanged.
+        bool pend = false, last = false;
+        int age = 0;
+        float tn[kWide];
+        int ref[kWide], kind[kWide];
         for (;;) {
-            const float4* n = sc.nodes + size_t(node) * kNodeWords;
-            float4 w[kNodeWords];
+            if (!pend) {
+                const float4* n = sc.nodes + size_t(node) * kNodeWords;
+                float4 w[kNodeWords];
 #pragma unroll
-            for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
-            ++tl.nodes;
-            const float tc = h.t + h.t * (1.0f / 4096.0f);
-            float tn[kWide];
-            int ref[kWide], kind[kWide];
+                for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
+                ++tl.nodes;
+                const float tc = h.t + h.t * (1.0f / 4096.0f);
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g) {
-                const float4 lox = w[node_lo_word(0, g)], hix = w[node_hi_word(0, g)];
-                const float4 loy = w[node_lo_word(1, g)], hiy = w[node_hi_word(1, g)];
-                const float4 loz = w[node_lo_word(2, g)], hiz = w[node_hi_word(2, g)];
-                const float4 rf = w[kNodeRefWord + g], kd = w[kNodeKindWord + g];
-                const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-                const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-                const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-                const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
-                const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
+                for (int g = 0; g < kGroups; ++g) {
+                    const float4 lox = w[node_lo_word(0, g)], hix = w[node_hi_word(0, g)];
+                    const float4 loy = w[node_lo_word(1, g)], hiy = w[node_hi_word(1, g)];
+                    const float4 loz = w[node_lo_word(2, g)], hiz = w[node_hi_word(2, g)];
+                    const float4 rf = w[kNodeRefWord + g], kd = w[kNodeKindWord + g];
+                    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+                    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+                    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+                    const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
+                    const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int c = 4 * g + j;
-                    const float ax = __builtin_fmaf(lx[j], ix, ox), bx = __builtin_fmaf(hx[j], ix, ox);
-                    const float ay = __builtin_fmaf(ly[j], iy, oy), by = __builtin_fmaf(hy[j], iy, oy);
-                    const float az = __builtin_fmaf(lz[j], iz, oz), bz = __builtin_fmaf(hz[j], iz, oz);
-                    const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-                    const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
-                    const bool ok = k4[j] >= 0 && nr <= fr;
-                    tn[c] = ok ? nr : __builtin_inff();
-                    ref[c] = r4[j];
-                    kind[c] = ok ? k4[j] : -1;
+                    for (int j = 0; j < 4; ++j) {
+                        const int c = 4 * g + j;
+                        const float ax = __builtin_fmaf(lx[j], ix, ox), bx = __builtin_fmaf(hx[j], ix, ox);
+                        const float ay = __builtin_fmaf(ly[j], iy, oy), by = __builtin_fmaf(hy[j], iy, oy);
+                        const float az = __builtin_fmaf(lz[j], iz, oz), bz = __builtin_fmaf(hz[j], iz, oz);
+                        const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+                        const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
+                        const bool ok = k4[j] >= 0 && nr <= fr;
+                        tn[c] = ok ? nr : __builtin_inff();
+                        ref[c] = r4[j];
+                        kind[c] = ok ? k4[j] : -1;
+                    }
                 }
-            }
-            // Sort the slots by entry distance (misses last): the optimal
-            // 5- and 19-comparator networks for four and eight inputs.
+                // Sort the slots by entry distance (misses last): the optimal
+                // 5- and 19-comparator networks for four and eight inputs.
 #define RM_CX(a, b)                                                   \
     if (tn[b] < tn[a]) {                                              \
         const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
         const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
         const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
     }
-            if constexpr (kWide == 4) {
-                RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
-            } else {
-                RM_CX(0, 2) RM_CX(1, 3) RM_CX(4, 6) RM_CX(5, 7) RM_CX(0, 4) RM_CX(1, 5) RM_CX(2, 6) RM_CX(3, 7)
-                RM_CX(0, 1) RM_CX(2, 3) RM_CX(4, 5) RM_CX(6, 7) RM_CX(2, 4) RM_CX(3, 5) RM_CX(1, 4) RM_CX(3, 6)
-                RM_CX(1, 2) RM_CX(3, 4) RM_CX(5, 6)
-            }
+                if constexpr (kWide == 4) {
+                    RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
+                } else {
+                    RM_CX(0, 2) RM_CX(1, 3) RM_CX(4, 6) RM_CX(5, 7) RM_CX(0, 4) RM_CX(1, 5) RM_CX(2, 6) RM_CX(3, 7)
+                    RM_CX(0, 1) RM_CX(2, 3) RM_CX(4, 5) RM_CX(6, 7) RM_CX(2, 4) RM_CX(3, 5) RM_CX(1, 4) RM_CX(3, 6)
+                    RM_CX(1, 2) RM_CX(3, 4) RM_CX(5, 6)
+                }
 #undef RM_CX
-            // Leaves first, nearest first.
 #pragma unroll
-            for (int c = 0; c < kWide; ++c) {
-                if (kind[c] > 0 && tn[c] <= h.t + h.t * (1.0f / 4096.0f)) {
-                    const int first = ref[c], count = kind[c];
-                    for (int k = 0; k < count; ++k) {
-                        const float4* g = sc.geo + size_t(first + k) * kTriGeoWords;
-                        const float4 g0 = g[0], g1 = g[1], g2 = g[2];
-                        ++tl.tris;
-                        const V3 v0{g0.x, g0.y, g0.z}, v1{g0.w, g1.x, g1.y}, v2{g1.z, g1.w, g2.x};
-                        const int id = ibits(g2.y);
-                        const V3 e1 = v1 - v0, e2 = v2 - v0;
-                        const V3 p = cross3(rd, e2);
-                        const float det = dot3(e1, p);
-                        if (fabsf(det) < 1e-12f) continue;
-                        const V3 tv = ro - v0;
-                        const float u = dot3(tv, p) / det;
-                        if (u < 0.0f || u > 1.0f) continue;
-                        const V3 q = cross3(tv, e1);
-                        const float v = dot3(rd, q) / det;
-                        if (v < 0.0f || u + v > 1.0f) continue;
-                        const float d = dot3(e2, q) / det;
-                        if (d > 1e-8f && (d < h.t || (d == h.t && h.tri >= 0 && id < h.tri))) {
-                            h.t = d;
-                            h.tri = id;
-                            h.u = u;
-                            h.v = v;
-                            if (any) return h;
+                for (int c = 0; c < kWide; ++c) pend = pend || kind[c] > 0;
+                // Inner children: continue with the nearest, stack the rest.
+                int next = -1;
+#pragma unroll
+                for (int c = kWide - 1; c >= 0; --c) {
+                    if (kind[c] == 0) {
+                        if (next >= 0) {
+                            if (sp < S) {
+                                stk[sp * kBlock] = uint32_t(next);
+                                ++sp;
+                            } else {
+                                tl.overflow = 1;
+                            }
+                        }
+                        next = ref[c];
+                    }
+                }
+                if (next < 0) {
+                    if (sp == 0) {
+                        last = true;
+                    } else {
+                        --sp;
+                        next = int(stk[sp * kBlock]);
+                    }
+                }
+                node = next;
+            }
+            const unsigned long long wait = __ballot(pend);
+            if (wait) {
+                const unsigned long long here = __ballot(true);
+                if (8 * __popcll(wait) >= kLeafFrac8 * __popcll(here) || ++age >= kLeafMaxAge) {
+                    age = 0;
+                    if (pend) {
+                        pend = false;
+                        // Leaves nearest first.
+#pragma unroll
+                        for (int c = 0; c < kWide; ++c) {
+                            if (kind[c] > 0 && tn[c] <= h.t + h.t * (1.0f / 4096.0f)) {
+                                const int first = ref[c], count = kind[c];
+                                for (int k = 0; k < count; ++k) {
+                                    const float4* g = sc.geo + size_t(first + k) * kTriGeoWords;
+                                    const float4 g0 = g[0], g1 = g[1], g2 = g[2];
+                                    ++tl.tris;
+                                    const V3 v0{g0.x, g0.y, g0.z}, v1{g0.w, g1.x, g1.y}, v2{g1.z, g1.w, g2.x};
+                                    const int id = ibits(g2.y);
+                                    const V3 e1 = v1 - v0, e2 = v2 - v0;
+                                    const V3 p = cross3(rd, e2);
+                                    const float det = dot3(e1, p);
+                                    if (fabsf(det) < 1e-12f) continue;
+                                    const V3 tv = ro - v0;
+                                    const float u = dot3(tv, p) / det;
+                                    if (u < 0.0f || u > 1.0f) continue;
+                                    const V3 q = cross3(tv, e1);
+                                    const float v = dot3(rd, q) / det;
+                                    if (v < 0.0f || u + v > 1.0f) continue;
+                                    const float d = dot3(e2, q) / det;
+                                    if (d > 1e-8f && (d < h.t || (d == h.t && h.tri >= 0 && id < h.tri))) {
+                                        h.t = d;
+                                        h.tri = id;
+                                        h.u = u;
+                                        h.v = v;
+                                        if (any) return h;
+                                    }
+                                }
+                            }
                         }
                     }
                 }
             }
-            // Inner children: continue with the nearest, stack the rest.
-            int next = -1;
-#pragma unroll
-            for (int c = kWide - 1; c >= 0; --c) {
-                if (kind[c] == 0) {
-                    if (next >= 0) {
-                        if (sp < S) {
-                            stk[sp * kBlock] = uint32_t(next);
-                            ++sp;
-                        } else {
-                            tl.overflow = 1;
-                        }
-                    }
-                    next = ref[c];
-                }
-            }
-            if (next < 0) {
-                if (sp == 0) break;
-                --sp;
-                next = int(stk[sp * kBlock]);
-            }
-            node = next;
+            if (last && !pend) break;
         }
         return h;
     }
