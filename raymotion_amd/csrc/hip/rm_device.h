@@ -199,6 +199,175 @@ struct Tracer {
     }
 
     // ------------------------------------------------------------ traversal
+    // Resumable traversal of one ray (intersect, k_persistent): the ray, the
+    // closest hit so far, the node to visit next, and the sorted child slots
+    // of the last node while its hit leaves wait (pend) for a leaf batch.
+    struct Trav {
+        V3 ro, rd;
+        float ix, iy, iz, ox, oy, oz;
+        Hit h;
+        int node, sp;
+        bool any, pend, last;
+        float tn[kWide];
+        int ref[kWide], kind[kWide];
+    };
+    // Starts a ray; false when there is nothing to traverse (T.h is the miss).
+    __device__ __forceinline__ bool trav_begin(Trav& T, V3 ro, V3 rd, float tmax, bool any) {
+        T.h.t = tmax;
+        T.h.tri = -1;
+        T.h.u = 0.0f;
+        T.h.v = 0.0f;
+        ++tl.rays;
+        T.ro = ro;
+        T.rd = rd;
+        T.any = any;
+        T.pend = false;
+        T.last = false;
+        T.sp = 0;
+        T.node = 0;
+        if (sc.nnodes == 0) return false;
+        T.ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x);
+        T.iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y);
+        T.iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
+        T.ox = -ro.x * T.ix;
+        T.oy = -ro.y * T.iy;
+        T.oz = -ro.z * T.iz;
+        return true;
+    }
+    // Visits T.node: tests its four child boxes, sorts the hit slots by entry
+    // distance, marks hit leaf slots as pending, stacks the inner children but
+    // the nearest, which becomes T.node; sets T.last when nothing is left.
+    __device__ __forceinline__ void trav_node(Trav& T) {
+        const float4* n = sc.nodes + size_t(T.node) * kNodeWords;
+        float4 w[kNodeWords];
+#pragma unroll
+        for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
+        ++tl.nodes;
+        const float tc = T.h.t + T.h.t * (1.0f / 4096.0f);
+        float* tn = T.tn;
+        int* ref = T.ref;
+        int* kind = T.kind;
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            const float4 lox = w[node_lo_word(0, g)], hix = w[node_hi_word(0, g)];
+            const float4 loy = w[node_lo_word(1, g)], hiy = w[node_hi_word(1, g)];
+            const float4 loz = w[node_lo_word(2, g)], hiz = w[node_hi_word(2, g)];
+            const float4 rf = w[kNodeRefWord + g], kd = w[kNodeKindWord + g];
+            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+            const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
+            const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = 4 * g + j;
+                const float ax = __builtin_fmaf(lx[j], T.ix, T.ox), bx = __builtin_fmaf(hx[j], T.ix, T.ox);
+                const float ay = __builtin_fmaf(ly[j], T.iy, T.oy), by = __builtin_fmaf(hy[j], T.iy, T.oy);
+                const float az = __builtin_fmaf(lz[j], T.iz, T.oz), bz = __builtin_fmaf(hz[j], T.iz, T.oz);
+                const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+                const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
+                const bool ok = k4[j] >= 0 && nr <= fr;
+                tn[c] = ok ? nr : __builtin_inff();
+                ref[c] = r4[j];
+                kind[c] = ok ? k4[j] : -1;
+            }
+        }
+        // Sort the slots by entry distance (misses last): the optimal
+        // 5- and 19-comparator networks for four and eight inputs.
+#define RM_CX(a, b)                                                   \
+    if (tn[b] < tn[a]) {                                              \
+        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
+        const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
+        const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
+    }
+        if constexpr (kWide == 4) {
+            RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
+        } else {
+            RM_CX(0, 2) RM_CX(1, 3) RM_CX(4, 6) RM_CX(5, 7) RM_CX(0, 4) RM_CX(1, 5) RM_CX(2, 6) RM_CX(3, 7)
+            RM_CX(0, 1) RM_CX(2, 3) RM_CX(4, 5) RM_CX(6, 7) RM_CX(2, 4) RM_CX(3, 5) RM_CX(1, 4) RM_CX(3, 6)
+            RM_CX(1, 2) RM_CX(3, 4) RM_CX(5, 6)
+        }
+#undef RM_CX
+#pragma unroll
+        for (int c = 0; c < kWide; ++c) T.pend = T.pend || kind[c] > 0;
+        // Inner children: continue with the nearest, stack the rest.
+        int next = -1;
+#pragma unroll
+        for (int c = kWide - 1; c >= 0; --c) {
+            if (kind[c] == 0) {
+                if (next >= 0) {
+                    if (T.sp < S) {
+                        stk[T.sp * kBlock] = uint32_t(next);
+                        ++T.sp;
+                    } else {
+                        tl.overflow = 1;
+                    }
+                }
+                next = ref[c];
+            }
+        }
+        if (next < 0) {
+            if (T.sp == 0) {
+                T.last = true;
+            } else {
+                --T.sp;
+                next = int(stk[T.sp * kBlock]);
+            }
+        }
+        T.node = next;
+    }
+    // Intersects the pending leaf slots of the last node, nearest first, each
+    // while its entry lies within the current hit (plus 2^-12 relative).
+    // Returns true when an any-hit ray has found its hit.
+    __device__ __forceinline__ bool trav_leaves(Trav& T) {
+        T.pend = false;
+#pragma unroll
+        for (int c = 0; c < kWide; ++c) {
+            if (T.kind[c] > 0 && T.tn[c] <= T.h.t + T.h.t * (1.0f / 4096.0f)) {
+                const int first = T.ref[c], count = T.kind[c];
+                for (int k = 0; k < count; ++k) {
+                    const float4* g = sc.geo + size_t(first + k) * kTriGeoWords;
+                    const float4 g0 = g[0], g1 = g[1], g2 = g[2];
+                    ++tl.tris;
+                    const V3 v0{g0.x, g0.y, g0.z}, v1{g0.w, g1.x, g1.y}, v2{g1.z, g1.w, g2.x};
+                    const int id = ibits(g2.y);
+                    const V3 e1 = v1 - v0, e2 = v2 - v0;
+                    const V3 p = cross3(T.rd, e2);
+                    const float det = dot3(e1, p);
+                    if (fabsf(det) < 1e-12f) continue;
+                    const V3 tv = T.ro - v0;
+                    const float u = dot3(tv, p) / det;
+                    if (u < 0.0f || u > 1.0f) continue;
+                    const V3 q = cross3(tv, e1);
+                    const float v = dot3(T.rd, q) / det;
+                    if (v < 0.0f || u + v > 1.0f) continue;
+                    const float d = dot3(e2, q) / det;
+                    if (d > 1e-8f && (d < T.h.t || (d == T.h.t && T.h.tri >= 0 && id < T.h.tri))) {
+                        T.h.t = d;
+                        T.h.tri = id;
+                        T.h.u = u;
+                        T.h.v = v;
+                        if (T.any) return true;
+                    }
+                }
+            }
+        }
+        return false;
+    }
+    // Whether the wave runs a leaf batch now, for the lanes `trav` that are
+    // traversing and `wait` of them whose leaves are pending: when kLeafFrac8
+    // eighths of the traversing lanes wait (so also when none can go on) or
+    // they have waited kLeafMaxAge iterations (age is the wave's counter).
+    static __device__ __forceinline__ bool leaf_batch_now(unsigned long long wait, unsigned long long trav,
+                                                          int& age) {
+        if (!wait) return false;
+        if (8 * __popcll(wait) >= kLeafFrac8 * __popcll(trav) || ++age >= kLeafMaxAge) {
+            age = 0;
+            return true;
+        }
+        return false;
+    }
+
     // Closest hit (or, with any, the first hit) along o + t d for t in
     // (1e-8, tmax). Children of each 4-wide node are tested together; hit leaves
     // are intersected nearest-first and hit inner children are descended
@@ -206,153 +375,20 @@ struct Tracer {
     // when its entry lies beyond the current hit by more than 2^-12 (relative),
     // and equal distances resolve to the smaller triangle index, so the result
     // is the closest hit regardless of visit order (same rule as the oracle).
+    // A lane whose node has hit leaf slots waits (pend) until the wave
+    // intersects leaves for all waiting lanes together (leaf_batch_now); each
+    // lane's own sequence (the node's boxes, its leaves with the bound of that
+    // moment, then the next node) is unchanged.
     __device__ __forceinline__ Hit intersect(V3 ro, V3 rd, float tmax, bool any) {
-        Hit h;
-        h.t = tmax;
-        h.tri = -1;
-        h.u = 0.0f;
-        h.v = 0.0f;
-        ++tl.rays;
-        if (sc.nnodes == 0) return h;
-        const float ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x);
-        const float iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y);
-        const float iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
-        const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
-        int sp = 0;
-        int node = 0;
-        // A lane whose node has hit leaf slots waits (pend) until the wave
-        // intersects leaves for all waiting lanes together: when kLeafFrac8
-        // eighths of the traversing lanes wait (so also when none can
-        // traverse) or they have waited kLeafMaxAge iterations. The leaf code then runs for most lanes at
-        // once instead of whenever any one lane needs it. Each lane's own
-        // sequence (the node's boxes, its leaves with the bound of that moment,
-        // then the next node) is unchanged.
-        bool pend = false, last = false;
+        Trav T;
+        if (!trav_begin(T, ro, rd, tmax, any)) return T.h;
         int age = 0;
-        float tn[kWide];
-        int ref[kWide], kind[kWide];
         for (;;) {
-            if (!pend) {
-                const float4* n = sc.nodes + size_t(node) * kNodeWords;
-                float4 w[kNodeWords];
-#pragma unroll
-                for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
-                ++tl.nodes;
-                const float tc = h.t + h.t * (1.0f / 4096.0f);
-#pragma unroll
-                for (int g = 0; g < kGroups; ++g) {
-                    const float4 lox = w[node_lo_word(0, g)], hix = w[node_hi_word(0, g)];
-                    const float4 loy = w[node_lo_word(1, g)], hiy = w[node_hi_word(1, g)];
-                    const float4 loz = w[node_lo_word(2, g)], hiz = w[node_hi_word(2, g)];
-                    const float4 rf = w[kNodeRefWord + g], kd = w[kNodeKindWord + g];
-                    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-                    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-                    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-                    const int r4[4] = {ibits(rf.x), ibits(rf.y), ibits(rf.z), ibits(rf.w)};
-                    const int k4[4] = {ibits(kd.x), ibits(kd.y), ibits(kd.z), ibits(kd.w)};
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int c = 4 * g + j;
-                        const float ax = __builtin_fmaf(lx[j], ix, ox), bx = __builtin_fmaf(hx[j], ix, ox);
-                        const float ay = __builtin_fmaf(ly[j], iy, oy), by = __builtin_fmaf(hy[j], iy, oy);
-                        const float az = __builtin_fmaf(lz[j], iz, oz), bz = __builtin_fmaf(hz[j], iz, oz);
-                        const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-                        const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
-                        const bool ok = k4[j] >= 0 && nr <= fr;
-                        tn[c] = ok ? nr : __builtin_inff();
-                        ref[c] = r4[j];
-                        kind[c] = ok ? k4[j] : -1;
-                    }
-                }
-                // Sort the slots by entry distance (misses last): the optimal
-                // 5- and 19-comparator networks for four and eight inputs.
-#define RM_CX(a, b)                                                   \
-    if (tn[b] < tn[a]) {                                              \
-        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
-        const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
-        const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
-    }
-                if constexpr (kWide == 4) {
-                    RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
-                } else {
-                    RM_CX(0, 2) RM_CX(1, 3) RM_CX(4, 6) RM_CX(5, 7) RM_CX(0, 4) RM_CX(1, 5) RM_CX(2, 6) RM_CX(3, 7)
-                    RM_CX(0, 1) RM_CX(2, 3) RM_CX(4, 5) RM_CX(6, 7) RM_CX(2, 4) RM_CX(3, 5) RM_CX(1, 4) RM_CX(3, 6)
-                    RM_CX(1, 2) RM_CX(3, 4) RM_CX(5, 6)
-                }
-#undef RM_CX
-#pragma unroll
-                for (int c = 0; c < kWide; ++c) pend = pend || kind[c] > 0;
-                // Inner children: continue with the nearest, stack the rest.
-                int next = -1;
-#pragma unroll
-                for (int c = kWide - 1; c >= 0; --c) {
-                    if (kind[c] == 0) {
-                        if (next >= 0) {
-                            if (sp < S) {
-                                stk[sp * kBlock] = uint32_t(next);
-                                ++sp;
-                            } else {
-                                tl.overflow = 1;
-                            }
-                        }
-                        next = ref[c];
-                    }
-                }
-                if (next < 0) {
-                    if (sp == 0) {
-                        last = true;
-                    } else {
-                        --sp;
-                        next = int(stk[sp * kBlock]);
-                    }
-                }
-                node = next;
-            }
-            const unsigned long long wait = __ballot(pend);
-            if (wait) {
-                const unsigned long long here = __ballot(true);
-                if (8 * __popcll(wait) >= kLeafFrac8 * __popcll(here) || ++age >= kLeafMaxAge) {
-                    age = 0;
-                    if (pend) {
-                        pend = false;
-                        // Leaves nearest first.
-#pragma unroll
-                        for (int c = 0; c < kWide; ++c) {
-                            if (kind[c] > 0 && tn[c] <= h.t + h.t * (1.0f / 4096.0f)) {
-                                const int first = ref[c], count = kind[c];
-                                for (int k = 0; k < count; ++k) {
-                                    const float4* g = sc.geo + size_t(first + k) * kTriGeoWords;
-                                    const float4 g0 = g[0], g1 = g[1], g2 = g[2];
-                                    ++tl.tris;
-                                    const V3 v0{g0.x, g0.y, g0.z}, v1{g0.w, g1.x, g1.y}, v2{g1.z, g1.w, g2.x};
-                                    const int id = ibits(g2.y);
-                                    const V3 e1 = v1 - v0, e2 = v2 - v0;
-                                    const V3 p = cross3(rd, e2);
-                                    const float det = dot3(e1, p);
-                                    if (fabsf(det) < 1e-12f) continue;
-                                    const V3 tv = ro - v0;
-                                    const float u = dot3(tv, p) / det;
-                                    if (u < 0.0f || u > 1.0f) continue;
-                                    const V3 q = cross3(tv, e1);
-                                    const float v = dot3(rd, q) / det;
-                                    if (v < 0.0f || u + v > 1.0f) continue;
-                                    const float d = dot3(e2, q) / det;
-                                    if (d > 1e-8f && (d < h.t || (d == h.t && h.tri >= 0 && id < h.tri))) {
-                                        h.t = d;
-                                        h.tri = id;
-                                        h.u = u;
-                                        h.v = v;
-                                        if (any) return h;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            if (last && !pend) break;
+            if (!T.pend) trav_node(T);
+            if (leaf_batch_now(__ballot(T.pend), __ballot(true), age) && T.pend && trav_leaves(T)) return T.h;
+            if (T.last && !T.pend) break;
         }
-        return h;
+        return T.h;
     }
 
     // cast_ray (pathtrace.metal:165-204): hit + interpolated normal + material.

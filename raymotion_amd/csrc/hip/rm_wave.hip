@@ -83,6 +83,9 @@ struct PathEngine {
     static constexpr int kMinBlocks = S <= 48 ? 3 : 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     static constexpr bool kDeferNee = true;
+    // Traversal inside the persistent loop with refill (k_persistent): lost
+    // here, the extra live state spills in this register-bound engine.
+    static constexpr bool kFused = false;
     // Set by k_persistent: shade_hit leaves the light sampling to the loop
     // (phase kNee, step result kWait) instead of running it at once.
     bool defer_nee = false;
@@ -562,6 +565,7 @@ struct GenEngine {
     // shadow rays: latency hiding wins over the spills (LDS allows 4 blocks at S = 32)
     static constexpr int kMinBlocks = S <= 32 ? 4 : S <= 48 ? 3 : 2;
     static constexpr bool kDeferNee = false;
+    static constexpr bool kFused = true;  // traversal inside the persistent loop, with refill
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -766,6 +770,10 @@ struct BandQueue {
 #define RM_NEE_FRAC8 6
 #endif
 constexpr int kNeeMaxAge = RM_NEE_MAX_AGE;
+#ifndef RM_TRAV_EXIT8
+#define RM_TRAV_EXIT8 4
+#endif
+constexpr int kTravExit8 = RM_TRAV_EXIT8;
 constexpr int kNeeFrac8 = RM_NEE_FRAC8;  // waiting share of the active lanes, in eighths, that runs the batch
 
 // The persistent loop shared by the engines: every iteration traces one ray
@@ -793,6 +801,14 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
     typename E::Lane L;
     L.phase = kIdle;
     bool exhausted = false;
+    // E::kFused: the lane's ray is traversed in the loop below (trav) until
+    // it is done (ready), then handed to the engine.
+    typename Tracer<S>::Trav T;
+    bool trav = false, ready = false;
+    auto begin = [&]() {
+        if (pt.trav_begin(T, L.ro, L.rd, L.tmax, eng.any_hit(L))) trav = true;
+        else ready = true;
+    };
     for (;;) {
         if (!exhausted) {
             const unsigned long long need = __ballot(L.phase == kIdle);
@@ -811,6 +827,7 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                                                                             : q.order[seg1 - 1u - (i - front)];
                         L.phase = kExt;  // start() sets the real phase
                         if (eng.start(L, int(px)) != kTrace) L.phase = kIdle;
+                        else if (E::kFused) begin();
                     }
                 }
                 if (base + unsigned(count) >= size) {  // this band is done: the next one
@@ -842,16 +859,46 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                 const int nw = __popcll(wait), na = __popcll(act);
                 if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge) {
                     nee_age = 0;
-                    if (pt.ris_coop_applies()) {
-                        const bool waiting = L.phase == kNee;
-                        if (eng.run_nee_batch(L, wait) == kPixelDone && waiting) L.phase = kIdle;
-                    } else if (L.phase == kNee && eng.run_nee(L) == kPixelDone) {
-                        L.phase = kIdle;
+                    const bool waiting = L.phase == kNee;
+                    const int r = pt.ris_coop_applies() ? eng.run_nee_batch(L, wait)
+                                                        : (waiting ? eng.run_nee(L) : int(kTrace));
+                    if (waiting) {
+                        if (r == kPixelDone) L.phase = kIdle;
+                        else if (E::kFused && r == kTrace) begin();
                     }
                 }
             }
         }
-        if (L.phase == kExt || L.phase == kShadow) {
+        if constexpr (E::kFused) {
+            // Lanes whose ray is done take their step; a new ray starts at once.
+            if (ready) {
+                ready = false;
+                const int r = eng.step(L, T.h);
+                if (r == kTrace) begin();
+                else if (r != kWait) L.phase = kIdle;
+            }
+            // Traverse until no more than kTravExit8 eighths of the working lanes
+            // still traverse, so lanes with short rays take new ones instead of
+            // waiting for the wave's longest traversal.
+            {
+                const int nwork = __popcll(__ballot(L.phase != kIdle));
+                int leaf_age = 0;
+                unsigned long long tv;
+                do {
+                    if (trav && !T.pend) pt.trav_node(T);
+                    if (Tracer<S>::leaf_batch_now(__ballot(trav && T.pend), __ballot(trav), leaf_age) && trav &&
+                        T.pend && pt.trav_leaves(T)) {
+                        trav = false;
+                        ready = true;
+                    }
+                    if (trav && T.last && !T.pend) {
+                        trav = false;
+                        ready = true;
+                    }
+                    tv = __ballot(trav);
+                } while (8 * __popcll(tv) > kTravExit8 * nwork);
+            }
+        } else if (L.phase == kExt || L.phase == kShadow) {
             const Hit h = pt.intersect(L.ro, L.rd, L.tmax, eng.any_hit(L));
             const int r = eng.step(L, h);
             if (r != kTrace && r != kWait) L.phase = kIdle;
