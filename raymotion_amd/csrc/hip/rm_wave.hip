@@ -77,14 +77,15 @@ template <int S, bool SHADE>
 struct PathEngine {
     using Lane = PathLane;
     static constexpr int kPass = SHADE ? kCntShade : kCntMega;
-    // Resident 256-lane blocks per CU the register allocation targets (3 waves
-    // per SIMD: a little spilling, measured faster than 2 waves without). The
-    // stack takes S KiB of LDS per block: 160 KiB hold 3 blocks up to S = 48, 2 at 64.
-    static constexpr int kMinBlocks = S <= 48 ? 3 : 2;
+    // Resident 256-lane blocks per CU the register allocation targets: 2
+    // waves per SIMD without spills (C1 shade 10.0 ms) against 3 with 196 B
+    // of spills per lane (10.3-10.7 ms), since the light-sampling and leaf
+    // batches; the stack takes S KiB of LDS per block.
+    static constexpr int kMinBlocks = 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     static constexpr bool kDeferNee = true;
-    // Traversal inside the persistent loop with refill (k_persistent): lost
-    // here, the extra live state spills in this register-bound engine.
+    // Traversal inside the persistent loop with refill (k_persistent): not
+    // kept here (C1 shade 10.1-10.2 ms against 9.9-10.0 ms with the call).
     static constexpr bool kFused = false;
     // Set by k_persistent: shade_hit leaves the light sampling to the loop
     // (phase kNee, step result kWait) instead of running it at once.
