@@ -78,9 +78,10 @@ struct PathEngine {
     using Lane = PathLane;
     static constexpr int kPass = SHADE ? kCntShade : kCntMega;
     // Resident 256-lane blocks per CU the register allocation targets: 2
-    // waves per SIMD without spills (C1 shade 10.0 ms) against 3 with 196 B
-    // of spills per lane (10.3-10.7 ms), since the light-sampling and leaf
-    // batches; the stack takes S KiB of LDS per block.
+    // (waves per SIMD) without spills serves trees that stay in the caches
+    // (327 K triangles, 163 K nodes: shade 10.1 ms against 10.2-10.3 ms at 3
+    // with 196 B of spills per lane); PathEngineDeep below serves large trees.
+    // The stack takes S KiB of LDS per block.
     static constexpr int kMinBlocks = 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     static constexpr bool kDeferNee = true;
@@ -557,6 +558,20 @@ struct GenLane {
     float sel_lu, sel_lv;
     int c_light;  // candidate waiting for its shadow ray
     float c_lu, c_lv, c_wu, c_ph;
+};
+
+// The path engine at 3 blocks per CU (with some spilling) for large trees,
+// whose node fetches miss the caches more and want more waves to hide them
+// (2.1 M triangles, 1.03 M nodes: shade 7.5 ms against 8.6 ms at 2 blocks).
+// launch_wave takes it for trees of more than kDeepTreeNodes nodes; 3 blocks'
+// stacks fit the LDS up to S = 48.
+constexpr int kDeepTreeNodes = 1 << 19;
+
+template <int S, bool SHADE>
+struct PathEngineDeep : PathEngine<S, SHADE> {
+    static_assert(S <= 48, "LDS holds 3 blocks up to S = 48");
+    using PathEngine<S, SHADE>::PathEngine;
+    static constexpr int kMinBlocks = 3;
 };
 
 template <int S>
@@ -1259,8 +1274,16 @@ void launch_wave(int stack, bool shade, const DevScene& sc, const FrameParams& P
         q.order = order;
         q.order_n = order_n;
     }
+    const bool deep = sc.nnodes > kDeepTreeNodes;
     with_stack(stack, [&](auto cap) {
         constexpr int S = decltype(cap)::value;
+        if constexpr (S <= 48) {
+            if (deep) {
+                if (shade) launch_persistent<S, PathEngineDeep<S, true>>(sc, P, io, q, cnt, st);
+                else launch_persistent<S, PathEngineDeep<S, false>>(sc, P, io, q, cnt, st);
+                return;
+            }
+        }
         if (shade) launch_persistent<S, PathEngine<S, true>>(sc, P, io, q, cnt, st);
         else launch_persistent<S, PathEngine<S, false>>(sc, P, io, q, cnt, st);
     });

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_bindings as ob
-from helpers import cornell, cornell_frame, tri
+from helpers import cornell, cornell_frame, frame, rel_err, tri
 from raymotion_amd import Scene, scenes
 from raymotion_amd.denoise import restir_frame
 from raymotion_amd.rng import random_rays
@@ -150,3 +150,16 @@ def test_full_heightfield_oracle(backend):
     assert st.stack_depth <= 48  # the kernels with a 48-entry stack keep their occupancy
     bad, t_bad = _hits_equal(backend, e, 1 << 17, 9)
     assert bad == 0 and t_bad == 0
+    # frames on this tree (over 2^19 nodes: the path engine's large-tree
+    # instance) against the oracle's frames on the host tree
+    assert st.bvh_nodes > 1 << 19
+    cam = scenes.heightfield_camera(3)
+    for name, f in (("restir", frame(96, 54, 4, 6, cam=cam, candidates=8, spatial=4, radius=16.0)),
+                    ("megakernel", frame(96, 54, 4, 6, cam=cam))):
+        g_rgb, g_hdr = backend.render(f, 4321)
+        c_rgb, c_hdr, info = ob.render_frame(e, f, 4321)
+        err = rel_err(g_hdr, c_hdr)
+        print(f"heightfield {name} frame: path {info['path']}, max rel {err.max():.3g}, "
+              f"exact {np.mean(g_hdr == c_hdr):.6f}")
+        assert info["path"] == (1 if name == "restir" else 0)
+        assert err.max() <= 1e-3
