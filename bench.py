@@ -27,11 +27,16 @@ goes through the drop-in boundary: the host marshals the whole scene, the
 backend uploads what changed and refits the resident tree on the GPU (a
 rebuild from the host tree every 32nd frame), then renders. ms/frame is that boundary path;
 the scene program's own vertex update and FP64 host refit are reported beside.
+
+--workload rmse states each rendering config's image RMSE (linear and
+tonemapped) against a 4096-spp path-traced reference of the same view, whose
+64 launches of 64 spp are accumulated in FP64.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import re
 import os
 import sys
@@ -237,6 +242,110 @@ def run_heightfield(args) -> dict:
     return line
 
 
+def cornell_frame_c0() -> rm.Frame:
+    """The Cornell box frame: 256x256, 4 spp, depth 5, NEE+MIS, ReSTIR and
+    denoiser off; pinhole at (0.5, 0.5, -1.4) with a 40 degree vertical field
+    of view (focal = (H/2) / tan(20 deg))."""
+    f = rm.Frame()
+    f.width, f.height, f.spp, f.bounces = 256, 256, 4, 5
+    f.resolution, f.clamp_value = 1.0, 16.0
+    f.adapt_min, f.adapt_step, f.adapt_rel, f.adapt_abs = 16, 8, 0.02, 0.002
+    f.nomis = 0
+    f.restir_candidates, f.restir_spatial, f.restir_mcap, f.restir_radius = 1, 0, 128, 16.0
+    f.denoise, f.denoise_iterations, f.temporal = 0, 0, 0
+    cam = scenes.cornell_camera()
+    cam.focal = 128.0 / math.tan(math.radians(20.0))
+    f.camera = cam
+    return f
+
+
+def reference_of(f: rm.Frame) -> rm.Frame:
+    """Path-traced reference settings for frame f: same size, camera, depth and
+    clamp; legacy light sampling, no reuse, no denoiser, no adaptive stop."""
+    r = rm.Frame.from_buffer_copy(f)
+    r.spp = REF_BATCH_SPP
+    r.adapt_min = 0
+    r.restir_candidates, r.restir_spatial = 1, 0
+    r.denoise, r.temporal = 0, 0
+    return r
+
+
+REF_SPP, REF_BATCH_SPP = 4096, 64
+
+
+def reference_image(backend, f: rm.Frame, seed0: int):
+    """REF_SPP samples per pixel as REF_SPP / REF_BATCH_SPP launches with
+    distinct seeds; the launches' per-pixel means are accumulated in FP64.
+    Returns the mean and its standard error per pixel channel."""
+    r = reference_of(f)
+    n = REF_SPP // REF_BATCH_SPP
+    acc = None
+    for k in range(n):
+        _, hdr = backend.render(r, seed0 + 7919 * k)
+        x = hdr.astype(np.float64)
+        if acc is None:
+            acc, acc2 = np.zeros_like(x), np.zeros_like(x)
+        acc += x
+        acc2 += x * x
+    mean = acc / n
+    stderr = np.sqrt(np.maximum(acc2 / n - mean * mean, 0.0) / (n - 1))
+    return mean, stderr
+
+
+def image_errors(img, ref, ref_stderr) -> dict:
+    from raymotion_amd.denoise import rmse, tonemapped_rmse
+    return {"rmse": round(rmse(img, ref), 6), "rmse_tonemapped": round(tonemapped_rmse(img, ref), 6),
+            "reference_noise_rms": round(float(np.sqrt(np.mean(ref_stderr ** 2))), 6)}
+
+
+def run_rmse(args) -> dict:
+    """Each rendering config's output frame against a 4096-spp path-traced
+    reference of the same view accumulated in FP64: the Cornell box (4 spp
+    megakernel), the mixed scene (frame 4 of the orbit after 4 frames of
+    history, 16 spp ReSTIR + 5 a-trous passes), the animated height field
+    (frame 4 after 4 frames, 8 spp ReSTIR + 2 a-trous passes, GPU tree)."""
+    out = {}
+    t0 = time.perf_counter()
+    backend = rm.Backend(0)
+    # Cornell box
+    s = scenes.cornell_box()
+    s.build_bvh()
+    backend.set_scene(s)
+    f = cornell_frame_c0()
+    _, img = backend.render(f, 12345)
+    ref, se = reference_image(backend, f, 500_000)
+    out["cornell_256_4spp"] = image_errors(img, ref, se)
+    # mixed scene, orbit frames 0..4
+    s = scenes.mixed_spheres(seed=0)
+    s.build_bvh()
+    backend.set_scene(s, gpu_tree=True, rebuild=True)
+    for k in range(5):
+        f = frame_settings(k)
+        f.temporal = 0 if k == 0 else 1
+        _, img = backend.render(f, 12345 + k)
+    ref, se = reference_image(backend, f, 600_000)
+    out["mixed_720p_16spp_restir_atrous"] = image_errors(img, ref, se)
+    # animated height field, frames 0..4
+    s = scenes.heightfield(1024, seed=0)
+    anim = scenes.HeightFieldAnimation(1024, seed=0)
+    for k in range(5):
+        s.set_positions(anim.positions(k / 60.0))
+        backend.set_scene(s, gpu_tree=True, rebuild=k == 0)
+        f = heightfield_frame(k)
+        f.temporal = 0 if k == 0 else 1
+        _, img = backend.render(f, 12345 + k)
+    ref, se = reference_image(backend, f, 700_000)
+    out["heightfield_1080p_8spp_restir_atrous"] = image_errors(img, ref, se)
+    backend.close()
+    return {"metric": "image RMSE against a 4096-spp FP64-accumulated path-traced reference", "value": None,
+            "unit": "linear radiance RMSE", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": None,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (procedural scenes, seed 0)",
+            "config": {"workload": "per-config image error", "reference_spp": REF_SPP,
+                       "reference_batches": REF_SPP // REF_BATCH_SPP, "seconds": round(time.perf_counter() - t0, 1)},
+            "errors": out}
+
+
 def adjoint_cost(steps: int = 5) -> dict:
     """Forward vs path-replay adjoint GPU time on the 16-material inverse-rendering
     scene (the mixed generator, 327,698 triangles) at 512x512, 16 spp, depth 8."""
@@ -403,7 +512,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed frames (10; heightfield 8)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed frames first (2; heightfield 1)")
-    ap.add_argument("--workload", choices=("mixed", "heightfield", "inverse", "denoiser-train", "denoiser-infer"),
+    ap.add_argument("--workload", choices=("mixed", "heightfield", "inverse", "denoiser-train", "denoiser-infer", "rmse"),
                     default="mixed")
     ap.add_argument("--views", type=int, default=512, help="denoiser-train: training views")
     ap.add_argument("--ref-spp", type=int, default=4096, help="denoiser-train: reference samples per pixel")
@@ -442,6 +551,8 @@ def main() -> int:
         line = run_denoiser_train(args)
     elif args.workload == "denoiser-infer":
         line = run_denoiser_infer(args)
+    elif args.workload == "rmse":
+        line = run_rmse(args)
     else:
         line = run_heightfield(args) if hf else run_mixed(args)
         if args.workload == "mixed" and not args.no_extras:
