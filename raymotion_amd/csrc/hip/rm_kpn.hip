@@ -48,76 +48,153 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// One wave computes 32 pixels x COUT channels: two 16-pixel row tiles times
-// COUT / 16 column tiles; a block is 4 waves (128 pixels).
+// Pixels per wave tile (kConvJ row tiles of 16), k-steps of A fragments
+// loaded ahead, and resident blocks per CU. Measured on the 1280x720 network
+// (ms for the 6 layers): J = 4 with 1 / 2 / 3 steps ahead 1.20 / 1.18 / 1.15
+// (before the epilogue fix), J = 2 1.33; J = 8 spills.
+constexpr int kConvJ = 4;
+constexpr int kConvDepth = 2;
+constexpr int kConvBlocksPerCU = 2;
+
+// A block stages the layer's whole weight matrix in LDS once, already in the
+// MFMA B-fragment order (k-step s, column tile t, lane: 16 bytes each, so a
+// wave's fragment read is one contiguous 1 KiB line), then its 4 waves loop
+// over tiles of 16 * kConvJ pixels x COUT channels. Per k-step a wave loads
+// kConvJ A fragments (8 channels of one shifted pixel per lane, from global
+// memory through L1) and reads COUT / 16 B fragments from LDS, feeding
+// kConvJ * COUT / 16 MFMAs. Each output's sum runs over the k-steps in order,
+// as a plain GEMM loop would. At most 72 KiB of LDS per block: 2 blocks per CU.
 template <int CIN, int COUT, int MODE>
-__global__ __launch_bounds__(256) void k_conv(const __bf16* __restrict__ X, const __bf16* __restrict__ Wt,
-                                              const float* __restrict__ bias, const __bf16* __restrict__ act,
-                                              void* __restrict__ out, int N, int H, int W) {
+__global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv(const __bf16* __restrict__ X,
+                                                              const __bf16* __restrict__ Wt,
+                                                              const float* __restrict__ bias,
+                                                              const __bf16* __restrict__ act,
+                                                              void* __restrict__ out, int N, int H, int W) {
     constexpr int KP = (9 * CIN + 31) / 32 * 32;
+    constexpr int KS = KP / 32;
     constexpr int NT = COUT / 16;
+    constexpr int J = kConvJ;
+    __shared__ bf16x8 wl[KS * NT * 64];
+    for (int f = threadIdx.x; f < KS * NT * 64; f += 256) {
+        const int l = f & 63, t = (f >> 6) % NT, s = (f >> 6) / NT;
+        const int co = t * 16 + (l & 15), k = s * 32 + 8 * (l >> 4);
+        wl[f] = k < 9 * CIN ? *reinterpret_cast<const bf16x8*>(Wt + size_t(co) * KP + k) : zero8();
+    }
+    __syncthreads();
     const int P = N * H * W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = (blockIdx.x * 4 + wave) * 32;
-    if (m0 >= P) return;  // whole wave out of range (no barriers below)
     const int r16 = lane & 15, kq = lane >> 4;
-    int pn[2], py[2], px[2];
-    bool pv[2];
+    const int tiles = (P + 16 * J - 1) / (16 * J);
+    // The lane's output channels are 16 t + r16: their biases, read once.
+    float bco[NT];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const int p = m0 + 16 * j + r16;
-        pv[j] = p < P;
-        const int pp = pv[j] ? p : 0;
-        pn[j] = pp / (H * W);
-        const int rem = pp - pn[j] * H * W;
-        py[j] = rem / W;
-        px[j] = rem - py[j] * W;
-    }
-    f32x4 acc[2][NT];
+    for (int t = 0; t < NT; ++t) bco[t] = MODE == 2 ? 0.0f : bias[16 * t + r16];
+    for (int tile = blockIdx.x * 4 + wave; tile < tiles; tile += gridDim.x * 4) {
+        const int m0 = tile * 16 * J;
+        // Per row tile: the lane's pixel and which of the 9 taps lie inside
+        // the image (bit 3 (dy + 1) + (dx + 1)); none for pixels beyond P.
+        const __bf16* base[J];
+        unsigned taps[J];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int ks = 0; ks < KP; ks += 32) {
-        const int k = ks + 8 * kq;
-        const int tap = k / CIN, ci = k - tap * CIN;
-        bf16x8 a[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            a[j] = zero8();
-            if (tap < 9 && pv[j]) {
-                const int yy = py[j] + tap / 3 - 1, xx = px[j] + tap % 3 - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    a[j] = *reinterpret_cast<const bf16x8*>(X + ((size_t(pn[j]) * H + yy) * W + xx) * CIN + ci);
+        for (int j = 0; j < J; ++j) {
+            const int p = m0 + 16 * j + r16;
+            base[j] = X;
+            taps[j] = 0;
+            if (p < P) {
+                const int rem = p % (H * W);
+                const int y = rem / W, x = rem - (rem / W) * W;
+                const unsigned xm = (x > 0 ? 1u : 0u) | 2u | (x < W - 1 ? 4u : 0u);
+                taps[j] = (y > 0 ? xm : 0u) | (xm << 3) | (y < H - 1 ? xm << 6 : 0u);
+                base[j] = X + size_t(p) * CIN;
             }
         }
+        f32x4 acc[J][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wt + size_t(t * 16 + r16) * KP + k);
-            acc[0][t] = mfma(a[0], b, acc[0][t]);
-            acc[1][t] = mfma(a[1], b, acc[1][t]);
+        for (int j = 0; j < J; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        // A fragments of k-step s: 8 channels of the tap-shifted pixel, zero
+        // outside the image and in the K padding (taps 9 and up).
+        auto load_a = [&](int s, bf16x8 (&a)[J]) {
+            const int k = s * 32 + 8 * kq;
+            const int tap = k / CIN, ci = k - tap * CIN;
+            const long off = long((tap / 3 - 1) * W + (tap % 3 - 1)) * CIN + ci;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                a[j] = zero8();
+                if ((taps[j] >> tap) & 1u) a[j] = *reinterpret_cast<const bf16x8*>(base[j] + off);
+            }
+        };
+        // kConvDepth k-steps of A fragments in flight ahead of the MFMAs.
+        bf16x8 a[kConvDepth + 1][J];
+#pragma unroll
+        for (int d = 0; d < kConvDepth; ++d)
+            if (d < KS) load_a(d, a[d]);
+#pragma unroll 1
+        for (int s = 0; s < KS; ++s) {
+            if (s + kConvDepth < KS) load_a(s + kConvDepth, a[kConvDepth]);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 b = wl[(s * NT + t) * 64 + lane];
+#pragma unroll
+                for (int j = 0; j < J; ++j) acc[j][t] = mfma(a[0][j], b, acc[j][t]);
+            }
+#pragma unroll
+            for (int d = 0; d < kConvDepth; ++d)
+#pragma unroll
+                for (int j = 0; j < J; ++j) a[d][j] = a[d + 1][j];
+        }
+        // acc[j][t][r] is output pixel m0 + 16 j + 4 kq + r, channel 16 t + r16.
+        const bool full = m0 + 16 * J <= P;
+        if constexpr (MODE == 2) {
+            // the activation mask of every output first (one bit each), then
+            // the stores, so the loads are all in flight together
+            static_assert(J * NT * 4 <= 64, "one mask bit per output of the lane");
+            __bf16 av[J][NT][4];
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = min(m0 + 16 * j + 4 * kq + r, P - 1);  // beyond P: not stored
+                        av[j][t][r] = act[size_t(m) * COUT + 16 * t + r16];
+                    }
+            unsigned long long on = 0;
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        on |= (unsigned long long)(float(av[j][t][r]) > 0.0f) << ((j * NT + t) * 4 + r);
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = m0 + 16 * j + 4 * kq + r;
+                        if (!full && m >= P) continue;
+                        static_cast<__bf16*>(out)[size_t(m) * COUT + 16 * t + r16] =
+                            (on >> ((j * NT + t) * 4 + r)) & 1ull ? (__bf16)acc[j][t][r] : (__bf16)0.0f;
+                    }
+        } else {
+#pragma unroll
+            for (int j = 0; j < J; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int m = m0 + 16 * j + 4 * kq + r;
+                        if (!full && m >= P) continue;
+                        const size_t o = size_t(m) * COUT + 16 * t + r16;
+                        const float v = acc[j][t][r] + bco[t];
+                        if (MODE == 0) static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v, 0.0f);
+                        else static_cast<float*>(out)[o] = v;
+                    }
         }
     }
-    // acc[j][t][r] is output pixel m0 + 16 j + 4 kq + r, channel 16 t + r16.
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = m0 + 16 * j + 4 * kq + r;
-                if (m >= P) continue;
-                const int co = 16 * t + r16;
-                const size_t o = size_t(m) * COUT + co;
-                const float v = acc[j][t][r];
-                if (MODE == 0) {
-                    static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v + bias[co], 0.0f);
-                } else if (MODE == 1) {
-                    static_cast<float*>(out)[o] = v + bias[co];
-                } else {
-                    static_cast<__bf16*>(out)[o] = float(act[o]) > 0.0f ? (__bf16)v : (__bf16)0.0f;
-                }
-            }
 }
 
 template <int CIN, int COUT>
@@ -388,7 +465,15 @@ unsigned blocks_of(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
 void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, const float* b, const __bf16* act,
                  void* out, int n, int h, int wdt, hipStream_t st) {
     const size_t P = size_t(n) * h * wdt;
-    const dim3 grid(blocks_of(P, 128)), block(256);
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const size_t tiles = (P + 16 * kConvJ - 1) / (16 * kConvJ);
+    const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * kConvBlocksPerCU))), block(256);
 #define RM_CONV(CI, CO, MO) \
     hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
     if (cin == 16 && cout == 64 && mode == 0) RM_CONV(16, 64, 0);
