@@ -303,7 +303,10 @@ def run_rmse(args) -> dict:
     reference of the same view accumulated in FP64: the Cornell box (4 spp
     megakernel), the mixed scene (frame 4 of the orbit after 4 frames of
     history, 16 spp ReSTIR + 5 a-trous passes), the animated height field
-    (frame 4 after 4 frames, 8 spp ReSTIR + 2 a-trous passes, GPU tree)."""
+    (frame 4 after 4 frames, 8 spp ReSTIR + 2 a-trous passes, GPU tree), the
+    inverse-rendering scene at its ground truth (512x512, 16 spp path tracing)
+    and the learned denoiser (mixed-scene frame 4, 4 spp ReSTIR + the network
+    with the committed weights)."""
     out = {}
     t0 = time.perf_counter()
     backend = rm.Backend(0)
@@ -336,6 +339,28 @@ def run_rmse(args) -> dict:
         _, img = backend.render(f, 12345 + k)
     ref, se = reference_image(backend, f, 700_000)
     out["heightfield_1080p_8spp_restir_atrous"] = image_errors(img, ref, se)
+    # inverse-rendering scene at its ground-truth parameters: the 16-spp
+    # path-tracing frame the loss is taken on
+    from raymotion_amd.diff import DiffScene, path_frame
+    s, groups = scenes.inverse_scene(seed=0)
+    ds = DiffScene(s.export(), groups)
+    backend.set_scene(ds.exported(ds.initial))
+    f = path_frame(512, 512, 16, 8, scenes.mixed_camera(0))
+    _, img = backend.render(f, 12345)
+    ref, se = reference_image(backend, f, 800_000)
+    out["inverse_512_16spp_path"] = image_errors(img, ref, se)
+    # learned denoiser at 4 spp (mixed-scene orbit frame 4 after 4 frames)
+    if WEIGHTS.exists():
+        from raymotion_amd import denoise
+        s = scenes.mixed_spheres(seed=0)
+        s.build_bvh()
+        backend.set_scene(s, gpu_tree=True, rebuild=True)
+        backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
+        for k in range(5):
+            f = denoise.restir_frame(1280, 720, 4, scenes.mixed_camera(k), denoise=2, temporal=0 if k == 0 else 1)
+            _, img = backend.render(f, 12345 + k)
+        ref, se = reference_image(backend, f, 900_000)
+        out["mixed_720p_4spp_restir_learned_denoiser"] = image_errors(img, ref, se)
     backend.close()
     return {"metric": "image RMSE against a 4096-spp FP64-accumulated path-traced reference", "value": None,
             "unit": "linear radiance RMSE", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": None,
