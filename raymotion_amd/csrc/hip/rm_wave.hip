@@ -869,11 +869,15 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
             // (Tracer::nee_prepare_coop: ceil(Mc * waiting / 64) candidate
             // evaluations per lane and an Mc-step fold, against Mc evaluations
             // whenever any lane samples alone). A lane's own sequence of
-            // operations and random draws is unchanged.
+            // operations and random draws is unchanged. Once the queue is
+            // exhausted the wave's remaining lanes are the long per-pixel
+            // chains that set the end of the pass, and they no longer wait:
+            // C1 shade 10.4-10.5 -> 10.2-10.4 ms, C2 7.7 -> 7.3 ms (the same
+            // for leaf batches made both slower).
             const unsigned long long wait = __ballot(L.phase == kNee);
             if (wait) {
                 const int nw = __popcll(wait), na = __popcll(act);
-                if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge) {
+                if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge || exhausted) {
                     nee_age = 0;
                     const bool waiting = L.phase == kNee;
                     const int r = pt.ris_coop_applies() ? eng.run_nee_batch(L, wait)
