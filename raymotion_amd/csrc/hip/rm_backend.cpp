@@ -115,6 +115,10 @@ struct rm_ctx {
     DevBuf shade, den_a, hist_color;  // float4 per pixel
     DevBuf rgb8, hdr3, mega_hdr;
     DevBuf counters;
+    // Pinned staging of each frame's read-back (counters, then the RGB8
+    // image), copied to the caller's buffers after the frame's one wait.
+    uint8_t* read_host = nullptr;
+    size_t read_cap = 0;
     DevBuf queue;  // kQueueWords work counters of the persistent passes (band counters, shade order counts)
     DevBuf shade_order;  // restir_shade work order, one pixel index per entry
     // Wavefront schedule: two slot-state buffers, hits, control words, and a
@@ -428,6 +432,7 @@ void rm_destroy(rm_ctx* c) {
     for (auto& e : c->wave_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->wave_host) (void)hipHostFree(c->wave_host);
+    if (c->read_host) (void)hipHostFree(c->read_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -690,6 +695,45 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
 
 namespace {
 
+// Frame read-back through the context's pinned staging buffer: the counters
+// at offset 0, the RGB8 image (rgb_bytes, may be 0) after them. A copy into
+// pageable memory would be staged by the runtime with a wait of its own.
+constexpr size_t kReadImageOffset = 256;
+static_assert(kNumCounters * sizeof(unsigned long long) <= kReadImageOffset, "counters fit before the image");
+
+int queue_readback(rm_ctx* c, size_t rgb_bytes, const unsigned long long* cnt) {
+    const size_t need = kReadImageOffset + rgb_bytes;
+    if (c->read_cap < need) {
+        // no copy into the old buffer may still be in flight (a frame that
+        // failed after queueing its read-back did not wait for it)
+        if (int rc = check(c, hipStreamSynchronize(c->stream), "read-back")) return rc;
+        if (c->read_host) (void)hipHostFree(c->read_host);
+        c->read_host = nullptr;
+        c->read_cap = 0;
+        void* h = nullptr;
+        if (hipHostMalloc(&h, need, hipHostMallocDefault) != hipSuccess) {
+            c->drop_history();
+            return c->fail(RM_ERR_ALLOC, "HIP pinned read-back allocation failed");
+        }
+        c->read_host = static_cast<uint8_t*>(h);
+        c->read_cap = need;
+    }
+    if (int rc = check(c, hipMemcpyAsync(c->read_host, cnt, kNumCounters * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, c->stream), "counters"))
+        return rc;
+    if (rgb_bytes)
+        if (int rc = check(c, hipMemcpyAsync(c->read_host + kReadImageOffset, c->rgb8.ptr, rgb_bytes,
+                                             hipMemcpyDeviceToHost, c->stream), "readback"))
+            return rc;
+    return RM_OK;
+}
+
+// After the frame's wait: the counters to host_cnt, the image to rgb8 (if given).
+void finish_readback(const rm_ctx* c, uint8_t* rgb8, size_t rgb_bytes, unsigned long long* host_cnt) {
+    std::memcpy(host_cnt, c->read_host, kNumCounters * sizeof(unsigned long long));
+    if (rgb8) std::memcpy(rgb8, c->read_host + kReadImageOffset, rgb_bytes);
+}
+
 // rm_render with the context locked; with feat_host / rad_host (ReSTIR frames
 // only) also the denoiser's input features and noisy radiance.
 int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, float* hdr, float* feat_host,
@@ -797,9 +841,7 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
         launch_tonemap(npx, final_img, c->rgb8.as<uint8_t>(), hdr ? c->hdr3.as<float>() : nullptr, c->stream);
         mark(6);
         if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
-        if (rgb8)
-            if (int rc = check(c, hipMemcpyAsync(rgb8, c->rgb8.ptr, px * 3, hipMemcpyDeviceToHost, c->stream), "readback"))
-                return rc;
+        if (int rc = queue_readback(c, rgb8 ? px * 3 : 0, cnt)) return rc;
         if (hdr)
             if (int rc = check(c, hipMemcpyAsync(hdr, c->hdr3.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                                  c->stream), "readback"))
@@ -812,10 +854,9 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
             if (int rc = check(c, hipMemcpyAsync(rad_host, c->kpn_rad.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                                  c->stream), "readback"))
                 return rc;
-        unsigned long long host_cnt[kNumCounters];
-        if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
-            return rc;
         if (int rc = check(c, hipStreamSynchronize(c->stream), "frame")) return rc;
+        unsigned long long host_cnt[kNumCounters];
+        finish_readback(c, rgb8, px * 3, host_cnt);
         if (host_cnt[kCntOverflow]) {
             c->drop_history();
             return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
@@ -850,17 +891,14 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
         mark(1);
         if (hdr) launch_tonemap(npx, c->mega_hdr.as<float4>(), c->rgb8.as<uint8_t>(), c->hdr3.as<float>(), c->stream);
         if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
-        if (rgb8)
-            if (int rc = check(c, hipMemcpyAsync(rgb8, c->rgb8.ptr, px * 3, hipMemcpyDeviceToHost, c->stream), "readback"))
-                return rc;
+        if (int rc = queue_readback(c, rgb8 ? px * 3 : 0, cnt)) return rc;
         if (hdr)
             if (int rc = check(c, hipMemcpyAsync(hdr, c->hdr3.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToHost,
                                                  c->stream), "readback"))
                 return rc;
-        unsigned long long host_cnt[kNumCounters];
-        if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
-            return rc;
         if (int rc = check(c, hipStreamSynchronize(c->stream), "frame")) return rc;
+        unsigned long long host_cnt[kNumCounters];
+        finish_readback(c, rgb8, px * 3, host_cnt);
         if (host_cnt[kCntOverflow]) {
             c->drop_history();
             return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
