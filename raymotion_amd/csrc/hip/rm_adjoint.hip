@@ -102,7 +102,9 @@ template <int S>
 __global__ __launch_bounds__(kBlock) void k_adjoint(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
                                                      unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
-    __shared__ double grad[kMaxGroups * 5];
+    // The block's gradient table, 5 per group, sized at launch to the scene's
+    // groups (640 B at 16 groups instead of 5 KiB for the largest count).
+    extern __shared__ double grad[];
     for (int i = threadIdx.x; i < A.ngroup * 5; i += kBlock) grad[i] = 0.0;
     __syncthreads();
     const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
@@ -152,7 +154,8 @@ void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams&
     const size_t threads = size_t(P.width) * size_t(P.height) * size_t(lanes);
     const dim3 grid(unsigned((threads + kBlock - 1) / kBlock)), block(kBlock);
     with_stack(stack, [&](auto cap) {
-        hipLaunchKernelGGL((k_adjoint<decltype(cap)::value>), grid, block, 0, st, sc, P, A, lanes, cnt);
+        hipLaunchKernelGGL((k_adjoint<decltype(cap)::value>), grid, block, size_t(A.ngroup) * 5 * sizeof(double), st,
+                           sc, P, A, lanes, cnt);
     });
 }
 

@@ -85,6 +85,11 @@ struct PathEngine {
     static constexpr int kMinBlocks = 2;
     static constexpr int kLogicBlocks = 2;  // wavefront logic kernel (no LDS)
     static constexpr bool kDeferNee = true;
+    // Lanes stop waiting for light-sampling batches once the queue is
+    // exhausted (k_persistent): the shade pass's tail is its long camera-path
+    // chains; the megakernel's lanes measured faster waiting (C3 forward
+    // 9.2 ms against 9.9 ms).
+    static constexpr bool kTailNoWait = SHADE;
     // Traversal inside the persistent loop with refill (k_persistent): not
     // kept here (C1 shade 10.1-10.2 ms against 9.9-10.0 ms with the call).
     static constexpr bool kFused = false;
@@ -869,15 +874,15 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
             // (Tracer::nee_prepare_coop: ceil(Mc * waiting / 64) candidate
             // evaluations per lane and an Mc-step fold, against Mc evaluations
             // whenever any lane samples alone). A lane's own sequence of
-            // operations and random draws is unchanged. Once the queue is
-            // exhausted the wave's remaining lanes are the long per-pixel
-            // chains that set the end of the pass, and they no longer wait:
-            // C1 shade 10.4-10.5 -> 10.2-10.4 ms, C2 7.7 -> 7.3 ms (the same
-            // for leaf batches made both slower).
+            // operations and random draws is unchanged. With E::kTailNoWait,
+            // once the queue is exhausted the wave's remaining lanes are the
+            // long per-pixel chains that set the end of the pass, and they no
+            // longer wait: C1 shade 10.4-10.5 -> 10.2-10.4 ms, C2 7.7 -> 7.3 ms
+            // (the same for leaf batches made both slower).
             const unsigned long long wait = __ballot(L.phase == kNee);
             if (wait) {
                 const int nw = __popcll(wait), na = __popcll(act);
-                if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge || exhausted) {
+                if (8 * nw >= kNeeFrac8 * na || ++nee_age >= kNeeMaxAge || (E::kTailNoWait && exhausted)) {
                     nee_age = 0;
                     const bool waiting = L.phase == kNee;
                     const int r = pt.ris_coop_applies() ? eng.run_nee_batch(L, wait)
