@@ -87,8 +87,7 @@ struct PathEngine {
     static constexpr bool kDeferNee = true;
     // Lanes stop waiting for light-sampling batches once the queue is
     // exhausted (k_persistent): the shade pass's tail is its long camera-path
-    // chains; the megakernel's lanes measured faster waiting (C3 forward
-    // 9.2 ms against 9.9 ms).
+    // chains. The megakernel keeps waiting (no difference measured there).
     static constexpr bool kTailNoWait = SHADE;
     // Traversal inside the persistent loop with refill (k_persistent): not
     // kept here (C1 shade 10.1-10.2 ms against 9.9-10.0 ms with the call).
