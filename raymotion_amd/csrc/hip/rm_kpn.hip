@@ -48,6 +48,37 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// Cross-lane shifts of A fragments within each row of 16 lanes (one lane per
+// pixel of a 16-pixel tile): next() gives lane l the fragment of lane l + 1 of
+// cur and lane 15 that of lane 0 of nxt (the pixel to the right); prev() lane
+// l that of lane l - 1 and lane 0 that of lane 15 of prv (the pixel to the
+// left). DPP row_shl:1 / row_shr:1, the row's end filled from a row_ror of
+// the neighbouring tile.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kDppRowShl1 = 0x101, kDppRowShr1 = 0x111, kDppRowRor1 = 0x121, kDppRowRor15 = 0x12F;
+
+__device__ __forceinline__ bf16x8 frag_next(bf16x8 cur, bf16x8 nxt) {
+    const i32x4 a = __builtin_bit_cast(i32x4, cur), b = __builtin_bit_cast(i32x4, nxt);
+    i32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int wrap = __builtin_amdgcn_mov_dpp(b[i], kDppRowRor15, 0xF, 0xF, false);
+        r[i] = __builtin_amdgcn_update_dpp(wrap, a[i], kDppRowShl1, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(bf16x8, r);
+}
+
+__device__ __forceinline__ bf16x8 frag_prev(bf16x8 cur, bf16x8 prv) {
+    const i32x4 a = __builtin_bit_cast(i32x4, cur), b = __builtin_bit_cast(i32x4, prv);
+    i32x4 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int wrap = __builtin_amdgcn_mov_dpp(b[i], kDppRowRor1, 0xF, 0xF, false);
+        r[i] = __builtin_amdgcn_update_dpp(wrap, a[i], kDppRowShr1, 0xF, 0xF, false);
+    }
+    return __builtin_bit_cast(bf16x8, r);
+}
+
 // Pixels per wave tile (kConvJ row tiles of 16), k-steps of A fragments
 // loaded ahead, and resident blocks per CU. Measured on the 1280x720 network
 // (ms for the 6 layers): J = 4 with 1 / 2 / 3 steps ahead 1.20 / 1.18 / 1.15
@@ -113,6 +144,68 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv(const __bf16* __
         for (int j = 0; j < J; ++j)
 #pragma unroll
             for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (CIN == 64) {
+            // k-step s = 2 tap + half. Per tap row dy the wave loads the centre
+            // column (dx = 0, both channel halves) of its J tiles and the one
+            // pixel on either side, and forms the dx = -1 / +1 fragments by
+            // cross-lane shifts: each input line is fetched once per tap row
+            // instead of three times. Steps still run in order s = 0 .. 17.
+            // centre columns of tap row dyi (both halves), next row in flight
+            // during this row's MFMAs
+            auto load_rows = [&](int dyi, bf16x8 (&c)[2][J + 2]) {
+                const long row = long(dyi - 1) * W * CIN;
+#pragma unroll
+                for (int jj = 0; jj < J + 2; ++jj) {
+                    bool ok;
+                    const __bf16* src;
+                    if (jj >= 1 && jj <= J) {
+                        ok = (taps[jj - 1] >> (3 * dyi + 1)) & 1u;
+                        src = base[jj - 1];
+                    } else {
+                        const int q = m0 + 16 * (jj - 1) + r16;
+                        ok = (jj == 0 ? r16 == 15 : r16 == 0) && q >= 0 && q < P;
+                        if (ok) {
+                            const int yq = (q % (H * W)) / W + dyi - 1;
+                            ok = yq >= 0 && yq < H;
+                        }
+                        src = X + size_t(ok ? q : 0) * CIN;
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        c[h][jj] = ok ? *reinterpret_cast<const bf16x8*>(src + row + 32 * h + 8 * kq) : zero8();
+                }
+            };
+            bf16x8 c[2][J + 2], cn[2][J + 2];  // [half][tile + 1]: tiles -1 and J hold one pixel each
+            load_rows(0, c);
+#pragma unroll 1
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                if (dyi < 2) load_rows(dyi + 1, cn);
+#pragma unroll
+                for (int dxi = 0; dxi < 3; ++dxi)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int tap = 3 * dyi + dxi, s = 2 * tap + h;
+                        bf16x8 a[J];
+#pragma unroll
+                        for (int j = 0; j < J; ++j) {
+                            const bf16x8 v = dxi == 1 ? c[h][j + 1]
+                                             : dxi == 2 ? frag_next(c[h][j + 1], c[h][j + 2])
+                                                        : frag_prev(c[h][j + 1], c[h][j]);
+                            a[j] = (taps[j] >> tap) & 1u ? v : zero8();
+                        }
+#pragma unroll
+                        for (int t = 0; t < NT; ++t) {
+                            const bf16x8 b = wl[(s * NT + t) * 64 + lane];
+#pragma unroll
+                            for (int j = 0; j < J; ++j) acc[j][t] = mfma(a[j], b, acc[j][t]);
+                        }
+                    }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int jj = 0; jj < J + 2; ++jj) c[h][jj] = cn[h][jj];
+            }
+        } else {
         // A fragments of k-step s: 8 channels of the tap-shifted pixel, zero
         // outside the image and in the K padding (taps 9 and up).
         auto load_a = [&](int s, bf16x8 (&a)[J]) {
@@ -143,6 +236,7 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv(const __bf16* __
             for (int d = 0; d < kConvDepth; ++d)
 #pragma unroll
                 for (int j = 0; j < J; ++j) a[d][j] = a[d + 1][j];
+        }
         }
         // acc[j][t][r] is output pixel m0 + 16 j + 4 kq + r, channel 16 t + r16.
         const bool full = m0 + 16 * J <= P;
