@@ -359,9 +359,10 @@ struct Tracer {
     // eighths of the traversing lanes wait (so also when none can go on) or
     // they have waited kLeafMaxAge iterations (age is the wave's counter).
     static __device__ __forceinline__ bool leaf_batch_now(unsigned long long wait, unsigned long long trav,
-                                                          int& age, int max_age = kLeafMaxAge) {
+                                                          int& age, int max_age = kLeafMaxAge,
+                                                          int frac8 = kLeafFrac8) {
         if (!wait) return false;
-        if (8 * __popcll(wait) >= kLeafFrac8 * __popcll(trav) || ++age >= max_age) {
+        if (8 * __popcll(wait) >= frac8 * __popcll(trav) || ++age >= max_age) {
             age = 0;
             return true;
         }

@@ -586,10 +586,12 @@ struct GenEngine {
     static constexpr int kMinBlocks = S <= 32 ? 4 : S <= 48 ? 3 : 2;
     static constexpr bool kDeferNee = false;
     static constexpr bool kFused = true;  // traversal inside the persistent loop, with refill
-    // Iterations a lane with hit leaf slots waits for a leaf batch in that
-    // loop: 5 here (C1 generate 3.85-3.90 ms against 3.97-4.02 at 3; the
-    // call-based traversal of the other passes keeps 3).
+    // Leaf batches in that loop: when half the traversing lanes wait or
+    // after 5 iterations (C1 generate 3.84-3.87 ms against 3.97-4.02 at 3;
+    // 6/8 with 5: 3.93, 4/8 with 8: 3.84-3.89; the call-based traversal of
+    // the other passes keeps 3).
     static constexpr int kLeafMaxAge = 5;
+    static constexpr int kLeafFrac8 = 4;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                 do {
                     if (trav && !T.pend) pt.trav_node(T);
                     if (Tracer<S>::leaf_batch_now(__ballot(trav && T.pend), __ballot(trav), leaf_age,
-                                                  E::kLeafMaxAge) && trav &&
+                                                  E::kLeafMaxAge, E::kLeafFrac8) && trav &&
                         T.pend && pt.trav_leaves(T)) {
                         trav = false;
                         ready = true;
