@@ -80,9 +80,11 @@ __device__ __forceinline__ bf16x8 frag_prev(bf16x8 cur, bf16x8 prv) {
 }
 
 // Pixels per wave tile (kConvJ row tiles of 16), k-steps of A fragments
-// loaded ahead, and resident blocks per CU. Measured on the 1280x720 network
-// (ms for the 6 layers): J = 4 with 1 / 2 / 3 steps ahead 1.20 / 1.18 / 1.15
-// (before the epilogue fix), J = 2 1.33; J = 8 spills.
+// loaded ahead (layers with fewer than 64 input channels; those with 64 keep
+// one tap row ahead), and resident blocks per CU. Measured on the 1280x720
+// network (ms for the 6 layers, all on the per-step path then): J = 4 with
+// 1 / 2 / 3 steps ahead 1.20 / 1.18 / 1.15 (before the epilogue fix), J = 2
+// 1.33; J = 8 spills.
 constexpr int kConvJ = 4;
 constexpr int kConvDepth = 2;
 constexpr int kConvBlocksPerCU = 2;
@@ -90,9 +92,10 @@ constexpr int kConvBlocksPerCU = 2;
 // A block stages the layer's whole weight matrix in LDS once, already in the
 // MFMA B-fragment order (k-step s, column tile t, lane: 16 bytes each, so a
 // wave's fragment read is one contiguous 1 KiB line), then its 4 waves loop
-// over tiles of 16 * kConvJ pixels x COUT channels. Per k-step a wave loads
-// kConvJ A fragments (8 channels of one shifted pixel per lane, from global
-// memory through L1) and reads COUT / 16 B fragments from LDS, feeding
+// over tiles of 16 * kConvJ pixels x COUT channels. Per k-step a wave takes
+// kConvJ A fragments (8 channels of one shifted pixel per lane, loaded from
+// global memory through L1, or for 64 input channels shifted across lanes from
+// the tap row's centre column) and reads COUT / 16 B fragments from LDS, feeding
 // kConvJ * COUT / 16 MFMAs. Each output's sum runs over the k-steps in order,
 // as a plain GEMM loop would. At most 72 KiB of LDS per block: 2 blocks per CU.
 template <int CIN, int COUT, int MODE>
