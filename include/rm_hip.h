@@ -165,6 +165,32 @@ int rm_trace(rm_ctx* ctx, const float* origins, const float* dirs, const float* 
  * before denoising (W*H*3). Any pointer may be NULL. */
 int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade);
 
+/* The ReSTIR merge pass alone (restir_spatial, pathtrace.metal:919-1054) over
+ * caller-supplied G-buffer and initial reservoirs (W*H each, the layouts
+ * rm_read_restir returns), with shadow rays against the resident scene. Reads
+ * the frame's width, height, camera, restir_spatial, restir_radius and
+ * restir_mcap; seed is the frame seed of the spatial pass's streams. When
+ * hist_gbuf, hist_res and prev_camera are all given, the temporal reuse of
+ * pathtrace.metal:956-1002 runs first against them; all three NULL means no
+ * history. rmerged receives W*H reservoirs. The context's rendering history
+ * is not touched. No separate reference entry point: the reference runs this
+ * pass only inside render_image_metal (metal.mm:299-310). */
+int rm_restir_merge(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, const rm_gbuffer* gbuf,
+                    const rm_reservoir* rinit, const rm_gbuffer* hist_gbuf, const rm_reservoir* hist_res,
+                    const rm_camera* prev_camera, rm_reservoir* rmerged);
+
+/* The a-trous denoiser alone (denoise_temporal, then frame->denoise_iterations
+ * denoise_atrous passes; pathtrace.metal:1263-1354) over a caller-supplied
+ * linear image (W*H*3) and its G-buffer. Reads the frame's width, height,
+ * camera and denoise_iterations (0..30). With hist_color, hist_gbuf and
+ * prev_camera all given the temporal pass blends that history; all three NULL
+ * passes the image through it unchanged. out receives W*H*3 linear values
+ * (before tonemapping). No scene is needed and the context's rendering history
+ * is not touched. The reference runs these passes only inside
+ * render_image_metal (metal.mm:321-366). */
+int rm_denoise_image(rm_ctx* ctx, const rm_frame* frame, const float* color, const rm_gbuffer* gbuf,
+                     const float* hist_color, const rm_gbuffer* hist_gbuf, const rm_camera* prev_camera, float* out);
+
 /* Path-replay adjoint of the path-tracing frame (the megakernel path of
  * rm_render: no ReSTIR, legacy light sampling, no adaptive stop) with the
  * same frame and seed: the samples rm_render averaged are re-traced from

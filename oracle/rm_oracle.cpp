@@ -1700,6 +1700,65 @@ int orc_render_frame(const orc_scene* sc, const orc_frame* f, uint64_t seed, voi
     return 0;
 }
 
+// restir_spatial alone (pathtrace.metal:919-1054) over caller-supplied
+// G-buffer and initial reservoirs (W*H each); temporal reuse from hist_g /
+// hist_r through prev_camera when all three are given. Shadow rays are traced
+// against the scene. rout receives W*H merged reservoirs. Returns 0 or -1.
+int orc_restir_merge(const orc_scene* sc, const orc_frame* f, uint64_t seed, const orc_gbuffer* gbuf,
+                     const orc_reservoir* rinit, const orc_gbuffer* hist_g, const orc_reservoir* hist_r,
+                     const orc_camera* prev_camera, int threads, orc_reservoir* rout) {
+    using namespace orc;
+    if (f->width <= 0 || f->height <= 0 || !gbuf || !rinit || !rout) return -1;
+    const bool hist = hist_g && hist_r && prev_camera;
+    orc_params P{};
+    P.camera = f->camera;
+    P.width = f->width; P.height = f->height; P.spp = f->spp; P.bounces = f->bounces;
+    P.nodes = int32_t(sc->nnode); P.lights = int32_t(sc->nlight);
+    P.nomis = f->nomis;
+    P.opaque = 1;
+    for (size_t i = 0; i < sc->ntri; ++i)
+        if (!(sc->tris[i].alpha >= 1)) { P.opaque = 0; break; }
+    P.light_total = sc->light_total;
+    P.resolution = f->resolution; P.clamp_value = f->clamp_value;
+    P.restir_candidates = f->restir_candidates; P.restir_spatial = f->restir_spatial;
+    P.restir_mcap = f->restir_mcap; P.restir_radius = f->restir_radius;
+    P.has_history = hist ? 1 : 0;
+    if (hist) P.prev_camera = *prev_camera;
+    Frame fr{sc, P, seed, nullptr};
+    const uint32_t px = uint32_t(f->width) * uint32_t(f->height);
+    parallel_for(px, threads, [&](uint32_t i) {
+        restir_spatial(fr, i, nullptr, gbuf, rinit, rout, hist ? hist_g : gbuf, hist ? hist_r : rinit);
+    });
+    return 0;
+}
+
+// The denoiser alone: denoise_temporal, then f->denoise_iterations a-trous
+// passes (pathtrace.metal:1263-1354) over a linear image (W*H*3) and its
+// G-buffer; the temporal pass blends hist_color (seen through prev_camera,
+// with hist_g) when all three are given. out: W*H*3. Returns 0 or -1.
+int orc_denoise_image(const orc_frame* f, const float* color, const orc_gbuffer* gbuf, const float* hist_color,
+                      const orc_gbuffer* hist_g, const orc_camera* prev_camera, int threads, float* out) {
+    using namespace orc;
+    if (f->width <= 0 || f->height <= 0 || !color || !gbuf || !out) return -1;
+    const bool hist = hist_color && hist_g && prev_camera;
+    orc_params P{};
+    P.camera = f->camera;
+    P.width = f->width; P.height = f->height;
+    P.has_history = hist ? 1 : 0;
+    if (hist) P.prev_camera = *prev_camera;
+    const uint32_t px = uint32_t(f->width) * uint32_t(f->height);
+    std::vector<float> a(size_t(px) * 3), b(size_t(px) * 3);
+    parallel_for(px, threads, [&](uint32_t i) {
+        denoise_temporal(P, i, gbuf, hist ? hist_g : gbuf, color, hist ? hist_color : color, a.data());
+    });
+    for (int s = 0; s < std::max(0, f->denoise_iterations); ++s) {
+        parallel_for(px, threads, [&](uint32_t i) { denoise_atrous(P, i, s, gbuf, a.data(), b.data()); });
+        a.swap(b);
+    }
+    std::memcpy(out, a.data(), a.size() * sizeof(float));
+    return 0;
+}
+
 // Plain megakernel render (render_paths semantics, no adaptivity, lanes=1) in
 // FP64 with FP64 accumulation; hdr receives the per-pixel mean (W*H*3).
 int orc_render_reference_f64(const orc_scene* sc, const orc_frame* f, uint64_t seed, int threads, double* hdr) {

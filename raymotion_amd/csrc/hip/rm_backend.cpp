@@ -128,6 +128,12 @@ struct rm_ctx {
     hipEvent_t wave_ev[2] = {};
     size_t wave_cap = 0;
     DevBuf probe_o, probe_d, probe_t, probe_tri, probe_tuv;
+    // Standalone merge / denoise passes on caller data (rm_restir_merge,
+    // rm_denoise_image); separate from the frame buffers so the rendering
+    // history is left as it was.
+    GBufSet x_gbuf, x_hgbuf;
+    ResSet x_rinit, x_rout, x_hres;
+    DevBuf x_col, x_hcol, x_a, x_b;
     DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
     std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
 
@@ -420,6 +426,15 @@ void rm_destroy(rm_ctx* c) {
     c->rinit.release();
     c->rmerged.release();
     c->hist_res.release();
+    c->x_gbuf.release();
+    c->x_hgbuf.release();
+    c->x_rinit.release();
+    c->x_rout.release();
+    c->x_hres.release();
+    c->x_col.release();
+    c->x_hcol.release();
+    c->x_a.release();
+    c->x_b.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     c->geo_scene.release();
@@ -929,6 +944,64 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
     return RM_OK;
 }
 
+// Reference-layout records (76-B G-buffer, 60-B reservoir) to and from the
+// device's SoA float4 arrays (GBufferSoA, ReservoirSoA).
+void gbuf_to_soa(const rm_gbuffer* g, size_t px, std::vector<float4> (&o)[5]) {
+    for (auto& v : o) v.resize(px);
+    for (size_t i = 0; i < px; ++i) {
+        float kind;
+        std::memcpy(&kind, &g[i].kind, 4);
+        o[0][i] = make_float4(g[i].pos.x, g[i].pos.y, g[i].pos.z, kind);
+        o[1][i] = make_float4(g[i].normal.x, g[i].normal.y, g[i].normal.z, g[i].rough);
+        o[2][i] = make_float4(g[i].albedo.x, g[i].albedo.y, g[i].albedo.z, g[i].metallic);
+        o[3][i] = make_float4(g[i].wo.x, g[i].wo.y, g[i].wo.z, g[i].ior);
+        o[4][i] = make_float4(g[i].emission.x, g[i].emission.y, g[i].emission.z, 0.0f);
+    }
+}
+
+void res_to_soa(const rm_reservoir* r, size_t px, std::vector<float4> (&o)[4]) {
+    for (auto& v : o) v.resize(px);
+    for (size_t i = 0; i < px; ++i) {
+        float m, valid;
+        std::memcpy(&m, &r[i].m, 4);
+        std::memcpy(&valid, &r[i].valid, 4);
+        o[0][i] = make_float4(r[i].light_pos.x, r[i].light_pos.y, r[i].light_pos.z, r[i].p_hat);
+        o[1][i] = make_float4(r[i].light_normal.x, r[i].light_normal.y, r[i].light_normal.z, r[i].alpha);
+        o[2][i] = make_float4(r[i].emission.x, r[i].emission.y, r[i].emission.z, r[i].sum_ws);
+        o[3][i] = make_float4(r[i].sum_wu, m, valid, 0.0f);
+    }
+}
+
+void res_from_soa(const std::vector<float4> (&a)[4], size_t px, rm_reservoir* out) {
+    for (size_t i = 0; i < px; ++i) {
+        rm_reservoir& o = out[i];
+        o.light_pos = {a[0][i].x, a[0][i].y, a[0][i].z};
+        o.p_hat = a[0][i].w;
+        o.light_normal = {a[1][i].x, a[1][i].y, a[1][i].z};
+        o.alpha = a[1][i].w;
+        o.emission = {a[2][i].x, a[2][i].y, a[2][i].z};
+        o.sum_ws = a[2][i].w;
+        o.sum_wu = a[3][i].x;
+        std::memcpy(&o.m, &a[3][i].y, 4);
+        std::memcpy(&o.valid, &a[3][i].z, 4);
+    }
+}
+
+// Queues host-to-device copies of SoA arrays into a set's buffers.
+template <size_t N>
+bool upload_soa(rm_ctx* c, DevBuf* bufs, const std::vector<float4> (&v)[N]) {
+    for (size_t k = 0; k < N; ++k)
+        if (!upload(c, bufs[k], v[k])) return false;
+    return true;
+}
+
+// Linear RGB triples to float4 per pixel.
+std::vector<float4> rgb_to_float4(const float* rgb, size_t px) {
+    std::vector<float4> v(px);
+    for (size_t i = 0; i < px; ++i) v[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.0f);
+    return v;
+}
+
 // Grow-only buffer that keeps its contents (the training set).
 bool grow_keep(DevBuf& b, size_t bytes, hipStream_t st) {
     if (bytes <= b.cap) return true;
@@ -1323,6 +1396,116 @@ int rm_read_restir(rm_ctx* c, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoi
             shade[3 * i + 2] = a[i].z;
         }
     }
+    return RM_OK;
+}
+
+int rm_restir_merge(rm_ctx* c, const rm_frame* f, uint64_t seed, const rm_gbuffer* gbuf, const rm_reservoir* rinit,
+                    const rm_gbuffer* hist_gbuf, const rm_reservoir* hist_res, const rm_camera* prev_camera,
+                    rm_reservoir* rmerged) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!f || !gbuf || !rinit || !rmerged || f->width <= 0 || f->height <= 0 ||
+        size_t(f->width) * size_t(f->height) > 100000000)
+        return c->fail(RM_ERR_INVALID, "rm_restir_merge: invalid arguments");
+    const bool hist = hist_gbuf || hist_res || prev_camera;
+    if (hist && !(hist_gbuf && hist_res && prev_camera))
+        return c->fail(RM_ERR_INVALID, "rm_restir_merge: history needs hist_gbuf, hist_res and prev_camera");
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    const size_t px = size_t(f->width) * size_t(f->height);
+    FrameParams P = frame_params(*f, seed);
+    P.has_history = hist ? 1 : 0;
+    if (hist) P.prev = *prev_camera;
+    if (!c->x_gbuf.ensure(px) || !c->x_rinit.ensure(px) || !c->x_rout.ensure(px) ||
+        (hist && (!c->x_hgbuf.ensure(px) || !c->x_hres.ensure(px))))
+        return c->fail(RM_ERR_ALLOC, "HIP ReSTIR buffer allocation failed");
+    std::vector<float4> g[5], r[4], hg[5], hr[4];
+    gbuf_to_soa(gbuf, px, g);
+    res_to_soa(rinit, px, r);
+    if (hist) {
+        gbuf_to_soa(hist_gbuf, px, hg);
+        res_to_soa(hist_res, px, hr);
+    }
+    if (!upload_soa(c, c->x_gbuf.p, g) || !upload_soa(c, c->x_rinit.r, r) ||
+        (hist && (!upload_soa(c, c->x_hgbuf.p, hg) || !upload_soa(c, c->x_hres.r, hr))))
+        return c->fail(RM_ERR_DEVICE, "rm_restir_merge: upload failed");
+    unsigned long long* cnt = c->counters.as<unsigned long long>();
+    if (int rc = check(c, hipMemsetAsync(cnt, 0, kNumCounters * sizeof(unsigned long long), c->stream), "counters"))
+        return rc;
+    // Without history the kernel never reads the history arrays (has_history 0).
+    const GBufferSoA hgs = hist ? c->x_hgbuf.soa() : c->x_gbuf.soa();
+    const ReservoirSoA hrs = hist ? c->x_hres.soa() : c->x_rinit.soa();
+    launch_spatial(c->stack_need, c->dev_scene(), P, c->x_gbuf.soa(), c->x_rinit.soa(), c->x_rout.soa(), hgs, hrs,
+                   cnt, c->stream);
+    if (int rc = check(c, hipGetLastError(), "restir_spatial")) return rc;
+    std::vector<float4> o[4];
+    for (int k = 0; k < 4; ++k) {
+        o[k].resize(px);
+        if (int rc = check(c, hipMemcpyAsync(o[k].data(), c->x_rout.r[k].ptr, px * sizeof(float4),
+                                             hipMemcpyDeviceToHost, c->stream), "readback"))
+            return rc;
+    }
+    unsigned long long host_cnt[kNumCounters];
+    if (int rc = check(c, hipMemcpyAsync(host_cnt, cnt, sizeof(host_cnt), hipMemcpyDeviceToHost, c->stream), "counters"))
+        return rc;
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "restir_spatial")) return rc;
+    if (host_cnt[kCntOverflow]) return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+    res_from_soa(o, px, rmerged);
+    c->err.clear();
+    return RM_OK;
+}
+
+int rm_denoise_image(rm_ctx* c, const rm_frame* f, const float* color, const rm_gbuffer* gbuf, const float* hist_color,
+                     const rm_gbuffer* hist_gbuf, const rm_camera* prev_camera, float* out) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!f || !color || !gbuf || !out || f->width <= 0 || f->height <= 0 ||
+        size_t(f->width) * size_t(f->height) > 100000000 || f->denoise_iterations < 0 || f->denoise_iterations > 30)
+        return c->fail(RM_ERR_INVALID, "rm_denoise_image: invalid arguments");
+    const bool hist = hist_color || hist_gbuf || prev_camera;
+    if (hist && !(hist_color && hist_gbuf && prev_camera))
+        return c->fail(RM_ERR_INVALID, "rm_denoise_image: history needs hist_color, hist_gbuf and prev_camera");
+    const size_t px = size_t(f->width) * size_t(f->height);
+    FrameParams P = frame_params(*f, 0);
+    P.has_history = hist ? 1 : 0;
+    if (hist) P.prev = *prev_camera;
+    if (!c->x_gbuf.ensure(px) || !c->x_col.ensure(px * sizeof(float4)) || !c->x_a.ensure(px * sizeof(float4)) ||
+        !c->x_b.ensure(px * sizeof(float4)) ||
+        (hist && (!c->x_hgbuf.ensure(px) || !c->x_hcol.ensure(px * sizeof(float4)))))
+        return c->fail(RM_ERR_ALLOC, "HIP denoiser buffer allocation failed");
+    std::vector<float4> g[5], hg[5];
+    gbuf_to_soa(gbuf, px, g);
+    const std::vector<float4> col = rgb_to_float4(color, px);
+    std::vector<float4> hcol;
+    if (hist) {
+        gbuf_to_soa(hist_gbuf, px, hg);
+        hcol = rgb_to_float4(hist_color, px);
+    }
+    if (!upload_soa(c, c->x_gbuf.p, g) || !upload(c, c->x_col, col) ||
+        (hist && (!upload_soa(c, c->x_hgbuf.p, hg) || !upload(c, c->x_hcol, hcol))))
+        return c->fail(RM_ERR_DEVICE, "rm_denoise_image: upload failed");
+    float4* a = c->x_a.as<float4>();
+    float4* b = c->x_b.as<float4>();
+    const GBufferSoA gs = c->x_gbuf.soa();
+    launch_temporal(P, gs, hist ? c->x_hgbuf.soa() : gs, c->x_col.as<float4>(),
+                    hist ? c->x_hcol.as<float4>() : c->x_col.as<float4>(), a, c->stream);
+    for (int s = 0; s < f->denoise_iterations; ++s) {
+        launch_atrous(P, gs, a, b, s, c->stream);
+        std::swap(a, b);
+    }
+    if (int rc = check(c, hipGetLastError(), "denoise")) return rc;
+    std::vector<float4> res(px);
+    if (int rc = check(c, hipMemcpyAsync(res.data(), a, px * sizeof(float4), hipMemcpyDeviceToHost, c->stream),
+                       "readback"))
+        return rc;
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "denoise")) return rc;
+    for (size_t i = 0; i < px; ++i) {
+        out[3 * i] = res[i].x;
+        out[3 * i + 1] = res[i].y;
+        out[3 * i + 2] = res[i].z;
+    }
+    c->err.clear();
     return RM_OK;
 }
 

@@ -280,6 +280,41 @@ class Backend:
         self._check(lib().rm_read_restir(self._h, ptr(g), ptr(ri), ptr(rm), ptr(shade)), "rm_read_restir")
         return g, ri, rm, shade
 
+    def restir_merge(self, frame: Frame, seed: int, gbuf, rinit, hist_gbuf=None, hist_res=None,
+                     prev_camera: Camera | None = None) -> np.ndarray:
+        """The ReSTIR merge pass alone (restir_spatial) over given G-buffer and
+        reservoirs (W*H records each); history arrays and camera optional."""
+        px = frame.width * frame.height
+        g = np.ascontiguousarray(gbuf, GBUFFER_DTYPE)
+        ri = np.ascontiguousarray(rinit, RESERVOIR_DTYPE)
+        hg = None if hist_gbuf is None else np.ascontiguousarray(hist_gbuf, GBUFFER_DTYPE)
+        hr = None if hist_res is None else np.ascontiguousarray(hist_res, RESERVOIR_DTYPE)
+        if g.size != px or ri.size != px or (hg is not None and hg.size != px) or (hr is not None and hr.size != px):
+            raise ValueError("G-buffer and reservoir arrays must hold width * height records")
+        out = np.zeros(px, RESERVOIR_DTYPE)
+        cam = None if prev_camera is None else ctypes.byref(prev_camera)
+        self._check(lib().rm_restir_merge(self._h, ctypes.byref(frame), seed, ptr(g), ptr(ri), ptr(hg), ptr(hr), cam,
+                                          ptr(out)), "rm_restir_merge")
+        return out
+
+    def denoise_image(self, frame: Frame, color, gbuf, hist_color=None, hist_gbuf=None,
+                      prev_camera: Camera | None = None) -> np.ndarray:
+        """Temporal + frame.denoise_iterations a-trous passes over a linear image
+        (W*H*3) and its G-buffer; history optional. Returns W*H*3 values."""
+        px = frame.width * frame.height
+        col = np.ascontiguousarray(color, np.float32).reshape(-1)
+        g = np.ascontiguousarray(gbuf, GBUFFER_DTYPE)
+        hc = None if hist_color is None else np.ascontiguousarray(hist_color, np.float32).reshape(-1)
+        hg = None if hist_gbuf is None else np.ascontiguousarray(hist_gbuf, GBUFFER_DTYPE)
+        if col.size != 3 * px or g.size != px or (hc is not None and hc.size != 3 * px) or \
+                (hg is not None and hg.size != px):
+            raise ValueError("colour must hold width * height * 3 values and the G-buffer width * height records")
+        out = np.zeros(3 * px, np.float32)
+        cam = None if prev_camera is None else ctypes.byref(prev_camera)
+        self._check(lib().rm_denoise_image(self._h, ctypes.byref(frame), ptr(col), ptr(g), ptr(hc), ptr(hg), cam,
+                                           ptr(out)), "rm_denoise_image")
+        return out
+
     def set_profiling(self, on: bool):
         self._check(lib().rm_set_profiling(self._h, int(on)), "rm_set_profiling")
 

@@ -51,6 +51,9 @@ def orc() -> ctypes.CDLL:
                                                        ctypes.c_int, _P]),
             ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
+            ("orc_restir_merge", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame), ctypes.c_uint64,
+                                                _P, _P, _P, _P, _P, ctypes.c_int, _P]),
+            ("orc_denoise_image", ctypes.c_int, [ctypes.POINTER(Frame), _P, _P, _P, _P, _P, ctypes.c_int, _P]),
         ]
         for name, res, args in sigs:
             fn = getattr(lib, name)
@@ -126,6 +129,38 @@ def render_frame(exported, frame: Frame, seed: int, history: History | None = No
     info.update(path=path.value, lanes=lanes.value, has_history=hh.value, rays=int(stats[0]),
                 node_visits=int(stats[1]), tri_tests=int(stats[2]))
     return rgb, hdr, info
+
+
+def restir_merge(exported, frame: Frame, seed: int, gbuf, rinit, hist_gbuf=None, hist_res=None, prev_camera=None,
+                 threads: int = THREADS) -> np.ndarray:
+    """The oracle's restir_spatial alone over given G-buffer and reservoirs."""
+    d = exported.desc()
+    g = np.ascontiguousarray(gbuf, GBUFFER_DTYPE)
+    ri = np.ascontiguousarray(rinit, RESERVOIR_DTYPE)
+    hg = None if hist_gbuf is None else np.ascontiguousarray(hist_gbuf, GBUFFER_DTYPE)
+    hr = None if hist_res is None else np.ascontiguousarray(hist_res, RESERVOIR_DTYPE)
+    out = np.zeros(frame.width * frame.height, RESERVOIR_DTYPE)
+    cam = None if prev_camera is None else ctypes.addressof(prev_camera)
+    rc = orc().orc_restir_merge(ctypes.byref(d), ctypes.byref(frame), seed, ptr(g), ptr(ri), ptr(hg), ptr(hr), cam,
+                                threads, ptr(out))
+    if rc:
+        raise RuntimeError("orc_restir_merge failed")
+    return out
+
+
+def denoise_image(frame: Frame, color, gbuf, hist_color=None, hist_gbuf=None, prev_camera=None,
+                  threads: int = THREADS) -> np.ndarray:
+    """The oracle's temporal + a-trous passes alone (W*H*3 out)."""
+    col = np.ascontiguousarray(color, np.float32).reshape(-1)
+    g = np.ascontiguousarray(gbuf, GBUFFER_DTYPE)
+    hc = None if hist_color is None else np.ascontiguousarray(hist_color, np.float32).reshape(-1)
+    hg = None if hist_gbuf is None else np.ascontiguousarray(hist_gbuf, GBUFFER_DTYPE)
+    out = np.zeros(col.size, np.float32)
+    cam = None if prev_camera is None else ctypes.addressof(prev_camera)
+    rc = orc().orc_denoise_image(ctypes.byref(frame), ptr(col), ptr(g), ptr(hc), ptr(hg), cam, threads, ptr(out))
+    if rc:
+        raise RuntimeError("orc_denoise_image failed")
+    return out
 
 
 def render_reference_f64(exported, frame: Frame, seed: int, threads: int = THREADS) -> np.ndarray:
