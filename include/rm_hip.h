@@ -165,6 +165,19 @@ int rm_trace(rm_ctx* ctx, const float* origins, const float* dirs, const float* 
  * before denoising (W*H*3). Any pointer may be NULL. */
 int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade);
 
+/* rm_trace on device memory, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory during the call
+ * (a stream being captured into a graph, a serving loop, a harness that
+ * forbids allocation). origins/dirs: 3*n floats, tmax: n floats or NULL, tri:
+ * n ints, tuv: 4*n floats (t, u, v, 0 per ray, 16-byte aligned) -- all device
+ * memory of the context's GPU, allocated by the caller. At most 2^30 rays per
+ * call. The traversal counters live in a fixed 256-byte array in device
+ * memory, one per GPU, shared by every call on that GPU: calls that overlap on
+ * one GPU race. Synchronous like the other entry points; rm_get_stats reports
+ * the call's counters. */
+int rm_trace_device(rm_ctx* ctx, const float* origins, const float* dirs, const float* tmax, size_t n, int any,
+                    int32_t* tri, float* tuv);
+
 /* The ReSTIR merge pass alone (restir_spatial, pathtrace.metal:919-1054) over
  * caller-supplied G-buffer and initial reservoirs (W*H each, the layouts
  * rm_read_restir returns), with shadow rays against the resident scene. Reads

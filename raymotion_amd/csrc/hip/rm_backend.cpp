@@ -19,12 +19,26 @@
 #include "rm_layout.h"
 #include "rm_pack.h"
 
+// Work space of rm_trace_device, the entry point that allocates nothing: the
+// traversal counters of one call (kNumCounters words, at most kScratchWords =
+// 256 bytes). Each GPU holds its own copy until the process ends; every
+// rm_trace_device call on a GPU uses that copy, so calls that overlap on one
+// GPU race. Host code reaches it only through the address hipGetSymbolAddress
+// returns for the device (rm_ctx::scratch_dev), never through the name.
+constexpr int kScratchWords = 32;
+constexpr size_t kTraceDeviceMaxRays = size_t(1) << 30;
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+static_assert(rmh::kNumCounters <= kScratchWords, "traversal counters fit the scratch array");
+
 namespace {
 
 using namespace rmh;
 
 std::mutex g_create_mu;
 std::string g_create_error;
+// Device address of `scratch` per HIP device, looked up once per device
+// (under g_create_mu) with that device current.
+std::vector<void*> g_scratch_dev;
 
 // Grow-only device allocation (the reference's upload/alloc helpers,
 // metal.mm:171-178,256-260).
@@ -155,6 +169,10 @@ struct rm_ctx {
     bool hist_valid = false;
 
     rm_stats stats{};
+
+    // rm_trace_device: the device's scratch array and a pinned copy of its counters.
+    unsigned long long* scratch_dev = nullptr;
+    unsigned long long* scratch_host = nullptr;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -400,6 +418,27 @@ int rm_create(rm_ctx** out, int hip_device) {
         g_create_error = "HIP counter allocation failed";
         return RM_ERR_ALLOC;
     }
+    // The scratch array's address on this device (looked up once per device),
+    // and pinned host memory for its read-back.
+    if (g_scratch_dev.size() < size_t(count)) g_scratch_dev.resize(size_t(count), nullptr);
+    if (!g_scratch_dev[size_t(hip_device)]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            rm_destroy(c);
+            g_create_error = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return RM_ERR_DEVICE;
+        }
+        g_scratch_dev[size_t(hip_device)] = p;
+    }
+    c->scratch_dev = static_cast<unsigned long long*>(g_scratch_dev[size_t(hip_device)]);
+    void* hp = nullptr;
+    if (hipHostMalloc(&hp, kScratchWords * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        rm_destroy(c);
+        g_create_error = "HIP pinned allocation failed";
+        return RM_ERR_ALLOC;
+    }
+    c->scratch_host = static_cast<unsigned long long*>(hp);
     g_create_error.clear();
     *out = c;
     return RM_OK;
@@ -448,6 +487,7 @@ void rm_destroy(rm_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->wave_host) (void)hipHostFree(c->wave_host);
     if (c->read_host) (void)hipHostFree(c->read_host);
+    if (c->scratch_host) (void)hipHostFree(c->scratch_host);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1260,6 +1300,58 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
     st.stack_depth = c->stack_need;
     if (c->profiling && hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]) != hipSuccess) st.ms[7] = 0.0f;
     c->stats = st;
+    return RM_OK;
+}
+
+int rm_trace_device(rm_ctx* c, const float* origins, const float* dirs, const float* tmax, size_t n, int any,
+                    int32_t* tri, float* tuv) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    if (n == 0) return RM_OK;
+    if (n > kTraceDeviceMaxRays)
+        return c->fail(RM_ERR_INVALID, "rm_trace_device: at most 2^30 rays per call");
+    // Every buffer must be device memory of this device holding the bytes the
+    // kernel touches (a host pointer would fault the GPU instead of failing).
+    struct Arg { const void* p; size_t bytes; const char* name; };
+    const Arg args[] = {{origins, n * 12, "origins"}, {dirs, n * 12, "dirs"}, {tmax, n * 4, "tmax"},
+                        {tri, n * 4, "tri"}, {tuv, n * 16, "tuv"}};
+    for (const Arg& a : args) {
+        if (!a.p) {
+            if (a.p == tmax) continue;
+            return c->fail(RM_ERR_INVALID, std::string("rm_trace_device: ") + a.name + " is NULL");
+        }
+        if (reinterpret_cast<uintptr_t>(a.p) % (a.p == tuv ? 16 : 4) != 0)
+            return c->fail(RM_ERR_INVALID, std::string("rm_trace_device: ") + a.name + " is misaligned");
+        hipPointerAttribute_t at{};
+        void* base = nullptr;
+        size_t size = 0;
+        if (hipPointerGetAttributes(&at, a.p) != hipSuccess || at.type != hipMemoryTypeDevice || at.device != c->device ||
+            hipMemGetAddressRange(&base, &size, const_cast<void*>(a.p)) != hipSuccess ||
+            static_cast<const char*>(a.p) + a.bytes > static_cast<const char*>(base) + size) {
+            (void)hipGetLastError();
+            return c->fail(RM_ERR_INVALID, std::string("rm_trace_device: ") + a.name +
+                                               " is not device memory of this GPU holding the call's rays");
+        }
+    }
+    hipStream_t s = c->stream;
+    launch_zero_u64(c->scratch_dev, kNumCounters, s);
+    const DevScene sc = c->dev_scene();
+    launch_trace(c->stack_need, sc, FrameParams{}, int(n), origins, dirs, tmax, any, tri, reinterpret_cast<float4*>(tuv),
+                 c->scratch_dev, s);
+    if (int rc = check(c, hipGetLastError(), "kernel launch")) return rc;
+    if (int rc = check(c, hipMemcpyAsync(c->scratch_host, c->scratch_dev, kNumCounters * sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, s), "counters"))
+        return rc;
+    if (int rc = check(c, hipStreamSynchronize(s), "trace")) return rc;
+    if (c->scratch_host[kCntOverflow]) return c->fail(RM_ERR_DEVICE, "traversal stack overflow");
+    rm_stats st{};
+    collect(st, c->scratch_host);
+    st.bvh_nodes = c->nnodes;
+    st.stack_depth = c->stack_need;
+    c->stats = st;
+    c->err.clear();
     return RM_OK;
 }
 

@@ -73,6 +73,7 @@ void launch_temporal(const FrameParams& P, const GBufferSoA& gb, const GBufferSo
                      const float4* hist, float4* out, hipStream_t st);
 void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in, float4* out, int step, hipStream_t st);
 void launch_tonemap(int npx, const float4* in, uint8_t* rgb, float* hdr3, hipStream_t st);
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st);
 void launch_trace(int stack, const DevScene& sc, const FrameParams& P, int n, const float* o, const float* d,
                   const float* tmax, int any, int32_t* tri, float4* tuv, unsigned long long* cnt, hipStream_t st);
 // Path-replay adjoint of render_paths (1, 8 or 32 lanes per pixel, no adaptive

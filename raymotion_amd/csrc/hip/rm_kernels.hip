@@ -345,6 +345,13 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, FrameParams P, in
     pt.flush(cnt, kCntProbe);
 }
 
+// Zero fill of a small word array (in place of a memset node, which a caller
+// capturing the stream into a graph cannot use).
+__global__ __launch_bounds__(kBlock) void k_zero_u64(unsigned long long* p, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -394,6 +401,10 @@ void launch_atrous(const FrameParams& P, const GBufferSoA& gb, const float4* in,
 
 void launch_tonemap(int npx, const float4* in, uint8_t* rgb, float* hdr3, hipStream_t st) {
     hipLaunchKernelGGL(k_tonemap, dim3(blocks_for(size_t(npx))), dim3(kBlock), 0, st, npx, in, rgb, hdr3);
+}
+
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(size_t(n))), dim3(kBlock), 0, st, p, n);
 }
 
 void launch_trace(int stack, const DevScene& sc, const FrameParams& P, int n, const float* o, const float* d,
