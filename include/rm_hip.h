@@ -165,6 +165,18 @@ int rm_trace(rm_ctx* ctx, const float* origins, const float* dirs, const float* 
  * before denoising (W*H*3). Any pointer may be NULL. */
 int rm_read_restir(rm_ctx* ctx, rm_gbuffer* gbuf, rm_reservoir* rinit, rm_reservoir* rmerged, float* shade);
 
+/* The resident traversal tree as the kernels read it: counts receives the
+ * number of 4-wide nodes, of triangles and the stack depth the tree needs;
+ * nodes (if not NULL, room for node_cap nodes) 32 floats per node: lower x,
+ * upper x, lower y, upper y, lower z, upper z bounds of the four children,
+ * their references and kinds as int bits (-1 empty, 0 inner node, k > 0 leaf
+ * of k triangles from leaf-order index ref); tris (if not NULL, room for
+ * tri_cap triangles) 12 floats per triangle in leaf order: v0, v1, v2, the
+ * scene index as int bits, two unused. For inspection and for counting a
+ * traversal's fetches on the host; the reference's counterpart is its
+ * acceleration structure, which it does not expose. */
+int rm_read_tree(rm_ctx* ctx, float* nodes, size_t node_cap, float* tris, size_t tri_cap, int32_t* counts);
+
 /* rm_trace on device memory, allocating nothing. No counterpart in the
  * reference: for callers that cannot allocate device memory during the call
  * (a stream being captured into a graph, a serving loop, a harness that

@@ -1590,7 +1590,8 @@ typedef struct {
     int32_t* path;           // out: 0 megakernel, 1 restir
     int32_t* lanes;          // out: megakernel lanes
     int32_t* has_history;    // out
-    uint64_t* stats;         // out: [rays, node visits, triangle tests], may be NULL
+    uint64_t* stats;         // out: [rays, node visits, triangle tests, then rays of the megakernel,
+                             // generate, spatial and shade passes], 7 words, may be NULL
 } orc_debug;
 
 void* orc_history_create(void) { return new orc::History(); }
@@ -1645,12 +1646,25 @@ int orc_render_frame(const orc_scene* sc, const orc_frame* f, uint64_t seed, voi
         std::vector<float> shade(size_t(px) * 3);
         if (hist->gbuf.size() != px) { hist->gbuf.assign(px, orc_gbuffer{}); hist->res.assign(px, orc_reservoir{}); }
         if (hist->color.size() != size_t(px) * 3) hist->color.assign(size_t(px) * 3, 0.0f);
+        auto rays_so_far = [&] {
+            uint64_t r = 0;
+            for (const auto& s : per) r += s.rays;
+            return r;
+        };
+        uint64_t pass_rays[3] = {0, 0, 0};
         parallel_for(px, threads, [&](uint32_t i) { restir_generate(fr, i, want_stats ? &per[i] : nullptr, gbuf.data(), rinit.data()); });
+        if (want_stats) pass_rays[0] = rays_so_far();
         parallel_for(px, threads, [&](uint32_t i) {
             restir_spatial(fr, i, want_stats ? &per[i] : nullptr, gbuf.data(), rinit.data(), rmerged.data(),
                            hist->gbuf.data(), hist->res.data());
         });
+        if (want_stats) pass_rays[1] = rays_so_far() - pass_rays[0];
         parallel_for(px, threads, [&](uint32_t i) { restir_shade(fr, i, want_stats ? &per[i] : nullptr, gbuf.data(), rmerged.data(), shade.data()); });
+        if (want_stats) {
+            pass_rays[2] = rays_so_far() - pass_rays[0] - pass_rays[1];
+            dbg->stats[3] = 0;
+            dbg->stats[4] = pass_rays[0]; dbg->stats[5] = pass_rays[1]; dbg->stats[6] = pass_rays[2];
+        }
         std::vector<float> final_img;
         if (f->denoise) {
             std::vector<float> a(size_t(px) * 3), b(size_t(px) * 3);
@@ -1696,6 +1710,7 @@ int orc_render_frame(const orc_scene* sc, const orc_frame* f, uint64_t seed, voi
     if (want_stats) {
         for (const auto& s : per) { total.rays += s.rays; total.nodes += s.nodes; total.tris += s.tris; }
         dbg->stats[0] = total.rays; dbg->stats[1] = total.nodes; dbg->stats[2] = total.tris;
+        if (!use_spatial) { dbg->stats[3] = total.rays; dbg->stats[4] = dbg->stats[5] = dbg->stats[6] = 0; }
     }
     return 0;
 }
@@ -1856,6 +1871,106 @@ int orc_trace(const orc_scene* sc, const float* o, const float* d, const float* 
         stats[0] = stats[1] = 0;
         for (const auto& s : per) { stats[0] += s.nodes; stats[1] += s.tris; }
     }
+    return 0;
+}
+
+// Host walk of the backend's own traversal tree, to count what its traversal
+// fetches. nodes: nnode 4-wide nodes of 8 float4 words each (lower bounds x,
+// upper x, lower y, upper y, lower z, upper z of the four children, then
+// their references and kinds as int bits: -1 empty, 0 inner node, k > 0 leaf
+// of k triangles from leaf-order index ref); geo: the triangles in leaf order,
+// 3 float4 words each (v0.xyz v1.x | v1.yz v2.xy | v2.z id). The walk is the
+// device's (Tracer::intersect): slab tests with fused multiply-adds, children
+// sorted by entry distance, hit leaves intersected nearest-first within the
+// current hit plus 2^-12, inner children descended nearest-first with the rest
+// stacked. stats: [node visits, triangle tests] over all rays.
+int orc_trace_wide(const float* nodes, size_t nnode, const float* geo, const float* o, const float* d,
+                   const float* tmax, size_t n, int any, int threads, int32_t* tri, float* t, uint64_t* stats) {
+    using namespace orc;
+    std::vector<Stats> per(n);
+    auto ibits = [](float f) { int32_t i; std::memcpy(&i, &f, 4); return i; };
+    parallel_for(uint32_t(n), threads, [&](uint32_t i) {
+        const V3<float> ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]}, rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+        float ht = tmax ? tmax[i] : 1e30f;
+        int hid = -1;
+        Stats& st = per[i];
+        ++st.rays;
+        if (nnode > 0) {
+            const float ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x), iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y),
+                        iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
+            const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
+            std::vector<int> stack;
+            int node = 0;
+            bool done = false;
+            while (!done) {
+                const float* w = nodes + size_t(node) * 32;
+                ++st.nodes;
+                const float tc = ht + ht * (1.0f / 4096.0f);
+                float tn[4];
+                int ref[4], kind[4];
+                for (int c = 0; c < 4; ++c) {
+                    const float ax = std::fma(w[0 + c], ix, ox), bx = std::fma(w[4 + c], ix, ox);
+                    const float ay = std::fma(w[8 + c], iy, oy), by = std::fma(w[12 + c], iy, oy);
+                    const float az = std::fma(w[16 + c], iz, oz), bz = std::fma(w[20 + c], iz, oz);
+                    const float nr = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)),
+                                               std::fmax(std::fmin(az, bz), 0.0f));
+                    const float fr = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
+                                               std::fmin(std::fmax(az, bz), tc));
+                    const int k = ibits(w[28 + c]);
+                    const bool ok = k >= 0 && nr <= fr;
+                    tn[c] = ok ? nr : INFINITY;
+                    ref[c] = ibits(w[24 + c]);
+                    kind[c] = ok ? k : -1;
+                }
+                auto cx = [&](int a, int b) {
+                    if (tn[b] < tn[a]) { std::swap(tn[a], tn[b]); std::swap(ref[a], ref[b]); std::swap(kind[a], kind[b]); }
+                };
+                cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+                int next = -1;
+                for (int c = 3; c >= 0; --c)
+                    if (kind[c] == 0) {
+                        if (next >= 0) stack.push_back(next);
+                        next = ref[c];
+                    }
+                if (next < 0) {
+                    if (stack.empty()) done = true;
+                    else { next = stack.back(); stack.pop_back(); }
+                }
+                bool found = false;
+                for (int c = 0; c < 4 && !found; ++c) {
+                    if (!(kind[c] > 0 && tn[c] <= ht + ht * (1.0f / 4096.0f))) continue;
+                    for (int k = 0; k < kind[c] && !found; ++k) {
+                        const float* g = geo + size_t(ref[c] + k) * 12;
+                        ++st.tris;
+                        const V3<float> v0{g[0], g[1], g[2]}, v1{g[3], g[4], g[5]}, v2{g[6], g[7], g[8]};
+                        const int id = ibits(g[9]);
+                        const V3<float> e1 = v1 - v0, e2 = v2 - v0;
+                        const V3<float> p = cross3(rd, e2);
+                        const float det = dot3(e1, p);
+                        if (std::fabs(det) < 1e-12f) continue;
+                        const V3<float> tv = ro - v0;
+                        const float u = dot3(tv, p) / det;
+                        if (u < 0.0f || u > 1.0f) continue;
+                        const V3<float> q = cross3(tv, e1);
+                        const float v = dot3(rd, q) / det;
+                        if (v < 0.0f || u + v > 1.0f) continue;
+                        const float dd = dot3(e2, q) / det;
+                        if (dd > 1e-8f && (dd < ht || (dd == ht && hid >= 0 && id < hid))) {
+                            ht = dd;
+                            hid = id;
+                            if (any) found = true;
+                        }
+                    }
+                }
+                if (found) break;
+                node = next;
+            }
+        }
+        tri[i] = hid;
+        if (t) t[i] = ht;
+    });
+    stats[0] = stats[1] = 0;
+    for (const auto& s : per) { stats[0] += s.nodes; stats[1] += s.tris; }
     return 0;
 }
 

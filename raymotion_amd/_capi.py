@@ -95,6 +95,7 @@ _SIGNATURES = [
     ("rm_render", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P]),
     ("rm_trace", _I, [_P, _P, _P, _P, _SZ, _I, _P, _P, _P, _P]),
     ("rm_trace_device", _I, [_P, _P, _P, _P, _SZ, _I, _P, _P]),
+    ("rm_read_tree", _I, [_P, _P, _SZ, _P, _SZ, _P]),
     ("rm_read_restir", _I, [_P, _P, _P, _P, _P]),
     ("rm_restir_merge", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P, _P, _P, _P, _P]),
     ("rm_denoise_image", _I, [_P, ctypes.POINTER(Frame), _P, _P, _P, _P, _P, _P]),

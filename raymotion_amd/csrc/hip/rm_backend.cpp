@@ -1303,6 +1303,25 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
     return RM_OK;
 }
 
+int rm_read_tree(rm_ctx* c, float* nodes, size_t node_cap, float* tris, size_t tri_cap, int32_t* counts) {
+    if (!c || !counts) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    counts[0] = c->nnodes;
+    counts[1] = c->ntri;
+    counts[2] = c->stack_need;
+    const size_t nb = size_t(c->nnodes) * kNodeWords * sizeof(float4), tb = size_t(c->ntri) * kTriGeoWords * sizeof(float4);
+    if ((nodes && node_cap < size_t(c->nnodes)) || (tris && tri_cap < size_t(c->ntri)))
+        return c->fail(RM_ERR_INVALID, "rm_read_tree: output arrays too small");
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "readback")) return rc;
+    if (nodes && nb && hipMemcpy(nodes, c->nodes.ptr, nb, hipMemcpyDeviceToHost) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "readback failed");
+    if (tris && tb && hipMemcpy(tris, c->geo.ptr, tb, hipMemcpyDeviceToHost) != hipSuccess)
+        return c->fail(RM_ERR_DEVICE, "readback failed");
+    return RM_OK;
+}
+
 int rm_trace_device(rm_ctx* c, const float* origins, const float* dirs, const float* tmax, size_t n, int any,
                     int32_t* tri, float* tuv) {
     if (!c) return RM_ERR_INVALID;

@@ -315,6 +315,17 @@ class Backend:
                                            ptr(out)), "rm_denoise_image")
         return out
 
+    def read_tree(self):
+        """(nodes (N, 32) float32, leaf-order triangles (T, 12) float32, stack depth)
+        of the resident traversal tree, in the kernels' layout."""
+        counts = np.zeros(3, np.int32)
+        self._check(lib().rm_read_tree(self._h, None, 0, None, 0, ptr(counts)), "rm_read_tree")
+        nodes = np.zeros((int(counts[0]), 32), np.float32)
+        tris = np.zeros((int(counts[1]), 12), np.float32)
+        self._check(lib().rm_read_tree(self._h, ptr(nodes), len(nodes), ptr(tris), len(tris), ptr(counts)),
+                    "rm_read_tree")
+        return nodes, tris, int(counts[2])
+
     def set_profiling(self, on: bool):
         self._check(lib().rm_set_profiling(self._h, int(on)), "rm_set_profiling")
 

@@ -28,7 +28,7 @@ def frame(width, height, spp, bounces=5, *, cam=None, candidates=1, spatial=0, m
 
 def default_camera(width, focal=None):
     """Camera at the origin looking down +z (the reference's default Camera),
-    focal width * 0.8660254 unless given (tests/test_metal.cpp:45)."""
+    focal width * 0.8660254 unless given (tests/test_metal.cpp:21)."""
     c = rm.Camera()
     c.x = c.y = c.z = 0.0
     c.focal = width * 0.8660254 if focal is None else focal
@@ -42,7 +42,7 @@ def reference_default_camera():
 
 
 def tri(scene, a, b, c, color, emission=(0, 0, 0), metal=0.0, ior=1.0, rough=0.5, alpha=1.0, shader=0):
-    """triangle() of tests/test_metal.cpp:34-38: flat normal, given material."""
+    """triangle() of tests/test_metal.cpp:10-14: flat normal, given material."""
     a, b, c = (np.asarray(v, np.float64) for v in (a, b, c))
     n = np.cross(b - a, c - a)
     n = n / np.linalg.norm(n)
@@ -57,7 +57,7 @@ def quad(scene, a, b, c, d, color, **kw):
 
 
 def metal_test_scene():
-    """The lit mixed-material scene of tests/test_metal.cpp:53-59 (no texture)."""
+    """The lit mixed-material scene of tests/test_metal.cpp:29-35 (no texture)."""
     s = rm.Scene()
     quad(s, (-4, -2, 7), (-4, 3, 7), (4, 3, 7), (4, -2, 7), (.7, .7, .7))
     quad(s, (-4, -2, 1), (-4, -2, 7), (4, -2, 7), (4, -2, 1), (.7, .2, .1))
@@ -70,7 +70,7 @@ def metal_test_scene():
 
 
 def alpha_surface(scene, z, alpha, emission):
-    """surface() of tests/test_alpha.cpp:8-12."""
+    """surface() of tests/test_alpha.cpp:7-11."""
     n = np.array([0.0, 0.0, -1.0])
     m = material_rows(1, albedo=(0, 0, 0), emission=emission, metallic=0.0, ior=1.0, rough=0.5, alpha=alpha)
     scene.add_triangles(np.array([[(-10, -10, z), (0, 10, z), (10, -10, z)]], np.float64), m,

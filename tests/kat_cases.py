@@ -2,8 +2,8 @@
 `render(exported, frame, seed) -> rgb8` callable so that the same checks run
 against the CPU oracle and against the HIP backend.
 
-  metal_*  tests/test_metal.cpp:43-100
-  alpha_*  tests/test_alpha.cpp:13-30
+  metal_*  tests/test_metal.cpp:19-76
+  alpha_*  tests/test_alpha.cpp:12-29
 """
 from __future__ import annotations
 
@@ -25,15 +25,15 @@ def _exported(scene):
 
 
 def metal_empty(render):
-    """test_metal.cpp:44-52: an empty scene renders black."""
+    """test_metal.cpp:20-28: an empty scene renders black."""
     f = _cfg(96, 64, 512, adapt_min=0)
     img = render(_exported(rm.Scene()), f, 12345)
     assert np.all(img == 0), "empty scene is not black"
 
 
-def metal_lit_deterministic(render):
-    """test_metal.cpp:53-70: the lit mixed-material scene with a 2x2 texture
-    (UVs outside [0,1] wrap) is not black and renders identically twice."""
+def lit_textured_scene():
+    """The lit mixed-material scene of test_metal.cpp:29-41 with its 2x2
+    texture on every non-emissive triangle, and the test's frame settings."""
     f = _cfg(96, 64, 512, adapt_min=0)
     e = _exported(metal_test_scene())
     emissive = (e.tris["er"] + e.tris["eg"] + e.tris["eb"]) == 0
@@ -43,6 +43,13 @@ def metal_lit_deterministic(render):
     for name, value in (("tu0", -.2), ("tv0", .1), ("tu1", 1.4), ("tv1", .2), ("tu2", .3), ("tv2", 1.7)):
         e.tris[name][emissive] = value
     e.texels = np.array([1, .1, .1, .1, 1, .1, .1, .1, 1, .5, .5, .5], np.float32)
+    return e, f
+
+
+def metal_lit_deterministic(render):
+    """test_metal.cpp:29-46: the lit mixed-material scene with a 2x2 texture
+    (UVs outside [0,1] wrap) is not black and renders identically twice."""
+    e, f = lit_textured_scene()
     a = render(e, f, 12345)
     assert np.any(a > 0), "lit scene is black"
     b = render(e, f, 12345)
@@ -58,7 +65,7 @@ def _fullscreen_quad(x=0.0):
 
 
 def metal_emission_exact(render):
-    """test_metal.cpp:71-76: a full-screen emissive quad gives exactly (127,0,0)
+    """test_metal.cpp:47-52: a full-screen emissive quad gives exactly (127,0,0)
     with adaptive sampling, after a scene re-upload."""
     f = _cfg(96, 64, 512, adapt_min=16)
     img = render(_exported(_fullscreen_quad()), f, 98765).reshape(-1, 3)
@@ -66,7 +73,7 @@ def metal_emission_exact(render):
 
 
 def metal_alpha_coverage(render):
-    """test_metal.cpp:77-90: coverage 0, 0.5, 1 on the emissive quad at 2048 spp."""
+    """test_metal.cpp:53-66: coverage 0, 0.5, 1 on the emissive quad at 2048 spp."""
     f = _cfg(16, 16, 2048, adapt_min=0, cam=default_camera(16))
     e = _exported(_fullscreen_quad())
     for opacity in (0.0, 0.5, 1.0):
@@ -81,7 +88,7 @@ def metal_alpha_coverage(render):
 
 
 def metal_frames_refit(render):
-    """test_metal.cpp:91-100: 34 frames at an odd size alternate the quad in and
+    """test_metal.cpp:67-76: 34 frames at an odd size alternate the quad in and
     out of view (buffer reuse, dispatch tails, refits and the periodic rebuild)."""
     f = _cfg(37, 23, 9, adapt_min=0)
     for k in range(34):
@@ -99,7 +106,7 @@ def _alpha_cfg(w, h, spp):
 
 
 def alpha_two_layers(render):
-    """test_alpha.cpp:14-23: red surface of opacity o in front of opaque green;
+    """test_alpha.cpp:13-22: red surface of opacity o in front of opaque green;
     mean red 255 o/(1+o), mean green 255 (1-o)/(2-o), within 3."""
     f = _alpha_cfg(8, 8, 4096)
     for o in (0.0, 0.5, 1.0):
@@ -113,7 +120,7 @@ def alpha_two_layers(render):
 
 
 def alpha_transparent_stack(render):
-    """test_alpha.cpp:24-30: 64 fully transparent layers over opaque green give
+    """test_alpha.cpp:23-29: 64 fully transparent layers over opaque green give
     exactly (0, 127) at one sample: crossings consume no bounces."""
     s = rm.Scene()
     for i in range(64):

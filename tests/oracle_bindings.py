@@ -51,6 +51,8 @@ def orc() -> ctypes.CDLL:
                                                        ctypes.c_int, _P]),
             ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
+            ("orc_trace_wide", ctypes.c_int, [_P, _SZ, _P, _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
+                                              _P]),
             ("orc_restir_merge", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame), ctypes.c_uint64,
                                                 _P, _P, _P, _P, _P, ctypes.c_int, _P]),
             ("orc_denoise_image", ctypes.c_int, [ctypes.POINTER(Frame), _P, _P, _P, _P, _P, ctypes.c_int, _P]),
@@ -108,7 +110,7 @@ def render_frame(exported, frame: Frame, seed: int, history: History | None = No
     path = ctypes.c_int32(-1)
     lanes = ctypes.c_int32(0)
     hh = ctypes.c_int32(0)
-    stats = np.zeros(3, np.uint64)
+    stats = np.zeros(7, np.uint64)
     dbg.path = ctypes.addressof(path)
     dbg.lanes = ctypes.addressof(lanes)
     dbg.has_history = ctypes.addressof(hh)
@@ -127,7 +129,9 @@ def render_frame(exported, frame: Frame, seed: int, history: History | None = No
     if rc:
         raise RuntimeError(f"orc_render_frame failed ({rc})")
     info.update(path=path.value, lanes=lanes.value, has_history=hh.value, rays=int(stats[0]),
-                node_visits=int(stats[1]), tri_tests=int(stats[2]))
+                node_visits=int(stats[1]), tri_tests=int(stats[2]),
+                pass_rays={"megakernel": int(stats[3]), "generate": int(stats[4]), "spatial": int(stats[5]),
+                           "shade": int(stats[6])})
     return rgb, hdr, info
 
 
@@ -208,6 +212,23 @@ def trace(exported, origins, dirs, tmax=None, any_hit=False, threads: int = THRE
     orc().orc_trace(ctypes.byref(d), ptr(o), ptr(dd), ptr(tm), n, int(any_hit), threads, ptr(tri), ptr(t), ptr(u),
                     ptr(v), ptr(stats))
     return tri, t, u, v, stats
+
+
+def trace_wide(nodes, tris, origins, dirs, tmax=None, any_hit=False, threads: int = THREADS):
+    """Host walk of the backend's own 4-wide tree (Backend.read_tree): hit ids,
+    distances and the [node visits, triangle tests] it makes."""
+    nd = np.ascontiguousarray(nodes, np.float32)
+    tr = np.ascontiguousarray(tris, np.float32)
+    o = np.ascontiguousarray(origins, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = len(o)
+    tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+    tri = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    stats = np.zeros(2, np.uint64)
+    orc().orc_trace_wide(ptr(nd), len(nd), ptr(tr), ptr(o), ptr(d), ptr(tm), n, int(any_hit), threads, ptr(tri),
+                         ptr(t), ptr(stats))
+    return tri, t, stats
 
 
 def trace_brute(exported, origins, dirs, threads: int = THREADS):

@@ -99,7 +99,7 @@ def test_camera_matches_reference(pos, rot, fov):
 
 
 def test_reference_test_bvh_cases():
-    """tests/test_bvh.cpp:5-24 against the product engine."""
+    """tests/test_bvh.cpp:13-24 against the product engine."""
     def fill(s, z, n):
         s.clear()
         pos = np.array([[(3 * i - 1, -1, z), (3 * i + 1, -1, z), (3 * i, 1, z)] for i in range(n)], np.float64)
@@ -127,7 +127,7 @@ def test_reference_test_bvh_cases():
 
 
 def test_alpha_clamped_by_add_triangle():
-    """tests/test_alpha.cpp:31-32: alpha -1 -> 0, 2 -> 1, NaN -> 1."""
+    """tests/test_alpha.cpp:30-31: alpha -1 -> 0, 2 -> 1, NaN -> 1."""
     s = rm.Scene()
     pos = np.array([[(-10, -10, z), (0, 10, z), (10, -10, z)] for z in (1, 2, 3)], np.float64)
     mat = material_rows(3)

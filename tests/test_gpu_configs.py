@@ -34,6 +34,10 @@ def _check(backend, exported, f, seed, hist, expect_history):
     assert info["has_history"] == expect_history
     assert err.max() <= 1e-3
     assert np.array_equal(g_rgb[same], c_rgb[same])
+    st = backend.stats()
+    gpu = {"megakernel": st.rays[0], "generate": st.rays[1], "spatial": st.rays[2], "shade": st.rays[3]}
+    print(f"  rays per pass: GPU {gpu}, oracle {info['pass_rays']}", flush=True)
+    assert gpu == info["pass_rays"]
 
 
 def test_mixed_scene_720p_frames_match_oracle(backend):

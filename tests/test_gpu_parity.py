@@ -10,6 +10,7 @@ import pytest
 
 import oracle_bindings as ob
 from helpers import cornell, cornell_frame, frame, metal_test_scene, rel_err
+from kat_cases import lit_textured_scene
 from raymotion_amd import scenes
 from raymotion_amd.rng import random_rays
 
@@ -35,6 +36,31 @@ def test_trace_cornell_2p20_bitexact(backend):
     print(f"cornell 2^20 rays: id mismatches {bad.sum()}, max t rel {t_rel.max():.3g}")
     assert bad.sum() == 0
     assert t_rel.max() <= 1e-6
+
+
+@pytest.mark.parametrize("scene", ["cornell", "mixed"])
+def test_trace_counters_match_host_walk(backend, scene):
+    """Node visits and triangle tests the GPU counts (the roofline's traversal
+    bytes) equal a host walk of the same resident 4-wide tree over the same
+    rays, and so do the hits."""
+    if scene == "cornell":
+        e = cornell().export()
+        o, d = random_rays(1 << 20, 0)
+        backend.set_scene(e)
+    else:
+        s = scenes.mixed_spheres(seed=0, n_spheres=16)
+        e = s.export()
+        o, d = random_rays(1 << 17, 3, lo=(-5, -5, -5), hi=(5, 5, 5))
+        backend.set_scene(e, gpu_tree=True, rebuild=True)
+    g_tri, g_t, _, _ = backend.trace(o, d)
+    st = backend.stats()
+    nodes, tris, _ = backend.read_tree()
+    c_tri, c_t, walk = ob.trace_wide(nodes, tris, o, d)
+    print(f"{scene}: GPU {st.node_visits[4]} node visits, {st.tri_tests[4]} triangle tests; "
+          f"host walk {int(walk[0])}, {int(walk[1])}", flush=True)
+    assert np.array_equal(g_tri, c_tri) and np.array_equal(g_t[g_tri >= 0], c_t[c_tri >= 0])
+    assert st.rays[4] == len(o)
+    assert st.node_visits[4] == int(walk[0]) and st.tri_tests[4] == int(walk[1])
 
 
 def test_trace_cornell_anyhit(backend):
@@ -68,7 +94,16 @@ def _compare_render(backend, exported, f, seed, tol=1e-3):
     # 8-bit output must agree wherever the radiance agrees bit for bit.
     same = (g_hdr == c_hdr)
     assert np.all(g_rgb[same] == c_rgb[same])
+    check_pass_rays(backend.stats(), info)
     return info
+
+
+def check_pass_rays(st, info):
+    """The GPU's per-pass ray counters (which the throughput figures divide by)
+    equal the rays the oracle traced in the same passes."""
+    gpu = {"megakernel": st.rays[0], "generate": st.rays[1], "spatial": st.rays[2], "shade": st.rays[3]}
+    assert gpu == info["pass_rays"], (gpu, info["pass_rays"])
+    assert st.rays[5] == info["rays"]
 
 
 def test_render_cornell_c0(backend):
@@ -88,6 +123,30 @@ def test_render_adaptive_lanes8(backend):
     e = cornell().export()
     info = _compare_render(backend, e, cornell_frame(40, 32, 64, 5, adapt_min=16, adapt_step=8), 4242)
     assert info["lanes"] == 8
+
+
+def test_render_adaptive_one_lane(backend):
+    """adapt_step % 8 != 0 keeps the megakernel at one lane per pixel with the
+    adaptive stop (metal.mm:390-394, pathtrace.metal:759-765)."""
+    e = cornell().export()
+    info = _compare_render(backend, e, cornell_frame(40, 32, 32, 5, adapt_min=16, adapt_step=4), 4243)
+    assert info["lanes"] == 1
+
+
+def test_render_textured_lit_scene_hdr(backend):
+    """The lit scene of tests/test_metal.cpp:29-46 with its 2x2 texture (UVs
+    outside [0, 1] wrap), compared in linear radiance at the 1e-3 bar."""
+    e, f = lit_textured_scene()
+    f.spp = 64
+    _compare_render(backend, e, f, 12345)
+
+
+def test_render_hue_shader_scene(backend):
+    """shader == 1 triangles in a path-traced frame."""
+    from transport_pins import hue_scene
+    e, cam = hue_scene(120.0)
+    f = frame(48, 48, 16, 4, cam=cam, candidates=4, spatial=0)
+    _compare_render(backend, e, f, 99)
 
 
 def test_render_metal_scene_ris(backend):
@@ -125,6 +184,7 @@ def test_restir_pipeline_cornell(backend):
             for field in ("light_pos", "light_normal", "emission", "alpha", "p_hat", "sum_ws", "sum_wu", "m"):
                 assert np.array_equal(g_res[field][v], c_res[field][v]), (name, field)
         assert rel_err(gshade, info["shade"]).max() <= 1e-3
+        check_pass_rays(backend.stats(), info)
         err = rel_err(g_hdr, c_hdr)
         print(f"frame {k}: shade exact {np.mean(gshade == info['shade']):.6f}, final max rel {err.max():.3g}")
         assert err.max() <= 1e-3

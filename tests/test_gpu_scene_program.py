@@ -3,7 +3,7 @@ main() the .ray compiler generates (tests/cpp/scene_program.cpp: Objects from
 argv, push, render, finish) renders a frame through Renderer and
 render_image_hip and writes a PNG. Its scene is an emissive quad (emission
 (1, 0, 0)) filling the view, so every pixel is exactly (127, 0, 0), the
-reference's own known answer for emission (tests/test_metal.cpp:48-52), with
+reference's own known answer for emission (tests/test_metal.cpp:47-52), with
 the megakernel path and with the ReSTIR + denoiser path."""
 import os
 import struct

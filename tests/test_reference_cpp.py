@@ -31,3 +31,14 @@ def test_reference_test_runtime(tmp_path):
     r = _run("test_runtime", tmp_path, RAYMOTION_GPU_BVH="0")
     assert r.returncode == 0, r.stderr
     assert "runtime geometry passed" in r.stdout
+
+
+def test_reference_test_metal_skips_without_gpu(tmp_path):
+    # tests/test_metal.cpp:23-27 exits 77 (skip) when render_image_metal says
+    # "No Metal GPU found"; on a host without a GPU the backend says so.
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present (tests/test_gpu_reference_cpp.py runs the binary there)")
+    r = _run("test_metal", tmp_path)
+    assert r.returncode == 77, (r.returncode, r.stderr)
+    assert "No Metal GPU found" in r.stderr
