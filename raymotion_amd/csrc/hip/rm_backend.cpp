@@ -919,7 +919,10 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
         // Publish this frame as the next frame's history (metal.mm:374-385).
         std::swap(c->gbuf, c->hist_gbuf);
         std::swap(c->rmerged, c->hist_res);
-        if (f->denoise == 1 && !final_in_hist) std::swap(c->den_a, c->hist_color);
+        // The history colour is the frame's denoised output (metal.mm:378,
+        // which swaps it after every denoised frame); a learned frame's output
+        // lies in den_a as the a-trous chain's may.
+        if (f->denoise != 0 && !final_in_hist) std::swap(c->den_a, c->hist_color);
         c->hist_cam = f->camera;
         c->hist_w = f->width;
         c->hist_h = f->height;

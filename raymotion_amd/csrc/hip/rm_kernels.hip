@@ -94,13 +94,17 @@ __global__ __launch_bounds__(kBlock) void k_spatial(DevScene sc, FrameParams P, 
     if (id >= uint32_t(P.width) * uint32_t(P.height)) return;
     const uint32_t x = id % uint32_t(P.width), y = id / uint32_t(P.width);
     const float4 i0 = ri.r0[id], i1 = ri.r1[id], i2 = ri.r2[id], i3 = ri.r3[id];
-    ro_.r0[id] = i0; ro_.r1[id] = i1; ro_.r2[id] = i2; ro_.r3[id] = i3;
     const float4 g0 = gb.p0[id];
-    if (ibits(g0.w) != 1 || __float_as_uint(i3.z) == 0u) return;
     int Sn = P.restir_spatial;
     if (Sn < 0) Sn = 0;
     if (Sn > 8) Sn = 8;
-    if (Sn == 0) return;
+    // Pixels that merge nothing keep their initial reservoir; each output is
+    // written once (an early copy overwritten after the shadow rays would
+    // reach HBM twice).
+    if (ibits(g0.w) != 1 || __float_as_uint(i3.z) == 0u || Sn == 0) {
+        ro_.r0[id] = i0; ro_.r1[id] = i1; ro_.r2[id] = i2; ro_.r3[id] = i3;
+        return;
+    }
     float radius = P.restir_radius;
     if (!(radius > 0.0f)) radius = 16.0f;
     int mcap = P.restir_mcap;

@@ -562,13 +562,10 @@ unsigned blocks_of(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
 void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, const float* b, const __bf16* act,
                  void* out, int n, int h, int wdt, hipStream_t st) {
     const size_t P = size_t(n) * h * wdt;
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
     const size_t tiles = (P + 16 * kConvJ - 1) / (16 * kConvJ);
     const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * kConvBlocksPerCU))), block(256);
 #define RM_CONV(CI, CO, MO) \

@@ -17,6 +17,10 @@
 #include "rm_device.h"
 #include "rm_kernels.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 namespace rmh {
 namespace {
 
@@ -612,12 +616,18 @@ struct GenEngine {
         const uint32_t x = uint32_t(px) % uint32_t(P.width), y = uint32_t(px) / uint32_t(P.width);
         L.pixel = px;
         pt.rng.seed(pixel_seed(P.seed, x, y));
-        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        ri.r0[px] = zero; ri.r1[px] = zero; ri.r2[px] = zero; ri.r3[px] = zero;  // valid = 0
         float sx, sy;
         pt.jitter(x, y, sx, sy);
         set_ray(L, kExt, V3{P.cam.x, P.cam.y, P.cam.z}, pt.primary_dir(sx, sy, true), 1e30f);
         return kTrace;
+    }
+
+    // A pixel's reservoir is written once, when the pixel is done: an early
+    // "invalid" store at its start and the final one much later would reach
+    // HBM twice (the line is evicted in between).
+    __device__ void invalid_reservoir(int px) {
+        const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ri.r0[px] = zero; ri.r1[px] = zero; ri.r2[px] = zero; ri.r3[px] = zero;  // valid = 0
     }
 
     __device__ void finish_candidate(Lane& L, float wu, float ws, float ph, int li, float lu, float lv) {
@@ -696,6 +706,8 @@ struct GenEngine {
             ri.r1[px] = make_float4(l2.y, l2.z, l2.w, l3.w);
             ri.r2[px] = make_float4(l3.x, l3.y, l3.z, L.sumWs);
             ri.r3[px] = make_float4(L.sumWu, __uint_as_float(uint32_t(L.Mc)), __uint_as_float(1u), 0.0f);
+        } else {
+            invalid_reservoir(L.pixel);
         }
         return kPixelDone;
     }
@@ -721,7 +733,10 @@ struct GenEngine {
         gb.p2[px] = make_float4(hit.ar, hit.ag, hit.ab, metallic);
         gb.p3[px] = make_float4(wox, woy, woz, dielectric ? hit.ior : 1.0f);
         gb.p4[px] = make_float4(hit.er, hit.eg, hit.eb, 0.0f);
-        if (dielectric || pt.sc.nlights == 0 || !(pt.sc.light_total > 1e-12f)) return kPixelDone;
+        if (dielectric || pt.sc.nlights == 0 || !(pt.sc.light_total > 1e-12f)) {
+            invalid_reservoir(px);
+            return kPixelDone;
+        }
         int Mc = P.restir_candidates;
         if (Mc < 1) Mc = 1;
         if (Mc > 32) Mc = 32;
@@ -765,6 +780,7 @@ struct GenEngine {
             const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             const int px = L.pixel;
             gb.p0[px] = zero; gb.p1[px] = zero; gb.p2[px] = zero; gb.p3[px] = zero; gb.p4[px] = zero;
+            invalid_reservoir(px);
             return kPixelDone;
         }
         return primary_hit(L, pt.surface_at(h, L.ro, L.rd));
@@ -938,25 +954,33 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
     pt.flush(cnt, E::kPass);
 }
 
+// Blocks of `kernel` resident on the current device at once (CUs x blocks per
+// CU), looked up once per device and kernel.
 template <int S>
 int resident_blocks(const void* kernel) {
-    int dev = 0, cus = 1, occ = 0;
-    if (hipGetDevice(&dev) == hipSuccess) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(dev, kernel);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 1, occ = 0;
+    if (dev >= 0) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
             cus = prop.multiProcessorCount;
     }
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ < 1) occ = 1;
-    return cus * occ;
+    return cache[key] = cus * occ;
 }
 
-// Launches k_persistent<S, E> with one resident wave set (cached per instance).
+// Launches k_persistent<S, E> with one resident wave set.
 template <int S, class E>
 void launch_persistent(const DevScene& sc, const FrameParams& P, const PassIO& io, const BandQueue& q,
                        unsigned long long* cnt, hipStream_t st) {
-    static int resident = 0;
-    const void* kernel = reinterpret_cast<const void*>(&k_persistent<S, E>);
-    if (resident == 0) resident = resident_blocks<S>(kernel);
+    const int resident = resident_blocks<S>(reinterpret_cast<const void*>(&k_persistent<S, E>));
     const int npx = P.width * P.height;
     const int need = (npx + kBlock - 1) / kBlock;
     const dim3 grid(unsigned(need < resident ? need : resident)), block(kBlock);
@@ -1208,12 +1232,9 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_wf_finish(DevScene sc
 template <int S, class E>
 int run_wavefront(const DevScene& sc, const FrameParams& P, const PassIO& io, const WaveBufs& wb,
                   unsigned long long* cnt, hipStream_t st, WaveRun* run) {
-    static int trace_blocks = 0, logic_blocks = 0, finish_blocks = 0;
-    if (trace_blocks == 0) {
-        trace_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_trace<S, E>));
-        logic_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_logic<S, E>));
-        finish_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_finish<S, E>));
-    }
+    const int trace_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_trace<S, E>));
+    const int logic_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_logic<S, E>));
+    const int finish_blocks = resident_blocks<S>(reinterpret_cast<const void*>(&k_wf_finish<S, E>));
     const unsigned npx = unsigned(P.width) * unsigned(P.height);
     if (size_t(npx) > wb.stride) return int(hipErrorInvalidValue);
     const size_t stride = wb.stride;

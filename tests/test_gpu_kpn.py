@@ -123,3 +123,27 @@ def test_trained_network_beats_atrous(backend):
         err[name] = denoise.tonemapped_rmse(hdr, ref)
     print("held-out tonemapped RMSE:", {k: round(v, 5) for k, v in err.items()})
     assert err["kpn"] < err["atrous"] < err["noisy"]
+
+
+def test_learned_frame_feeds_the_next_temporal_pass(backend):
+    """A learned-denoiser frame leaves its output as the colour history: the
+    next a-trous frame's temporal pass blends that image (checked by running
+    the same passes alone on the read-back buffers)."""
+    s = scenes.cornell_box()
+    s.build_bvh()
+    backend.set_scene(s.export())
+    backend.kpn_set_weights(_params(3))
+    cam_a = scenes.cornell_camera()
+    cam_a.focal = 32 / np.tan(np.radians(20.0))
+    cam_b = scenes.cornell_camera()
+    cam_b.focal = cam_a.focal
+    cam_b.x += 0.01
+    fa = frame(64, 64, 4, 5, cam=cam_a, candidates=8, spatial=3, radius=8.0, denoise=2)
+    fb = frame(64, 64, 4, 5, cam=cam_b, candidates=8, spatial=3, radius=8.0, denoise=1, iterations=2)
+    _, hdr_a = backend.render(fa, 100)
+    g_a, _, _, _ = backend.read_restir(64, 64)
+    _, hdr_b = backend.render(fb, 101)
+    assert backend.stats().has_history == 1
+    g_b, _, _, shade_b = backend.read_restir(64, 64)
+    again = backend.denoise_image(fb, shade_b, g_b, hdr_a, g_a, cam_a)
+    assert np.array_equal(again, hdr_b)
