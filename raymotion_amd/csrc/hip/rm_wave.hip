@@ -26,7 +26,16 @@ namespace {
 
 // kNee: the lane stands at a shading point and waits for k_persistent to run
 // its light sampling together with the other waiting lanes of its wave.
-enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2, kNee = 3 };
+// kDual: the lane's shadow ray and its next extension ray are both pending (the
+// continuation was sampled right after the light sample); kShadowEnd: only the
+// shadow ray is pending and the path ends after it.
+enum Phase : int { kIdle = 0, kExt = 1, kShadow = 2, kNee = 3, kDual = 4, kShadowEnd = 5 };
+
+// Index of the r-th (from 0) set bit of m; m has more than r set bits.
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int r) {
+    for (int i = 0; i < r; ++i) m &= m - 1ull;
+    return __ffsll((long long)m) - 1;
+}
 
 // Position for the calling lane (when keep) in a list grown through *count,
 // one atomic per wave; undefined without keep.
@@ -66,6 +75,11 @@ struct PathLane {
     float h_ar, h_ag, h_ab, h_rough, h_alpha2, h_met, h_spec;
     float nr, ng, nb, nsumW;
     uint32_t nM;
+    // shadow ray kept aside in phase kDual / kShadowEnd (it leaves from
+    // p + 0.002 pn); nr, ng, nb then hold the light's contribution already
+    // multiplied by the vertex's throughput
+    V3 sd{0.0f, 0.0f, 0.0f};
+    float sdist = 0.0f;
 };
 
 // Buffers a persistent pass reads or writes.
@@ -99,6 +113,16 @@ struct PathEngine {
     // Set by k_persistent: shade_hit leaves the light sampling to the loop
     // (phase kNee, step result kWait) instead of running it at once.
     bool defer_nee = false;
+    // Set by k_persistent for scenes whose shadow rays draw no random numbers
+    // (every surface opaque): after the light sample the continuation is
+    // sampled at once, so the shadow ray and the extension ray are both
+    // pending (phase kDual) and an idle lane of the wave can trace the shadow
+    // ray in the same traversal round as the extension. The RNG draws and the
+    // order of the radiance sums stay those of the reference's loop
+    // (direct_lighting's draws, then the BSDF sample, then roulette; the
+    // light's term is added before the next vertex's emission).
+    bool dual_shadow = false;
+    static constexpr bool kDualShadow = true;
     Tracer<S>& pt;
     const FrameParams& P;
     const GBufferSoA& gb;
@@ -395,6 +419,19 @@ struct PathEngine {
         L.ar += dlr * L.tr;
         L.ag += dlg * L.tg;
         L.ab += dlb * L.tb;
+        return continue_path(L);
+    }
+    // The light term of a kDual / kShadowEnd lane once its shadow ray is done.
+    __device__ void add_light(Lane& L, bool lit) {
+        L.ar += lit ? L.nr : 0.0f;
+        L.ag += lit ? L.ng : 0.0f;
+        L.ab += lit ? L.nb : 0.0f;
+    }
+    __device__ V3 shadow_origin(const Lane& L) const {
+        return V3{L.p.x + L.pn.x * 0.002f, L.p.y + L.pn.y * 0.002f, L.p.z + L.pn.z * 0.002f};
+    }
+    // BSDF sample at the vertex kept in L and the end of the bounce.
+    __device__ int continue_path(Lane& L) {
         const float wox = -L.d.x, woy = -L.d.y, woz = -L.d.z;
         const V3 pn = L.pn;
         float nx2, ny2, nz2;
@@ -495,6 +532,21 @@ struct PathEngine {
         L.nsumW = ns.sumW;
         L.nM = ns.M;
         if (ns.shadow) {
+            if (dual_shadow) {
+                L.nr *= L.tr;  // the products after_nee would add (dl * throughput of this vertex)
+                L.ng *= L.tg;
+                L.nb *= L.tb;
+                L.sd = ns.d;
+                L.sdist = ns.dist;
+                const int r = continue_path(L);
+                if (r == kTrace) {
+                    L.phase = kDual;
+                    return kTrace;
+                }
+                // no continuation: the shadow ray alone, then the path ends
+                set_ray(L, kShadowEnd, ns.o, ns.d, ns.dist);
+                return kTrace;
+            }
             set_ray(L, kShadow, ns.o, ns.d, ns.dist);
             return kTrace;
         }
@@ -517,6 +569,10 @@ struct PathEngine {
             }
         }
         if (L.phase == kShadow) return after_nee(L, h.tri < 0);
+        if (L.phase == kShadowEnd) {
+            add_light(L, h.tri < 0);
+            return kPathEnd;
+        }
         if (h.tri < 0) return kPathEnd;
         return shade_hit(L, pt.surface_at(h, L.ro, L.rd));
     }
@@ -544,7 +600,9 @@ struct PathEngine {
         const int r = finish_nee(L, ns);
         return r == kPathEnd ? next_sample(L) : r;
     }
-    __device__ bool any_hit(const Lane& L) const { return L.phase == kShadow && pt.sc.opaque; }
+    __device__ bool any_hit(const Lane& L) const {
+        return (L.phase == kShadow || L.phase == kShadowEnd) && pt.sc.opaque;
+    }
 };
 
 
@@ -589,6 +647,7 @@ struct GenEngine {
     // shadow rays: latency hiding wins over the spills (LDS allows 4 blocks at S = 32)
     static constexpr int kMinBlocks = S <= 32 ? 4 : S <= 48 ? 3 : 2;
     static constexpr bool kDeferNee = false;
+    static constexpr bool kDualShadow = false;
     static constexpr bool kFused = true;  // traversal inside the persistent loop, with refill
     // Leaf batches in that loop: when half the traversing lanes wait or
     // after 5 iterations (C1 generate 3.84-3.87 ms against 3.97-4.02 at 3;
@@ -829,6 +888,7 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
     pt.stage_lights();
     E eng(pt, P, io);
     if constexpr (E::kDeferNee) eng.defer_nee = true;
+    if constexpr (E::kDualShadow) eng.dual_shadow = sc.opaque != 0;
     int nee_age = 0;  // iterations the wave's waiting lanes have waited
     const unsigned lane = __lane_id();
     int band = int(blockIdx.x) % kBands, tried = 0;
@@ -944,6 +1004,64 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene s
                     }
                     tv = __ballot(trav);
                 } while (8 * __popcll(tv) > kTravExit8 * nwork);
+            }
+        } else if constexpr (E::kDualShadow) {
+            // A lane with both rays pending (kDual) hands its shadow ray to an
+            // idle lane of the wave when there is one (the k-th such lane to
+            // the k-th idle lane) and traces its extension ray meanwhile, so
+            // the shadow ray leaves the pixel's chain of dependent rays; idle
+            // lanes exist once the pixel queue is exhausted, in the tail that
+            // sets the end of the pass. Without a helper the lane traces its
+            // shadow ray first and the extension in the next iteration.
+            const unsigned long long D = __ballot(L.phase == kDual), I = __ballot(L.phase == kIdle);
+            int partner = -1;
+            bool helper = false;
+            if (D && I) {
+                const int pairs = min(__popcll(D), __popcll(I));
+                const unsigned long long below = (1ull << lane) - 1ull;
+                if (L.phase == kDual) {
+                    const int r = __popcll(D & below);
+                    if (r < pairs) partner = nth_set_bit(I, r);
+                } else if (L.phase == kIdle) {
+                    const int r = __popcll(I & below);
+                    if (r < pairs) {
+                        partner = nth_set_bit(D, r);
+                        helper = true;
+                    }
+                }
+            }
+            const int src = partner >= 0 ? partner : int(lane);
+            const bool dual = L.phase == kDual;
+            const V3 so = dual ? eng.shadow_origin(L) : V3{0.0f, 0.0f, 0.0f};
+            const V3 bo{__shfl(so.x, src), __shfl(so.y, src), __shfl(so.z, src)};
+            const V3 bd{__shfl(L.sd.x, src), __shfl(L.sd.y, src), __shfl(L.sd.z, src)};
+            const float bdist = __shfl(L.sdist, src);
+            V3 ro = L.ro, rd = L.rd;
+            float tmax = L.tmax;
+            bool any = eng.any_hit(L);
+            if (helper || (dual && partner < 0)) {
+                ro = bo;
+                rd = bd;
+                tmax = bdist;
+                any = true;
+            }
+            const bool tracing = helper || dual || L.phase == kExt || L.phase == kShadow || L.phase == kShadowEnd;
+            Hit h;
+            h.t = 0.0f;
+            h.tri = -1;
+            h.u = h.v = 0.0f;
+            if (tracing) h = pt.intersect(ro, rd, tmax, any);
+            const int occluded = __shfl(int(h.tri >= 0), src);
+            if (dual) {
+                eng.add_light(L, (partner >= 0 ? occluded : int(h.tri >= 0)) == 0);
+                L.phase = kExt;  // the extension ray stays in L.ro / L.rd
+                if (partner >= 0) {
+                    const int r = eng.step(L, h);
+                    if (r != kTrace && r != kWait) L.phase = kIdle;
+                }
+            } else if (tracing && !helper) {
+                const int r = eng.step(L, h);
+                if (r != kTrace && r != kWait) L.phase = kIdle;
             }
         } else if (L.phase == kExt || L.phase == kShadow) {
             const Hit h = pt.intersect(L.ro, L.rd, L.tmax, eng.any_hit(L));
