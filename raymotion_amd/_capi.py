@@ -13,7 +13,9 @@ from pathlib import Path
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
-LIB_PATH = PKG_DIR / "librm_hip.so"
+# RAYMOTION_HIP_LIB selects another build of the same library (the
+# host-sanitizer variant librm_hip_san.so).
+LIB_PATH = Path(os.environ.get("RAYMOTION_HIP_LIB") or PKG_DIR / "librm_hip.so")
 
 RM_OK = 0
 RM_ERR_NO_DEVICE = 1

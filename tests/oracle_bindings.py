@@ -12,7 +12,7 @@ import numpy as np
 from raymotion_amd._capi import GBUFFER_DTYPE, RESERVOIR_DTYPE, Frame, SceneDesc, ptr
 
 ROOT = Path(__file__).resolve().parent.parent
-ORC_PATH = ROOT / "oracle" / "liborc.so"
+ORC_PATH = Path(os.environ.get("RAYMOTION_ORACLE_LIB") or ROOT / "oracle" / "liborc.so")
 REF_PATH = ROOT / "oracle" / "_ref" / "libref.so"
 THREADS = min(16, os.cpu_count() or 1)
 
