@@ -499,29 +499,32 @@ WEIGHTS = ROOT / "raymotion_amd" / "data" / "kpn_mixed.npy"
 
 
 def run_denoiser_train(args) -> dict:
-    """Trains the learned denoiser: 512 views of random mixed scenes (256x256,
-    4-spp ReSTIR input, 4096-spp path-traced reference), Adam lr 1e-3 with
-    batch 16, then evaluates it against the a-trous path."""
+    """Trains the learned denoiser: 512 random scenes of the mixed-scene
+    generator, one 256x256 crop each (4-spp ReSTIR input, 4096-spp
+    path-traced reference), 20,000 Adam steps at lr 1e-3 and batch 16, then
+    evaluates it against the a-trous path."""
     from raymotion_amd import denoise
     backend = rm.Backend(0)
     backend.set_profiling(True)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     data_s = denoise.make_training_set(backend, args.views, size=256, noisy_spp=4, ref_spp=args.ref_spp,
-                                       views_per_scene=8, seed=0, subdivisions=4, log=log)
+                                       views_per_scene=1, seed=0, subdivisions=4, log=log)
     t0 = time.perf_counter()
     losses = denoise.train(backend, denoise.init_params(0), args.steps, batch=16, lr=1e-3, seed=1, log=log)
     train_s = time.perf_counter() - t0
     weights = backend.kpn_get_weights()
     if args.save_weights:
         np.save(args.save_weights, weights)
-    ev = denoise.evaluate(backend, frames=(4, 20), log=log)
+    ev = denoise.evaluate(backend, frames=(4, 20), svgf_spp=(4, 8, 16, 32, 64, 128, 256), cnn_spp=(1, 2, 4),
+                          log=log)
     backend.close()
     return {"metric": METRIC, "value": round(train_s / max(args.steps, 1) * 1e3, 3), "unit": "ms/training step",
             "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": round(train_s / max(args.steps, 1) * 1e3, 3),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
             "data": "synthetic (random mixed scenes, rendered by the backend)",
             "config": {"workload": "kernel-predicting denoiser training", "views": args.views, "crop": 256,
-                       "input_spp": 4, "reference_spp": args.ref_spp, "batch": 16, "optimizer": "Adam lr 1e-3",
+                       "scenes": args.views, "input_spp": 4, "reference_spp": args.ref_spp, "batch": 16,
+                       "optimizer": "Adam lr 1e-3",
                        "training_set_s": round(data_s, 1), "training_s": round(train_s, 1),
                        "loss_first100": float(np.mean(losses[:100])), "loss_last100": float(np.mean(losses[-100:]))},
             "denoiser": ev}
@@ -535,7 +538,7 @@ def denoiser_extras() -> dict | None:
         return None
     backend = rm.Backend(0)
     backend.kpn_set_weights(denoise.load_weights(WEIGHTS))
-    ev = denoise.evaluate(backend, frames=(4, 20), svgf_spp=(4, 8, 16, 32, 64))
+    ev = denoise.evaluate(backend, frames=(4, 20), svgf_spp=(4, 8, 16, 32, 64, 128, 256), cnn_spp=(1, 2, 4))
     backend.close()
     return ev
 

@@ -94,17 +94,19 @@ def random_view(rng) -> object:
 
 
 def make_training_set(backend, n_views: int, size: int = 256, noisy_spp: int = 4, ref_spp: int = 1024,
-                      views_per_scene: int = 8, seed: int = 0, subdivisions: int = 3, log=None):
+                      views_per_scene: int = 1, seed: int = 0, subdivisions: int = 4, log=None):
     """Renders n_views (features, noisy, reference) triples of random mixed
-    scenes into the backend's training set; returns seconds spent."""
+    scenes (a new scene every views_per_scene views; tree built on the GPU)
+    into the backend's training set; returns seconds spent. A size x size
+    frame of random_view's camera is the centre crop of the same camera's
+    1280x720 frame: the focal length is fixed in pixels."""
     rng = np.random.default_rng(seed)
     backend.kpn_dataset_clear()
     t0 = time.perf_counter()
     for v in range(n_views):
         if v % views_per_scene == 0:
             s = scenes.mixed_spheres(seed=int(rng.integers(1 << 30)), subdivisions=subdivisions)
-            s.build_bvh()
-            backend.set_scene(s)
+            backend.set_scene(s, gpu_tree=True, rebuild=True)
         cam = random_view(rng)
         feat, noisy = backend.render_features(restir_frame(size, size, noisy_spp, cam, denoise=0, temporal=0),
                                               int(rng.integers(1 << 62)))
@@ -140,11 +142,16 @@ def tonemapped_rmse(a, b) -> float:
 
 
 def evaluate(backend, frames=(4,), ref_spp: int = 4096, history: int = 4, svgf_spp=(4, 8, 16, 32, 64),
-             cnn_spp=(4,), width: int = 1280, height: int = 720, log=None) -> dict:
+             cnn_spp=(1, 2, 4), width: int = 1280, height: int = 720, log=None) -> dict:
     """RMSE against a ref_spp path-traced render and GPU ms/frame of the a-trous
     path (5 iterations) and the learned path on the mixed scene's orbit. Each
     evaluated frame f is the last of the sequence f - history .. f, so both
-    paths have temporal history; ms/frame is the mean over the sequence."""
+    paths have temporal history; ms/frame is the mean over the sequence.
+
+    equal_rmse gives, per learned-path sample count, the a-trous cost at the
+    same error (log-log interpolation over its sample counts) and the
+    speed-up; where the a-trous sweep never reaches that error the cost of its
+    largest sample count is a lower bound (marked)."""
     s = scenes.mixed_spheres(seed=0)
     s.build_bvh()
     backend.set_scene(s)
@@ -172,26 +179,23 @@ def evaluate(backend, frames=(4,), ref_spp: int = 4096, history: int = 4, svgf_s
 
     svgf = [run(spp, 1) for spp in svgf_spp]
     cnn = [run(spp, 2) for spp in cnn_spp]
-    out = {"frames": list(frames), "reference_spp": ref_spp, "svgf": svgf, "cnn": cnn}
-    # a-trous cost at the learned path's error: log-log interpolation over spp,
-    # for linear and for tonemapped RMSE
-    c = cnn[0]
+    out = {"frames": list(frames), "reference_spp": ref_spp, "svgf": svgf, "cnn": cnn, "equal_rmse": {}}
     t = np.array([r["ms"] for r in svgf])
-    out["equal_rmse"] = {}
     for key in ("rmse", "rmse_tonemapped"):
         e = np.array([r[key] for r in svgf])
-        ms_eq, bound = None, None
-        if c[key] >= e[0]:
-            ms_eq = float(t[0])  # already matched at the lowest sample count swept
-        for i in range(len(svgf) - 1):
-            if ms_eq is None and e[i] >= c[key] >= e[i + 1]:
-                a = (math.log(c[key]) - math.log(e[i])) / (math.log(e[i + 1]) - math.log(e[i]))
-                ms_eq = float(math.exp(math.log(t[i]) + a * (math.log(t[i + 1]) - math.log(t[i]))))
-        if ms_eq is None:
-            bound = float(t[-1])  # not reached within the sweep: its cost is a lower bound
-        ms = ms_eq if ms_eq is not None else bound
-        out["equal_rmse"][key] = {"svgf_ms": round(ms, 3), "cnn_ms": c["ms"], "speedup": round(ms / c["ms"], 3),
-                                  "lower_bound": ms_eq is None}
+        rows = []
+        for c in cnn:
+            ms_eq = None
+            if c[key] >= e[0]:
+                ms_eq = float(t[0])  # matched at the lowest a-trous sample count swept
+            for i in range(len(svgf) - 1):
+                if ms_eq is None and e[i] >= c[key] >= e[i + 1]:
+                    a = (math.log(c[key]) - math.log(e[i])) / (math.log(e[i + 1]) - math.log(e[i]))
+                    ms_eq = float(math.exp(math.log(t[i]) + a * (math.log(t[i + 1]) - math.log(t[i]))))
+            ms = ms_eq if ms_eq is not None else float(t[-1])
+            rows.append({"cnn_spp": c["spp"], "svgf_ms": round(ms, 3), "cnn_ms": c["ms"],
+                         "speedup": round(ms / c["ms"], 3), "lower_bound": ms_eq is None})
+        out["equal_rmse"][key] = rows
     return out
 
 

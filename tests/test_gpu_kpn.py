@@ -147,3 +147,24 @@ def test_learned_frame_feeds_the_next_temporal_pass(backend):
     g_b, _, _, shade_b = backend.read_restir(64, 64)
     again = backend.denoise_image(fb, shade_b, g_b, hdr_a, g_a, cam_a)
     assert np.array_equal(again, hdr_b)
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="trained weights not committed")
+def test_forward_matches_torch_full_size(backend):
+    """The trained network on a full 1280x720 frame of the mixed scene's
+    features (the inference size) against the PyTorch reference."""
+    import torch
+    torch.set_num_threads(16)
+    s = scenes.mixed_spheres(seed=0)
+    backend.set_scene(s, gpu_tree=True, rebuild=True)
+    p = denoise.load_weights(WEIGHTS)
+    backend.kpn_set_weights(p)
+    feat, noisy = backend.render_features(denoise.restir_frame(1280, 720, 4, scenes.mixed_camera(4), denoise=0,
+                                                               temporal=0), 12349)
+    g = backend.kpn_denoise(feat, noisy)
+    r = ref.forward(p, feat[None], noisy[None]).detach().numpy()[0]
+    err = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
+    print(f"kpn forward 1280x720: max rel err {err.max():.3g}, mean {err.mean():.3g}, "
+          f"99.99th pct {np.percentile(err, 99.99):.3g}")
+    assert err.max() <= 1e-2
+    assert err.mean() <= 2e-4
