@@ -223,11 +223,33 @@ int rm_denoise_image(rm_ctx* ctx, const rm_frame* frame, const float* color, con
  * sum_px d_hdr . hdr with respect to albedo r, g, b, roughness and emission
  * scale (5 doubles per group). tri_group gives each triangle's group (scene
  * order, -1 for none); a group's emission is scale * group_emission (3 floats
- * per group). Sampling, MIS weights and roulette are held fixed. The
+ * per group). Sampling, MIS weights and roulette are held fixed: with
+ * clamp_value = 0 and nomis = 1 that is the exact derivative of the expected
+ * image; with the per-sample clamp, or with the reference's MIS (whose weight
+ * at a BSDF-sampled light hit uses the previous vertex's one-candidate light
+ * estimate, so the weights do not sum to one), the derivatives of the
+ * roulette probabilities and of the weights are left out. The
  * differentiable counterpart of the reference's forward
  * (pathtrace.metal:552-769); it has no reference counterpart. */
 int rm_render_adjoint(rm_ctx* ctx, const rm_frame* frame, uint64_t seed, const float* d_hdr, const int32_t* tri_group,
                       const float* group_emission, int ngroup, double* grad);
+
+/* Material-table update for inverse rendering: rewrites, in the resident
+ * scene, the albedo (r, g, b), roughness and emission of every triangle with
+ * tri_group[i] >= 0 (ntri entries, scene order, -1: unchanged) from params
+ * (5 floats per group: albedo r, g, b, roughness, emission scale; a group's
+ * emission is the float product group_emission[g] * scale, 3 floats per
+ * group), rewrites the lights' emission the same way and installs the
+ * caller's light CDF (nlight floats, CDF order) and total. Afterwards the
+ * resident scene renders as rm_set_scene of the scene with those values
+ * would; geometry, tree, textures and every other material field stay. The
+ * first call for a tri_group builds a per-(group, material) table (O(ntri));
+ * later calls with the same tri_group send only the material and light tables
+ * and the CDF. ntri and nlight must equal the resident scene's. Drops the
+ * rendering history. The reference re-exports the whole scene for a material
+ * change (metal.mm:140-196); this entry point has no reference counterpart. */
+int rm_set_group_materials(rm_ctx* ctx, const int32_t* tri_group, size_t ntri, int ngroup, const float* params,
+                           const float* group_emission, const float* light_cdf, size_t nlight, float light_total);
 
 /* ---- Learned kernel-predicting denoiser (no reference counterpart; the
  * reference's denoiser is the a-trous filter, pathtrace.metal:1263-1354, which

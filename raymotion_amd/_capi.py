@@ -102,6 +102,7 @@ _SIGNATURES = [
     ("rm_restir_merge", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P, _P, _P, _P, _P]),
     ("rm_denoise_image", _I, [_P, ctypes.POINTER(Frame), _P, _P, _P, _P, _P, _P]),
     ("rm_render_adjoint", _I, [_P, ctypes.POINTER(Frame), ctypes.c_uint64, _P, _P, _P, _I, _P]),
+    ("rm_set_group_materials", _I, [_P, _P, _SZ, _I, _P, _P, _P, _SZ, ctypes.c_float]),
     ("rm_kpn_param_count", _SZ, []),
     ("rm_kpn_set_weights", _I, [_P, _P, _SZ]),
     ("rm_kpn_get_weights", _I, [_P, _P, _SZ]),

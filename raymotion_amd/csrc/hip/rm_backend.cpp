@@ -150,6 +150,14 @@ struct rm_ctx {
     DevBuf x_col, x_hcol, x_a, x_b;
     DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
     std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
+    // rm_set_group_materials: the material table with one slot per (group,
+    // packed material) pair, each group's slots, every light's group and the
+    // light records, built for the tri_group of key grp_key (0: none yet; any
+    // scene upload resets it).
+    uint64_t grp_key = 0;
+    std::vector<std::vector<int32_t>> grp_slots;
+    std::vector<F4> grp_mats, grp_lights;
+    std::vector<int32_t> grp_light;
 
     // Learned denoiser: parameters (fp32 master, bf16 copy), gradient, Adam
     // moments, flipped weights for the input gradients, activations of the
@@ -614,6 +622,7 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     c->light_total = ps.light_total;
     c->ntri = int32_t(s->ntri);
     c->keys = keys;
+    c->grp_key = 0;
     c->hist_key = history_key(keys, s->light_total);
     c->has_scene = true;
     c->last_refit = refit ? 1 : rebuild ? 2 : 0;
@@ -735,6 +744,7 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
     c->light_total = ps.light_total;
     c->ntri = n;
     c->keys = keys;
+    c->grp_key = 0;
     c->hist_key = history_key(keys, s->light_total);
     c->has_scene = true;
     c->last_refit = refit ? 1 : rebuild ? 3 : 0;
@@ -1303,6 +1313,90 @@ int rm_trace(rm_ctx* c, const float* origins, const float* dirs, const float* tm
     st.stack_depth = c->stack_need;
     if (c->profiling && hipEventElapsedTime(&st.ms[7], c->ev[0], c->ev[1]) != hipSuccess) st.ms[7] = 0.0f;
     c->stats = st;
+    return RM_OK;
+}
+
+int rm_set_group_materials(rm_ctx* c, const int32_t* tri_group, size_t ntri, int ngroup, const float* params,
+                           const float* group_emission, const float* light_cdf, size_t nlight, float light_total) {
+    if (!c) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    if (!c->has_scene) return c->fail(RM_ERR_NO_SCENE, "no scene uploaded (rm_set_scene)");
+    const PackedScene& ps = c->pack;
+    if (!tri_group || !params || !group_emission || ngroup < 1 || ngroup > (1 << 20) || ntri != size_t(c->ntri) ||
+        nlight != size_t(c->nlights) || (nlight && !light_cdf) || ps.matid.size() != ntri ||
+        ps.lights.size() != nlight * kLightWords)
+        return c->fail(RM_ERR_INVALID, "rm_set_group_materials: arguments do not match the resident scene");
+    uint64_t key = 0xCBF29CE484222325ull ^ (uint64_t(ngroup) << 32) ^ ntri;
+    for (size_t i = 0; i < ntri; ++i) {
+        if (tri_group[i] < -1 || tri_group[i] >= ngroup)
+            return c->fail(RM_ERR_INVALID, "rm_set_group_materials: group index out of range");
+        key = (key ^ uint32_t(tri_group[i])) * 0x100000001B3ull;
+    }
+    if (key == 0) key = 1;
+    if (key != c->grp_key) {
+        // One slot per (group, packed material): a group's slots are written
+        // in place by every later call with the same grouping.
+        c->grp_mats.assign(ps.mats.begin(), ps.mats.end());
+        c->grp_slots.assign(size_t(ngroup), {});
+        std::vector<int32_t> matid(ntri);
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> seen(static_cast<size_t>(ngroup));  // (packed id, slot)
+        for (size_t i = 0; i < ntri; ++i) {
+            const int g = tri_group[i], orig = ps.matid[i];
+            if (g < 0) {
+                matid[i] = orig;
+                continue;
+            }
+            int32_t slot = -1;
+            for (const auto& pr : seen[size_t(g)])
+                if (pr.first == orig) slot = pr.second;
+            if (slot < 0) {
+                slot = int32_t(c->grp_mats.size() / kMatWords);
+                for (int w = 0; w < kMatWords; ++w) c->grp_mats.push_back(ps.mats[size_t(orig) * kMatWords + w]);
+                seen[size_t(g)].push_back({orig, slot});
+                c->grp_slots[size_t(g)].push_back(slot);
+            }
+            matid[i] = slot;
+        }
+        c->grp_light.resize(nlight);
+        for (size_t k = 0; k < nlight; ++k) c->grp_light[k] = tri_group[c->light_tri[k]];
+        c->grp_lights.assign(ps.lights.begin(), ps.lights.end());
+        if (!upload(c, c->matid, matid)) return c->fail(RM_ERR_ALLOC, "HIP material buffer allocation failed");
+        if (int rc = check(c, hipStreamSynchronize(c->stream), "material upload")) return rc;
+        c->grp_key = key;
+    }
+    // Material rows [ar ag ab alpha] [er eg eb ior] [rough metallic shader texoff] [...] and
+    // light records [...] [...] [...] [er eg eb alpha] of the groups.
+    for (int g = 0; g < ngroup; ++g) {
+        const float* p = params + 5 * size_t(g);
+        const float er = group_emission[3 * g] * p[4], eg = group_emission[3 * g + 1] * p[4],
+                    eb = group_emission[3 * g + 2] * p[4];
+        for (int32_t slot : c->grp_slots[size_t(g)]) {
+            F4* m = &c->grp_mats[size_t(slot) * kMatWords];
+            m[0].x = p[0]; m[0].y = p[1]; m[0].z = p[2];
+            m[1].x = er; m[1].y = eg; m[1].z = eb;
+            m[2].x = p[3];
+        }
+    }
+    for (size_t k = 0; k < nlight; ++k) {
+        const int g = c->grp_light[k];
+        if (g < 0) continue;
+        const float* p = params + 5 * size_t(g);
+        F4& e = c->grp_lights[4 * k + 3];
+        e.x = group_emission[3 * g] * p[4];
+        e.y = group_emission[3 * g + 1] * p[4];
+        e.z = group_emission[3 * g + 2] * p[4];
+    }
+    const std::vector<float> cdf(light_cdf, light_cdf + nlight);
+    if (!upload(c, c->mats, c->grp_mats) || !upload(c, c->lights, c->grp_lights) || !upload(c, c->cdf, cdf))
+        return c->fail(RM_ERR_ALLOC, "HIP material buffer allocation failed");
+    if (int rc = check(c, hipStreamSynchronize(c->stream), "material upload")) return rc;
+    c->light_total = light_total;
+    // The resident content no longer matches the last scene's keys: a later
+    // rm_set_scene re-sends materials and lights, and no history carries over.
+    c->keys.material = c->keys.lights = ~0ull;
+    c->drop_history();
+    c->err.clear();
     return RM_OK;
 }
 

@@ -19,14 +19,25 @@ PARAM_NAMES = ("albedo_r", "albedo_g", "albedo_b", "roughness", "emission_scale"
 NPARAM = len(PARAM_NAMES)
 
 
-def path_frame(width: int, height: int, spp: int, bounces: int, camera, clamp: float = 16.0) -> Frame:
+def path_frame(width: int, height: int, spp: int, bounces: int, camera, clamp: float = 16.0,
+               mis: bool = True) -> Frame:
     """A frame that takes the path-tracing path the adjoint differentiates:
-    one light sample per vertex, no spatial reuse, no adaptive stop."""
+    one light sample per vertex, no spatial reuse, no adaptive stop.
+
+    The adjoint holds the sample's discrete decisions (roulette, light pick,
+    lobe choice) and its MIS weights fixed. That is the exact derivative of
+    the expected image when the estimator is linear in those decisions:
+    clamp=0 (no per-sample clamp) and mis=False (light sampling only). With
+    the default per-sample clamp the clamped value of a path depends on the
+    roulette outcome, and the reference's MIS weight for a BSDF-sampled light
+    hit uses the previous vertex's one-candidate light estimate, not the
+    light pdf, so the weights do not sum to one; the derivatives of those
+    probabilities and weights are then left out of the gradient."""
     f = Frame()
     f.width, f.height, f.spp, f.bounces = width, height, spp, bounces
     f.resolution, f.clamp_value = 1.0, clamp
     f.adapt_min, f.adapt_step, f.adapt_rel, f.adapt_abs = 0, 8, 0.02, 0.002
-    f.nomis = 0
+    f.nomis = 0 if mis else 1
     f.restir_candidates, f.restir_spatial, f.restir_mcap, f.restir_radius = 1, 0, 128, 16.0
     f.denoise, f.denoise_iterations, f.temporal = 0, 2, 0
     f.camera = camera
@@ -77,6 +88,20 @@ class DiffScene:
         p[:, 4] = np.maximum(p[:, 4], 1e-4)
         return p
 
+    def light_table(self, params):
+        """(CDF float32, total) of the lights at these parameters."""
+        p = self.clamp(params)
+        lt = self.src.light_tri
+        g = self.tri_group[lt]
+        em = np.array([self.src.tris[k][lt] for k in ("er", "eg", "eb")], np.float32).T
+        sel = g >= 0
+        em[sel] = self.group_emission[g[sel]] * p[g[sel], 4:5].astype(np.float32)
+        lum = 0.2126 * em[:, 0].astype(np.float64) + 0.7152 * em[:, 1] + 0.0722 * em[:, 2]
+        w = self.light_area * np.maximum(lum, 1e-6)
+        cdf = np.cumsum(w)
+        total = float(cdf[-1]) if len(cdf) else 0.0
+        return cdf.astype(np.float32), float(np.float32(total))
+
     def exported(self, params) -> Exported:
         p = self.clamp(params)
         tris = self.src.tris.copy()
@@ -91,30 +116,43 @@ class DiffScene:
         tris["er"][sel] = em[:, 0]
         tris["eg"][sel] = em[:, 1]
         tris["eb"][sel] = em[:, 2]
-        lt = self.src.light_tri
-        lum = 0.2126 * tris["er"][lt].astype(np.float64) + 0.7152 * tris["eg"][lt] + 0.0722 * tris["eb"][lt]
-        w = self.light_area * np.maximum(lum, 1e-6)
-        cdf = np.cumsum(w)
-        total = float(cdf[-1]) if len(cdf) else 0.0
-        return Exported(tris, self.src.nodes, self.src.bvh_tri, lt, cdf.astype(np.float32), float(np.float32(total)),
-                        self.src.texels)
+        cdf, total = self.light_table(p)
+        return Exported(tris, self.src.nodes, self.src.bvh_tri, self.src.light_tri, cdf, total, self.src.texels)
 
 
 @dataclass
 class Renderer:
-    """Forward and adjoint renders of one DiffScene at a fixed frame."""
+    """Forward and adjoint renders of one DiffScene at a fixed frame.
+
+    The first render uploads the whole scene; later ones, while that upload is
+    still the backend's resident scene, rewrite only the grouped materials,
+    the light emission and the light CDF (rm_set_group_materials), which
+    renders exactly as a full upload of scene.exported(params) would.
+    full_upload=True re-sends the whole scene every time."""
     backend: Backend
     scene: DiffScene
     frame: Frame
+    full_upload: bool = False
+    _serial: int = -1
+
+    def load(self, params):
+        """Makes the backend's resident scene scene.exported(params)."""
+        if self.full_upload or self._serial != self.backend.scene_serial:
+            self.backend.set_scene(self.scene.exported(params))
+            self._serial = self.backend.scene_serial
+            return
+        p = self.scene.clamp(params)
+        cdf, total = self.scene.light_table(p)
+        self.backend.set_group_materials(self.scene.tri_group, p, self.scene.group_emission, cdf, total)
 
     def forward(self, params, seed: int) -> np.ndarray:
-        self.backend.set_scene(self.scene.exported(params))
+        self.load(params)
         _, hdr = self.backend.render(self.frame, seed, want_hdr=True)
         return hdr.reshape(self.frame.height, self.frame.width, 3)
 
     def adjoint(self, params, seed: int, d_hdr) -> np.ndarray:
         """(ngroup, 5) gradient of sum(d_hdr * forward(params, seed))."""
-        self.backend.set_scene(self.scene.exported(params))
+        self.load(params)
         return self.backend.render_adjoint(self.frame, seed, d_hdr, self.scene.tri_group, self.scene.group_emission)
 
 

@@ -155,6 +155,9 @@ class Backend:
         if rc:
             raise BackendError(rc, lib().rm_last_error(None).decode())
         self._h = h.value
+        # Counts scene uploads, so a caller can tell whether the scene it
+        # uploaded is still the resident one.
+        self.scene_serial = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -172,6 +175,7 @@ class Backend:
         """Uploads a scene. With gpu_tree the backend builds its traversal tree
         on the GPU (the scene needs no BVH; `rebuild` forces a new build,
         otherwise a resident GPU tree is refit while the triangle count stays)."""
+        self.scene_serial += 1
         if isinstance(scene, Scene):
             if gpu_tree:
                 self._check(lib().rm_scene_upload_gpu_tree(self._h, scene.handle, int(rebuild)),
@@ -216,6 +220,19 @@ class Backend:
         self._check(lib().rm_render_adjoint(self._h, ctypes.byref(frame), seed, ptr(d), ptr(tg), ptr(ge), len(ge),
                                             ptr(grad)), "rm_render_adjoint")
         return grad
+
+    def set_group_materials(self, tri_group, params, group_emission, light_cdf, light_total: float):
+        """Rewrites the grouped triangles' albedo, roughness and emission
+        (params (ngroup, 5): albedo r, g, b, roughness, emission scale) and the
+        light CDF of the resident scene without re-sending it."""
+        tg = np.ascontiguousarray(tri_group, np.int32)
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 5)
+        ge = np.ascontiguousarray(group_emission, np.float32).reshape(-1, 3)
+        cdf = np.ascontiguousarray(light_cdf, np.float32)
+        if len(p) != len(ge):
+            raise ValueError("params and group_emission need one row per group")
+        self._check(lib().rm_set_group_materials(self._h, ptr(tg), tg.size, len(p), ptr(p), ptr(ge), ptr(cdf),
+                                                 cdf.size, float(light_total)), "rm_set_group_materials")
 
     # ---- learned denoiser
     def kpn_set_weights(self, params):

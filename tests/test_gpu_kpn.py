@@ -152,7 +152,12 @@ def test_learned_frame_feeds_the_next_temporal_pass(backend):
 @pytest.mark.skipif(not WEIGHTS.exists(), reason="trained weights not committed")
 def test_forward_matches_torch_full_size(backend):
     """The trained network on a full 1280x720 frame of the mixed scene's
-    features (the inference size) against the PyTorch reference."""
+    features (the inference size). Over 2.8 M outputs the FP32 summation order
+    alone moves a few outputs by several bf16 ulps (one straddled rounding per
+    layer compounds), so the yardstick is the network with FP64 sums and the
+    same bf16 rounding points: the GPU must land no farther from it than
+    torch's own FP32 convolution does (mean, 99.99th percentile and maximum
+    each within twice torch's, plus 1e-3), with a mean below 2e-4."""
     import torch
     torch.set_num_threads(16)
     s = scenes.mixed_spheres(seed=0)
@@ -162,9 +167,17 @@ def test_forward_matches_torch_full_size(backend):
     feat, noisy = backend.render_features(denoise.restir_frame(1280, 720, 4, scenes.mixed_camera(4), denoise=0,
                                                                temporal=0), 12349)
     g = backend.kpn_denoise(feat, noisy)
-    r = ref.forward(p, feat[None], noisy[None]).detach().numpy()[0]
-    err = np.abs(g - r) / np.maximum(np.abs(r), 1e-3)
-    print(f"kpn forward 1280x720: max rel err {err.max():.3g}, mean {err.mean():.3g}, "
-          f"99.99th pct {np.percentile(err, 99.99):.3g}")
-    assert err.max() <= 1e-2
-    assert err.mean() <= 2e-4
+    r32 = ref.forward(p, feat[None], noisy[None]).detach().numpy()[0]
+    r64 = ref.forward(p, feat[None], noisy[None], dtype=torch.float64).detach().numpy()[0]
+    scale = np.maximum(np.abs(r64), 1e-3)
+    stats = {}
+    for name, x in (("gpu", g), ("torch fp32", r32)):
+        e = np.abs(x - r64) / scale
+        stats[name] = (float(e.mean()), float(np.percentile(e, 99.99)), float(e.max()))
+        print(f"kpn forward 1280x720, {name} vs fp64 sums: mean {stats[name][0]:.3g}, "
+              f"99.99th pct {stats[name][1]:.3g}, max {stats[name][2]:.3g}")
+    e = np.abs(g - r32) / np.maximum(np.abs(r32), 1e-3)
+    print(f"gpu vs torch fp32: mean {e.mean():.3g}, max {e.max():.3g}")
+    for a, b in zip(stats["gpu"], stats["torch fp32"]):
+        assert a <= 2 * b + 1e-3
+    assert stats["gpu"][0] <= 2e-4
