@@ -5,6 +5,7 @@
 // texture is attached only when all three corners carry UVs.
 #include "raymotion/obj.hpp"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <filesystem>
@@ -13,6 +14,8 @@
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
+
+#include "png.h"
 
 namespace raymotion {
 namespace {
@@ -35,12 +38,11 @@ std::string read_file(const std::string& path) {
 double srgb_to_linear(double c) { return c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4); }
 
 // Binary (P6) or ASCII (P3) PPM, 8-bit.
-std::shared_ptr<Texture> load_ppm(const std::string& path) {
-    const std::string data = read_file(path);
+std::shared_ptr<Texture> load_ppm(const std::string& data, const std::string& path) {
     std::istringstream in(data);
     std::string magic;
     in >> magic;
-    if (magic != "P6" && magic != "P3") throw std::runtime_error("unsupported texture format (PPM only): " + path);
+    if (magic != "P6" && magic != "P3") throw std::runtime_error("unsupported texture format (PNG or PPM): " + path);
     auto skip = [&] {
         in >> std::ws;
         while (in.peek() == '#') {
@@ -76,8 +78,34 @@ std::shared_ptr<Texture> load_ppm(const std::string& path) {
     return tex;
 }
 
+// A map_Kd image: PNG (any colour type, reduced to 8-bit RGB as stb_image
+// does for the reference) or PPM; texels are the sRGB-decoded bytes.
+std::shared_ptr<Texture> load_texture(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot load texture: " + path);
+    std::ostringstream s;
+    s << f.rdbuf();
+    const std::string data = s.str();
+    if (!is_png(data)) return load_ppm(data, path);
+    Rgb8Image img;
+    try {
+        img = decode_png(data);
+    } catch (const std::exception& e) {
+        throw std::runtime_error("cannot load texture: " + path + " (" + e.what() + ")");
+    }
+    auto tex = std::make_shared<Texture>();
+    tex->width = img.width;
+    tex->height = img.height;
+    tex->pixels.resize(size_t(img.width) * size_t(img.height));
+    for (size_t i = 0; i < tex->pixels.size(); ++i)
+        tex->pixels[i] = {srgb_to_linear(img.rgb[3 * i] / 255.0), srgb_to_linear(img.rgb[3 * i + 1] / 255.0),
+                          srgb_to_linear(img.rgb[3 * i + 2] / 255.0)};
+    return tex;
+}
+
 Materials parse_mtl(const std::string& path) {
     Materials out;
+    std::unordered_map<std::string, std::shared_ptr<const Texture>> textures;
     std::istringstream in(read_file(path));
     const std::filesystem::path dir = std::filesystem::path(path).parent_path();
     std::string line;
@@ -105,10 +133,19 @@ Materials parse_mtl(const std::string& path) {
         } else if (key == "Pm") {
             row >> cur->metallic;
         } else if (key == "map_Kd") {
+            // One path, optionally quoted, backslashes read as separators
+            // (src/engine/obj.cpp:108-113); each file is decoded once per MTL.
             std::string file;
             std::getline(row >> std::ws, file);
             while (!file.empty() && (file.back() == '\r' || file.back() == ' ')) file.pop_back();
-            cur->texture = load_ppm((dir / file).string());
+            if (file.empty() || file[0] == '-')
+                throw std::runtime_error("map_Kd requires a texture path (options unsupported): " + path);
+            if (file.size() > 1 && file.front() == '"' && file.back() == '"') file = file.substr(1, file.size() - 2);
+            std::replace(file.begin(), file.end(), '\\', '/');
+            const std::string full = (dir / file).lexically_normal().string();
+            auto found = textures.find(full);
+            if (found == textures.end()) found = textures.emplace(full, load_texture(full)).first;
+            cur->texture = found->second;
         }
     }
     return out;

@@ -85,8 +85,8 @@ def _gxx(*args, out):
 
 def test_runtime_semantics_cpp(tmp_path):
     exe = tmp_path / "test_runtime"
-    _gxx(ROOT / "tests" / "cpp" / "test_runtime.cpp", *(ENGINE / f for f in ("scene.cpp", "bvh.cpp", "camera.cpp",
-                                                                              "obj.cpp")), out=exe)
+    _gxx(ROOT / "tests" / "cpp" / "test_runtime.cpp",
+         *(ENGINE / f for f in ("scene.cpp", "bvh.cpp", "camera.cpp", "obj.cpp", "png.cpp")), out=exe)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert "runtime checks passed" in r.stdout
@@ -129,7 +129,7 @@ int main(int, char** argv) {
     return ok ? 0 : 1;
 }''')
     exe = tmp_path / "obj_test"
-    _gxx(src, ENGINE / "obj.cpp", ENGINE / "scene.cpp", out=exe)
+    _gxx(src, ENGINE / "obj.cpp", ENGINE / "png.cpp", ENGINE / "scene.cpp", out=exe)
     r = subprocess.run([str(exe), str(tmp_path / "model.obj"), str(tmp_path / "m" / "model.mtl")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
