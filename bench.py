@@ -8,7 +8,9 @@ seed 0) at 1280x720, 16 spp, max depth 8, FP32, through the ReSTIR DI path
 edge-aware denoiser with 5 a-trous passes; the camera orbits at radius 10 over
 34 frames with a static BVH, frame f renders with seed 12345 + f. The scene is
 resident on the GPU before the timed region; each frame is one synchronous
-rm_render call including the 2.7 MB RGB8 readback.
+rm_render call including the 2.7 MB RGB8 readback. The same frames are then
+timed through the drop-in boundary as well (render_image_hip's per-call
+scene marshal and keying, then the render): boundary_ms_per_frame.
 
 Prints one JSON line: Mrays/s (rays actually traced, counted on the GPU),
 ms/frame, the roofline of the dominant kernel measured with HIP events, and
@@ -664,6 +666,18 @@ def run_mixed(args) -> dict:
         prays.append([st.rays[i] for i in range(6)])
         if st.path != 1:
             raise RuntimeError("expected the ReSTIR + denoise path")
+    # The same frames through the drop-in boundary: render_image_hip marshals
+    # the Scene to 176-byte records and re-keys it every call (unchanged here,
+    # so nothing is re-sent), then renders.
+    boundary, resent = [], 0
+    if not args.host_tree:
+        backend.set_scene(scene, gpu_tree=True)  # untimed: the first marshal allocates its staging
+        for k in range(args.warmup, args.warmup + args.steps):
+            t0 = time.perf_counter()
+            backend.set_scene(scene, gpu_tree=True)
+            backend.render(frame_settings(k), 12345 + k, want_hdr=False)
+            boundary.append((time.perf_counter() - t0) * 1e3)
+            resent += backend.stats().refit != 0
     backend.close()
 
     ms_per_frame = float(np.mean(wall))
@@ -702,6 +716,10 @@ def run_mixed(args) -> dict:
                    "pass_nodes_per_ray": {tracing[i]: round(float(nd[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "pass_tris_per_ray": {tracing[i]: round(float(tr[i] / max(ry[i], 1.0)), 2) for i in tracing},
                    "gpu_frame_ms": round(float(pm[7]), 3),
+                   "boundary_ms_per_frame": round(float(np.mean(boundary)), 3) if boundary else None,
+                   "boundary_mrays_per_s": round(rays_per_frame / (float(np.mean(boundary)) / 1e3) / 1e6, 2)
+                   if boundary else None,
+                   "boundary_frames_resending_the_scene": resent if boundary else None,
                    "tree": "host SAH (the scene's own)" if args.host_tree else "built on the GPU",
                    "tree_nodes": tree_shape[0], "tree_stack_depth": tree_shape[1],
                    "schedule": {0: "persistent", 1: "wavefront", 2: "wavefront-only"}[args.schedule],

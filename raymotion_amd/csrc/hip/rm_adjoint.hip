@@ -4,9 +4,13 @@
 //
 // Sampling is detached: pdfs, MIS weights and roulette probabilities are held
 // constant, and only the integrand's BSDF values and emission are
-// differentiated. Every sample is replayed from its saved RNG state (nothing
-// is stored per path): a primal pass gives the sample's radiance C, then a
-// second pass re-traces the same path, and at each vertex
+// differentiated. Every sample is replayed from its saved RNG state (no path
+// vertices are stored; per sample stream the round in flight keeps 32 bytes):
+// a primal pass gives the sample's radiance C and, in one 64-bit word, the
+// outcomes of its light samples' shadow tests; a second pass re-traces the
+// same path without re-tracing those shadow rays (on scenes whose surfaces
+// are all opaque a shadow ray draws no random numbers, so skipping it keeps
+// the replay's draws; elsewhere it is traced again), and at each vertex
 //   emission      dC/d scale   = thr * base_emission * w
 //   NEE           dC/d albedo  = thr * Le * k * df/d albedo      (at the light)
 //                 dC/d rough   = thr * Le * k * df/d alpha2 * 4 r^3
@@ -28,6 +32,16 @@ struct AdjointHooks {
     const AdjointArgs& A;
     double* g;     // LDS: 5 per group
     V3 C, dC;      // primal radiance of the sample and dL/dC
+    uint64_t vis;  // the primal's shadow-test outcomes, one bit per light sample
+    bool reuse_vis;
+    int nvis = 0;
+
+    template <class Tr, class NS>
+    __device__ __forceinline__ bool visible(Tr& t, const NS& ns) {
+        const int i = nvis++;
+        if (reuse_vis && i < 64) return ((vis >> i) & 1u) != 0;
+        return ns.shadow && !t.occluded(ns.o, ns.d, ns.dist);
+    }
 
     __device__ void add(int grp, int k, float v) {
         if (v != 0.0f) atomicAdd(&g[grp * 5 + k], double(v));
@@ -98,9 +112,45 @@ struct AdjointHooks {
     }
 };
 
+// The sample streams of render_paths_simd (pathtrace.metal:773-815): lane l
+// of a pixel draws from pixel_seed ^ l * 0x94D049BB133111EB and takes the
+// pixel's samples l, l + lanes, ... One round r runs sample r * lanes + l of
+// every stream as two launches: the primal kernel traces it and stores its
+// radiance, start and end RNG states and shadow-test outcomes; the replay
+// kernel re-traces it from the start state with the gradient hooks. Each
+// kernel holds one path loop, so neither carries the other's registers.
+__device__ __forceinline__ uint64_t stream_seed(const FrameParams& P, uint32_t x, uint32_t y, uint32_t lane,
+                                                int lanes) {
+    uint64_t s = pixel_seed(P.seed, x, y);
+    if (lanes > 1) s ^= uint64_t(lane) * 0x94D049BB133111EBULL;
+    return s;
+}
+
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_adjoint(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
-                                                     unsigned long long* cnt) {
+__global__ __launch_bounds__(kBlock) void k_adjoint_primal(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
+                                                            int round, unsigned long long* cnt) {
+    __shared__ uint32_t stack[S * kBlock];
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t pixel = gid / uint32_t(lanes), lane = gid % uint32_t(lanes);
+    if (pixel >= uint32_t(P.width) * uint32_t(P.height)) return;
+    if (round * lanes + int(lane) >= P.spp) return;
+    const uint32_t x = pixel % uint32_t(P.width), y = pixel / uint32_t(P.width);
+    Tracer<S> pt(sc, P, stack + threadIdx.x);
+    if (round == 0) pt.rng.seed(stream_seed(P, x, y, lane, lanes));
+    else pt.rng.state = A.record[gid].y;
+    const uint32_t start = pt.rng.state;
+    float sx, sy;
+    pt.jitter(x, y, sx, sy);
+    VisibilityRecorder rec;
+    const V3 c = pt.pathtrace_t({P.cam.x, P.cam.y, P.cam.z}, pt.primary_dir(sx, sy, false), P.bounces, rec);
+    A.primal[gid] = make_float4(c.x, c.y, c.z, 0.0f);
+    A.record[gid] = make_uint4(start, pt.rng.state, uint32_t(rec.mask), uint32_t(rec.mask >> 32));
+    pt.flush(cnt, kCntProbe);
+}
+
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_adjoint_replay(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
+                                                            int round, unsigned long long* cnt) {
     __shared__ uint32_t stack[S * kBlock];
     // The block's gradient table, 5 per group, sized at launch to the scene's
     // groups (640 B at 16 groups instead of 5 KiB for the largest count).
@@ -109,37 +159,27 @@ __global__ __launch_bounds__(kBlock) void k_adjoint(DevScene sc, FrameParams P, 
     __syncthreads();
     const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pixel = gid / uint32_t(lanes), lane = gid % uint32_t(lanes);
-    const uint32_t npx = uint32_t(P.width) * uint32_t(P.height);
-    if (pixel < npx) {
+    if (pixel < uint32_t(P.width) * uint32_t(P.height) && round * lanes + int(lane) < P.spp) {
         const uint32_t x = pixel % uint32_t(P.width), y = pixel / uint32_t(P.width);
         Tracer<S> pt(sc, P, stack + threadIdx.x);
-        uint64_t s = pixel_seed(P.seed, x, y);
-        if (lanes > 1) s ^= uint64_t(lane) * 0x94D049BB133111EBULL;  // render_paths_simd's lane streams
-        pt.rng.seed(s);
+        const uint4 r = A.record[gid];
+        const float4 c4 = A.primal[gid];
+        const V3 c{c4.x, c4.y, c4.z};
+        pt.rng.state = r.x;
         const float fn = float(P.spp);
-        const V3 dI{A.d_img[3 * pixel] / fn, A.d_img[3 * pixel + 1] / fn, A.d_img[3 * pixel + 2] / fn};
-        const V3 cam{P.cam.x, P.cam.y, P.cam.z};
-        uint32_t bad = 0;
-        for (int k = int(lane); k < P.spp; k += lanes) {
-            const uint32_t st0 = pt.rng.state;
-            float sx, sy;
-            pt.jitter(x, y, sx, sy);
-            const V3 c = pt.sample_sensor(sx, sy);
-            V3 d = dI;
-            if (P.clamp_value > 0.0f) {  // min(c, clamp) passes no gradient where it clamps
-                if (c.x > P.clamp_value) d.x = 0.0f;
-                if (c.y > P.clamp_value) d.y = 0.0f;
-                if (c.z > P.clamp_value) d.z = 0.0f;
-            }
-            pt.rng.state = st0;
-            pt.jitter(x, y, sx, sy);
-            AdjointHooks hk{sc, A, grad, c, d};
-            const V3 c2 = pt.pathtrace_t(cam, pt.primary_dir(sx, sy, false), P.bounces, hk);
-            if (__float_as_uint(c2.x) != __float_as_uint(c.x) || __float_as_uint(c2.y) != __float_as_uint(c.y) ||
-                __float_as_uint(c2.z) != __float_as_uint(c.z))
-                ++bad;
+        V3 d{A.d_img[3 * pixel] / fn, A.d_img[3 * pixel + 1] / fn, A.d_img[3 * pixel + 2] / fn};
+        if (P.clamp_value > 0.0f) {  // min(c, clamp) passes no gradient where it clamps
+            if (c.x > P.clamp_value) d.x = 0.0f;
+            if (c.y > P.clamp_value) d.y = 0.0f;
+            if (c.z > P.clamp_value) d.z = 0.0f;
         }
-        if (bad) atomicAdd(A.mismatch, (unsigned long long)bad);
+        float sx, sy;
+        pt.jitter(x, y, sx, sy);
+        AdjointHooks hk{sc, A, grad, c, d, uint64_t(r.z) | (uint64_t(r.w) << 32), sc.opaque != 0};
+        const V3 c2 = pt.pathtrace_t({P.cam.x, P.cam.y, P.cam.z}, pt.primary_dir(sx, sy, false), P.bounces, hk);
+        if (__float_as_uint(c2.x) != __float_as_uint(c.x) || __float_as_uint(c2.y) != __float_as_uint(c.y) ||
+            __float_as_uint(c2.z) != __float_as_uint(c.z) || pt.rng.state != r.y)
+            atomicAdd(A.mismatch, 1ull);
         pt.flush(cnt, kCntProbe);
     }
     __syncthreads();
@@ -153,9 +193,13 @@ void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams&
                     unsigned long long* cnt, hipStream_t st) {
     const size_t threads = size_t(P.width) * size_t(P.height) * size_t(lanes);
     const dim3 grid(unsigned((threads + kBlock - 1) / kBlock)), block(kBlock);
+    const int rounds = (P.spp + lanes - 1) / lanes;
     with_stack(stack, [&](auto cap) {
-        hipLaunchKernelGGL((k_adjoint<decltype(cap)::value>), grid, block, size_t(A.ngroup) * 5 * sizeof(double), st,
-                           sc, P, A, lanes, cnt);
+        for (int r = 0; r < rounds; ++r) {
+            hipLaunchKernelGGL((k_adjoint_primal<decltype(cap)::value>), grid, block, 0, st, sc, P, A, lanes, r, cnt);
+            hipLaunchKernelGGL((k_adjoint_replay<decltype(cap)::value>), grid, block,
+                               size_t(A.ngroup) * 5 * sizeof(double), st, sc, P, A, lanes, r, cnt);
+        }
     });
 }
 

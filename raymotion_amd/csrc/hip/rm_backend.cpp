@@ -148,7 +148,7 @@ struct rm_ctx {
     GBufSet x_gbuf, x_hgbuf;
     ResSet x_rinit, x_rout, x_hres;
     DevBuf x_col, x_hcol, x_a, x_b;
-    DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad;
+    DevBuf adj_img, adj_tri_group, adj_light_group, adj_base, adj_grad, adj_bad, adj_primal, adj_record;
     std::vector<int32_t> light_tri;  // resident lights' triangles, CDF order
     // rm_set_group_materials: the material table with one slot per (group,
     // packed material) pair, each group's slots, every light's group and the
@@ -1499,7 +1499,9 @@ int rm_render_adjoint(rm_ctx* c, const rm_frame* f, uint64_t seed, const float* 
     hipStream_t st = c->stream;
     if (!c->adj_img.ensure(px * 3 * sizeof(float)) || !c->adj_tri_group.ensure(size_t(c->ntri) * 4) ||
         !c->adj_light_group.ensure(lg.size() * 4) || !c->adj_base.ensure(size_t(ngroup) * 3 * sizeof(float)) ||
-        !c->adj_grad.ensure(size_t(ngroup) * 5 * sizeof(double)) || !c->adj_bad.ensure(sizeof(unsigned long long)))
+        !c->adj_grad.ensure(size_t(ngroup) * 5 * sizeof(double)) || !c->adj_bad.ensure(sizeof(unsigned long long)) ||
+        !c->adj_primal.ensure(px * size_t(lanes) * sizeof(float4)) ||
+        !c->adj_record.ensure(px * size_t(lanes) * sizeof(uint4)))
         return c->fail(RM_ERR_ALLOC, "HIP adjoint buffer allocation failed");
     unsigned long long* cnt = c->counters.as<unsigned long long>();
     auto h2d = [&](DevBuf& b, const void* src, size_t bytes) {
@@ -1520,6 +1522,8 @@ int rm_render_adjoint(rm_ctx* c, const rm_frame* f, uint64_t seed, const float* 
     A.ngroup = ngroup;
     A.grad = c->adj_grad.as<double>();
     A.mismatch = c->adj_bad.as<unsigned long long>();
+    A.primal = c->adj_primal.as<float4>();
+    A.record = c->adj_record.as<uint4>();
     const DevScene sc = c->dev_scene();
     if (c->profiling) (void)hipEventRecord(c->ev[0], st);
     launch_adjoint(c->stack_need, lanes, sc, P, A, cnt, st);

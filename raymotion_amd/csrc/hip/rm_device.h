@@ -122,6 +122,7 @@ struct BsdfGrad {
 };
 
 // Hooks of the bounce loop that the forward leaves empty (pathtrace_t).
+// visible() answers a vertex's light-sample shadow test; the forward traces it.
 struct NoAdjoint {
     template <class... T>
     __device__ __forceinline__ void emission(const T&...) {}
@@ -131,6 +132,24 @@ struct NoAdjoint {
     __device__ __forceinline__ void scatter(const T&...) {}
     template <class... T>
     __device__ __forceinline__ void dielectric(const T&...) {}
+    template <class Tr, class NS>
+    __device__ __forceinline__ bool visible(Tr& t, const NS& ns) {
+        return ns.shadow && !t.occluded(ns.o, ns.d, ns.dist);
+    }
+};
+
+// The forward's hooks plus a record of the first 64 shadow-test outcomes of
+// the path, one bit per vertex with a light sample (the adjoint's primal pass).
+struct VisibilityRecorder : NoAdjoint {
+    uint64_t mask = 0;
+    int n = 0;
+    template <class Tr, class NS>
+    __device__ __forceinline__ bool visible(Tr& t, const NS& ns) {
+        const bool lit = ns.shadow && !t.occluded(ns.o, ns.d, ns.dist);
+        if (n < 64) mask |= uint64_t(lit) << n;
+        ++n;
+        return lit;
+    }
 };
 
 #ifndef RM_LEAF_MAX_AGE
@@ -1114,7 +1133,7 @@ struct Tracer {
                 {  // direct_lighting
                     const NeeSample ns = nee_prepare(hit.p.x, hit.p.y, hit.p.z, pnx, pny, pnz, wox, woy, woz, rough,
                                                      metallic, spec_prob, hit.ar, hit.ag, hit.ab);
-                    const bool lit = ns.shadow && !occluded(ns.o, ns.d, ns.dist);
+                    const bool lit = adj.visible(*this, ns);
                     cur_sumW = ns.sumW;
                     cur_M = ns.M;
                     const float dlr = lit ? ns.r : 0.0f, dlg = lit ? ns.g : 0.0f, dlb = lit ? ns.b : 0.0f;

@@ -88,6 +88,11 @@ struct AdjointArgs {
     int ngroup;
     double* grad;                // albedo r, g, b, roughness, emission scale per group
     unsigned long long* mismatch;  // samples whose replay differed from their primal value
+    // One record per (pixel, lane) stream for the sample round in flight:
+    // the primal radiance (xyz), the RNG state at the sample's start and
+    // after it, and the shadow-test outcomes of its path (two words).
+    float4* primal;
+    uint4* record;               // start state, next state, outcome bits 0-31, 32-63
 };
 void launch_adjoint(int stack, int lanes, const DevScene& sc, const FrameParams& P, const AdjointArgs& A,
                     unsigned long long* cnt, hipStream_t st);
