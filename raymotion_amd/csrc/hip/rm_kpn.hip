@@ -55,6 +55,17 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 // left). DPP row_shl:1 / row_shr:1, the row's end filled from a row_ror of
 // the neighbouring tile.
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// One all-zero 64-channel bf16 pixel (128 bytes), never written: the load
+// source of taps that fall outside the image.
+__device__ __align__(16) uint32_t kZeroPixel[32];
+
+// 16-byte load through a global-address-space pointer (a generic pointer
+// would make it a flat load, which also counts against the LDS wait counter).
+typedef __attribute__((address_space(1))) const bf16x8 global_bf16x8;
+__device__ __forceinline__ bf16x8 load_global(const __bf16* p) {
+    return *reinterpret_cast<global_bf16x8*>(reinterpret_cast<uintptr_t>(p));
+}
 constexpr int kDppRowShl1 = 0x101, kDppRowShr1 = 0x111, kDppRowRor1 = 0x121, kDppRowRor15 = 0x12F;
 
 __device__ __forceinline__ bf16x8 frag_next(bf16x8 cur, bf16x8 nxt) {
@@ -211,15 +222,16 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv(const __bf16* __
         } else {
         // A fragments of k-step s: 8 channels of the tap-shifted pixel, zero
         // outside the image and in the K padding (taps 9 and up).
+        // Taps outside the image (and the K padding) load from kZeroPixel,
+        // so no branch surrounds the loads (see k_conv64).
         auto load_a = [&](int s, bf16x8 (&a)[J]) {
             const int k = s * 32 + 8 * kq;
             const int tap = k / CIN, ci = k - tap * CIN;
             const long off = long((tap / 3 - 1) * W + (tap % 3 - 1)) * CIN + ci;
+            const __bf16* zero = reinterpret_cast<const __bf16*>(kZeroPixel);
 #pragma unroll
-            for (int j = 0; j < J; ++j) {
-                a[j] = zero8();
-                if ((taps[j] >> tap) & 1u) a[j] = *reinterpret_cast<const bf16x8*>(base[j] + off);
-            }
+            for (int j = 0; j < J; ++j)
+                a[j] = load_global((taps[j] >> tap) & 1u ? base[j] + off : zero);
         };
         // kConvDepth k-steps of A fragments in flight ahead of the MFMAs.
         bf16x8 a[kConvDepth + 1][J];
@@ -290,6 +302,170 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv(const __bf16* __
                         if (MODE == 0) static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v, 0.0f);
                         else static_cast<float*>(out)[o] = v;
                     }
+        }
+    }
+}
+
+// The 64-input-channel layers on 2-D tiles: a wave's tile is 4 image rows x
+// 16 pixels (row tile j = image row 4 rg + j, lane r16 = pixel 16 seg + r16)
+// x COUT channels, so the 3 tap rows of its 4 output rows are 6 image rows,
+// each loaded once (both channel halves of the 16 centre pixels, and the
+// pixel on either side for the row's left / right neighbours), instead of 3
+// tap rows per 64-pixel strip. A loaded row is kept while the tap rows that
+// use it run, its column shifts (dx = -1, +1) formed by DPP row shifts at
+// each use: tap row dy of row tile j reads image row j + dy + 1 of the
+// tile, so rows 0-3 serve dy = -1, rows 1-4 dy = 0 and rows 2-5 dy = +1
+// (four rows kept, the next one loaded during the current tap row).
+// Image rows and pixels outside the image load as zeros, which is the
+// convolution's zero padding, so no per-tap masks are needed. Each output
+// sums the k-steps s = 2 (3 dy + dx) + half in order, as in k_conv.
+template <int COUT, int MODE>
+__global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* __restrict__ X,
+                                                                const __bf16* __restrict__ Wt,
+                                                                const float* __restrict__ bias,
+                                                                const __bf16* __restrict__ act,
+                                                                void* __restrict__ out, int N, int H, int W) {
+    constexpr int CIN = 64;
+    constexpr int KP = 9 * CIN;
+    constexpr int KS = KP / 32;
+    constexpr int NT = COUT / 16;
+    __shared__ bf16x8 wl[KS * NT * 64];
+    for (int f = threadIdx.x; f < KS * NT * 64; f += 256) {
+        const int l = f & 63, t = (f >> 6) % NT, s = (f >> 6) / NT;
+        const int co = t * 16 + (l & 15), k = s * 32 + 8 * (l >> 4);
+        wl[f] = *reinterpret_cast<const bf16x8*>(Wt + size_t(co) * KP + k);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int segs = (W + 15) / 16, groups = (H + 3) / 4;
+    const int tiles = N * groups * segs;
+    float bco[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bco[t] = MODE == 2 ? 0.0f : bias[16 * t + r16];
+    // The 4 waves of a block take 4 neighbouring tiles of one row group.
+    for (int tile = int(blockIdx.x) * 4 + wave; tile < tiles; tile += int(gridDim.x) * 4) {
+        const int seg = tile % segs, rest = tile / segs;
+        const int rg = rest % groups, n = rest / groups;
+        const int x0 = seg * 16, y0 = rg * 4;
+        const int x = x0 + r16;
+        const size_t img = size_t(n) * H;
+        // Fragments of image row yy: c[h], the lane's pixel for channel half
+        // h; e[h], the row's ends in one register (lane 15: the pixel left of
+        // x0, lane 0: the pixel right of x0 + 15), which is where frag_prev
+        // and frag_next read them. Rows and pixels outside the image are
+        // loaded from kZeroPixel: every lane loads, with no branch or select
+        // after the load, so the compiler can wait for an older row while
+        // this one is still in flight.
+        auto load_row = [&](int yy, bf16x8 (&c)[2], bf16x8 (&e)[2]) {
+            const bool row = yy >= 0 && yy < H;
+            const __bf16* line = X + (img + size_t(row ? yy : 0)) * size_t(W) * CIN;
+            const bool okc = row && x < W;
+            const int xe = r16 == 15 ? x0 - 1 : x0 + 16;
+            const bool oke = row && ((r16 == 15 && x0 > 0) || (r16 == 0 && x0 + 16 < W));
+            const __bf16* zero = reinterpret_cast<const __bf16*>(kZeroPixel);
+            const __bf16* pcen = (okc ? line + size_t(x) * CIN : zero) + 8 * kq;
+            const __bf16* pend = (oke ? line + size_t(xe) * CIN : zero) + 8 * kq;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                c[h] = load_global(pcen + 32 * h);
+                e[h] = load_global(pend + 32 * h);
+            }
+        };
+        // rc[j], re[j]: tile row dyi + j (image row y0 - 1 + dyi + j), the
+        // input of row tile j at tap row dyi
+        bf16x8 rc[4][2], re[4][2];
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) load_row(y0 - 1 + rr, rc[rr], re[rr]);
+        {
+            f32x4 acc[4][NT];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            // Tap row dyi of row tile j reads tile row dyi + j: rows 0-3 (rc,
+            // re) and rows 4, 5, each loaded at the start of the tap row
+            // before its first use, so it is in flight during that tap row.
+            // The scheduling barriers keep each tap row's instructions
+            // together (unrolled freely, the compiler hoists the later rows'
+            // LDS reads and runs out of registers).
+            bf16x8 r4c[2], r4e[2], r5c[2], r5e[2];
+#pragma unroll
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                if (dyi == 0) load_row(y0 + 3, r4c, r4e);
+                if (dyi == 1) load_row(y0 + 4, r5c, r5e);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const int dxi = q >> 1, h = q & 1, s = 6 * dyi + q;
+                    bf16x8 a[4];  // the column shift dx = dxi - 1 of each row tile's input row
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int i = dyi + j;
+                        const bf16x8 c = i < 4 ? rc[i][h] : i == 4 ? r4c[h] : r5c[h];
+                        const bf16x8 e = i < 4 ? re[i][h] : i == 4 ? r4e[h] : r5e[h];
+                        a[j] = dxi == 1 ? c : dxi == 0 ? frag_prev(c, e) : frag_next(c, e);
+                    }
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const bf16x8 b = wl[(s * NT + t) * 64 + lane];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[j][t] = mfma(a[j], b, acc[j][t]);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // acc[j][t][r] is pixel (x0 + 4 kq + r, y0 + j), channel 16 t + r16.
+            auto pixel_of = [&](int j, int r) -> long {
+                const int xx = x0 + 4 * kq + r, yy = y0 + j;
+                return (xx < W && yy < H) ? long(img + size_t(yy)) * W + xx : -1;
+            };
+            if constexpr (MODE == 2) {
+                static_assert(4 * NT * 4 <= 64, "one mask bit per output of the lane");
+                __bf16 av[4][NT][4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const long m = pixel_of(j, r);
+                            av[j][t][r] = act[size_t(m < 0 ? 0 : m) * COUT + 16 * t + r16];
+                        }
+                unsigned long long on = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            on |= (unsigned long long)(float(av[j][t][r]) > 0.0f) << ((j * NT + t) * 4 + r);
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const long m = pixel_of(j, r);
+                            if (m < 0) continue;
+                            static_cast<__bf16*>(out)[size_t(m) * COUT + 16 * t + r16] =
+                                (on >> ((j * NT + t) * 4 + r)) & 1ull ? (__bf16)acc[j][t][r] : (__bf16)0.0f;
+                        }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const long m = pixel_of(j, r);
+                            if (m < 0) continue;
+                            const size_t o = size_t(m) * COUT + 16 * t + r16;
+                            const float v = acc[j][t][r] + bco[t];
+                            if (MODE == 0) static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v, 0.0f);
+                            else static_cast<float*>(out)[o] = v;
+                        }
+            }
         }
     }
 }
@@ -566,16 +742,22 @@ void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, 
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    const size_t tiles = (P + 16 * kConvJ - 1) / (16 * kConvJ);
+    // 64 input channels: 2-D tiles of 4 rows x 16 pixels (k_conv64); else
+    // strips of 16 * kConvJ pixels (k_conv).
+    const size_t tiles = cin == 64 ? size_t(n) * size_t((h + 3) / 4) * size_t((wdt + 15) / 16)
+                                   : (P + 16 * kConvJ - 1) / (16 * kConvJ);
     const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * kConvBlocksPerCU))), block(256);
 #define RM_CONV(CI, CO, MO) \
     hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
+#define RM_CONV64(CO, MO) \
+    hipLaunchKernelGGL((k_conv64<CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
     if (cin == 16 && cout == 64 && mode == 0) RM_CONV(16, 64, 0);
-    else if (cin == 64 && cout == 64 && mode == 0) RM_CONV(64, 64, 0);
-    else if (cin == 64 && cout == 32 && mode == 1) RM_CONV(64, 32, 1);
-    else if (cin == 64 && cout == 64 && mode == 2) RM_CONV(64, 64, 2);
+    else if (cin == 64 && cout == 64 && mode == 0) RM_CONV64(64, 0);
+    else if (cin == 64 && cout == 32 && mode == 1) RM_CONV64(32, 1);
+    else if (cin == 64 && cout == 64 && mode == 2) RM_CONV64(64, 2);
     else if (cin == 32 && cout == 64 && mode == 2) RM_CONV(32, 64, 2);
 #undef RM_CONV
+#undef RM_CONV64
 }
 
 void launch_flip_weights(int cin, int cout, const __bf16* w, __bf16* wd, hipStream_t st) {
