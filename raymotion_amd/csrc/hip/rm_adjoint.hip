@@ -129,7 +129,7 @@ __device__ __forceinline__ uint64_t stream_seed(const FrameParams& P, uint32_t x
 template <int S>
 __global__ __launch_bounds__(kBlock) void k_adjoint_primal(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
                                                             int round, unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pixel = gid / uint32_t(lanes), lane = gid % uint32_t(lanes);
     if (pixel >= uint32_t(P.width) * uint32_t(P.height)) return;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_primal(DevScene sc, FramePar
 template <int S>
 __global__ __launch_bounds__(kBlock) void k_adjoint_replay(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
                                                             int round, unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     // The block's gradient table, 5 per group, sized at launch to the scene's
     // groups (640 B at 16 groups instead of 5 KiB for the largest count).
     extern __shared__ double grad[];

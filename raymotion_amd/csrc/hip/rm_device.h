@@ -167,13 +167,23 @@ constexpr int kLeafFrac8 = RM_LEAF_FRAC8;  // waiting share of the traversing la
 __shared__ float4 lds_light_rec[kLdsLights * kLightWords];
 __shared__ float lds_light_cdf[kLdsLights];
 
-// Per-lane tracer state. S is the capacity of the lane's LDS traversal stack;
-// stk points at this lane's column of a [S][kBlock] shared array.
+// Entries of a lane's traversal stack kept in LDS: all of them up to
+// kLdsStack; the deep instance (S = kSpillStack, for trees whose stack bound
+// exceeds kLdsStack) keeps its first kLdsStack entries there and the rest in
+// a per-lane array in scratch memory.
+template <int S>
+constexpr int lds_stack_entries() { return S < kLdsStack ? S : kLdsStack; }
+
+// Per-lane tracer state. S is the capacity of the lane's traversal stack;
+// stk points at this lane's column of a [lds_stack_entries<S>()][kBlock]
+// shared array.
 template <int S>
 struct Tracer {
     const DevScene& sc;
     const FrameParams& P;
     uint32_t* stk;
+    // entries kLdsStack .. S - 1 of the deep instance (unused otherwise)
+    uint32_t deep[S > kLdsStack ? S - kLdsStack : 1];
     Rng rng;
     Tally tl;
 
@@ -316,7 +326,7 @@ struct Tracer {
             if (kind[c] == 0) {
                 if (next >= 0) {
                     if (T.sp < S) {
-                        stk[T.sp * kBlock] = uint32_t(next);
+                        stack_put(T.sp, uint32_t(next));
                         ++T.sp;
                     } else {
                         tl.overflow = 1;
@@ -330,10 +340,25 @@ struct Tracer {
                 T.last = true;
             } else {
                 --T.sp;
-                next = int(stk[T.sp * kBlock]);
+                next = int(stack_get(T.sp));
             }
         }
         T.node = next;
+    }
+    __device__ __forceinline__ void stack_put(int i, uint32_t v) {
+        if constexpr (S > kLdsStack) {
+            if (i >= kLdsStack) {
+                deep[i - kLdsStack] = v;
+                return;
+            }
+        }
+        stk[i * kBlock] = v;
+    }
+    __device__ __forceinline__ uint32_t stack_get(int i) const {
+        if constexpr (S > kLdsStack) {
+            if (i >= kLdsStack) return deep[i - kLdsStack];
+        }
+        return stk[i * kBlock];
     }
     // Intersects the pending leaf slots of the last node, nearest first, each
     // while its entry lies within the current hit (plus 2^-12 relative).

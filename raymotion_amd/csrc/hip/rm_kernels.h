@@ -10,18 +10,21 @@
 namespace rmh {
 
 // `stack` is the traversal stack depth the resident tree needs; it selects the
-// kernel instance with a large enough LDS stack (32, 48 or 64 entries per lane).
-// (A 48-entry LDS stack with deeper entries in per-lane scratch was measured
-// 5-7 % slower on restir_shade, although rays rarely reach it.)
-constexpr int kMaxStack = 64;
+// kernel instance with a large enough stack: 32, 48 or 64 entries per lane in
+// LDS, or for deeper trees kSpillStack entries of which those past kLdsStack
+// live in per-lane scratch memory. (A 48-entry LDS stack with deeper entries
+// in scratch was measured 5-7 % slower on restir_shade, although rays rarely
+// reach it, so trees that fit in LDS keep the LDS-only instances.)
+constexpr int kMaxStack = kSpillStack;
 
 // Calls f(std::integral_constant<int, S>{}) with the smallest stack capacity
-// S in {32, 48, 64} that holds `stack` entries.
+// S in {32, 48, 64, kSpillStack} that holds `stack` entries.
 template <class F>
 inline auto with_stack(int stack, F&& f) {
     if (stack <= 32) return f(std::integral_constant<int, 32>{});
     if (stack <= 48) return f(std::integral_constant<int, 48>{});
-    return f(std::integral_constant<int, 64>{});
+    if (stack <= kLdsStack) return f(std::integral_constant<int, kLdsStack>{});
+    return f(std::integral_constant<int, kSpillStack>{});
 }
 
 // Work queues of the persistent passes: kBands bands of image rows, one per

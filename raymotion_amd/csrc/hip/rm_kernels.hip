@@ -19,7 +19,7 @@ __device__ __forceinline__ V3 clamp3(V3 c, float cl) {
 template <int S, int LANES>
 __global__ __launch_bounds__(kBlock) void k_render_paths(DevScene sc, FrameParams P, uint8_t* __restrict__ out,
                                                           float4* __restrict__ hdr, unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pixel = gid / LANES, lane = gid % LANES;
     const uint32_t npx = uint32_t(P.width) * uint32_t(P.height);
@@ -89,7 +89,7 @@ template <int S>
 __global__ __launch_bounds__(kBlock) void k_spatial(DevScene sc, FrameParams P, GBufferSoA gb, ReservoirSoA ri,
                                                      ReservoirSoA ro_, GBufferSoA hg, ReservoirSoA hr,
                                                      unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
     if (id >= uint32_t(P.width) * uint32_t(P.height)) return;
     const uint32_t x = id % uint32_t(P.width), y = id / uint32_t(P.width);
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, FrameParams P, in
                                                    const float* __restrict__ d, const float* __restrict__ tmax, int any,
                                                    int32_t* __restrict__ tri, float4* __restrict__ tuv,
                                                    unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     Tracer<S> pt(sc, P, stack + threadIdx.x);
@@ -368,11 +368,15 @@ inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock
         } else if ((stack) <= 48) {      \
             constexpr int S = 48;        \
             launch;                      \
-        } else {                         \
+        } else if ((stack) <= 64) {      \
             constexpr int S = 64;        \
+            launch;                      \
+        } else {                         \
+            constexpr int S = 256;       \
             launch;                      \
         }                                \
     } while (0)
+static_assert(rmh::kLdsStack == 64 && rmh::kSpillStack == 256, "RM_STACK_DISPATCH instances");
 
 void launch_render_paths(int stack, int lanes, const DevScene& sc, const FrameParams& P, uint8_t* out, float4* hdr,
                          unsigned long long* cnt, hipStream_t st) {

@@ -96,6 +96,10 @@ constexpr int kMatWords = 4;      // [ar ag ab alpha] [er eg eb ior] [rough meta
 constexpr int kLightWords = 4;    // [v0.xyz v1.x] [v1.yz v2.xy] [v2.z nn.xyz] [er eg eb alpha]
 constexpr int kLdsLights = 64;    // scenes with at most this many lights sample them from LDS
 constexpr int kBlock = 256;       // threads per block of every pixel pass
+// Traversal stack: entries per lane kept in LDS, and the capacity of the deep
+// kernel instance (LDS entries first, the rest in per-lane scratch memory).
+constexpr int kLdsStack = 64;
+constexpr int kSpillStack = 256;
 
 // Per-pass counters: counters[3 * pass + {0 rays, 1 node visits, 2 triangle
 // tests}], then a stack-overflow flag. Lanes add once per kernel; the

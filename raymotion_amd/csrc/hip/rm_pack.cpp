@@ -32,66 +32,93 @@ struct BinTree {  // binary tree recovered from the preorder array
     }
 };
 
-// Collapses the binary subtree under `b` into wide node `dst` (already
-// allocated). Children are gathered by repeatedly opening the inner child with
-// the largest surface area until all kWide slots are used.
-void collapse(const BinTree& t, int b, int dst, int depth, int stack_here, PackedScene& out) {
+// One 4-wide node during collapse: its children's binary nodes and slots,
+// and the next slot whose inner child still has to be collapsed.
+struct CollapseFrame {
+    int dst, depth, stack_below, next;
+    int nk;
     int kids[kWide];
-    int nk = 0;
+    float lo[3][kWide], hi[3][kWide];
+    int ref[kWide], kind[kWide];
+};
+
+// Gathers the children of wide node `dst` from the binary subtree under `b`:
+// repeatedly opens the inner child with the largest surface area until all
+// kWide slots are used (a leaf root becomes the single child of the root).
+void gather(const BinTree& t, int b, int dst, int depth, int stack_here, CollapseFrame& f, PackedScene& out) {
+    f.dst = dst;
+    f.depth = depth;
+    f.next = 0;
+    f.nk = 0;
     if (t.leaf(b)) {
-        kids[nk++] = b;  // a leaf root becomes the single child of the root node
+        f.kids[f.nk++] = b;
     } else {
-        kids[nk++] = t.left(b);
-        kids[nk++] = t.right(b);
-        while (nk < kWide) {
+        f.kids[f.nk++] = t.left(b);
+        f.kids[f.nk++] = t.right(b);
+        while (f.nk < kWide) {
             int best = -1;
             float best_area = -1.0f;
-            for (int k = 0; k < nk; ++k)
-                if (!t.leaf(kids[k]) && t.area(kids[k]) > best_area) {
-                    best_area = t.area(kids[k]);
+            for (int k = 0; k < f.nk; ++k)
+                if (!t.leaf(f.kids[k]) && t.area(f.kids[k]) > best_area) {
+                    best_area = t.area(f.kids[k]);
                     best = k;
                 }
             if (best < 0) break;
-            const int open = kids[best];
-            kids[best] = t.left(open);
-            kids[nk++] = t.right(open);
+            const int open = f.kids[best];
+            f.kids[best] = t.left(open);
+            f.kids[f.nk++] = t.right(open);
         }
     }
     const float inf = std::numeric_limits<float>::infinity();
-    float lo[3][kWide], hi[3][kWide];
-    int ref[kWide], kind[kWide];
     int inner = 0;
     for (int c = 0; c < kWide; ++c) {
-        if (c >= nk) {
-            for (int a = 0; a < 3; ++a) { lo[a][c] = inf; hi[a][c] = -inf; }
-            ref[c] = 0;
-            kind[c] = -1;
+        if (c >= f.nk) {
+            for (int a = 0; a < 3; ++a) { f.lo[a][c] = inf; f.hi[a][c] = -inf; }
+            f.ref[c] = 0;
+            f.kind[c] = -1;
             continue;
         }
-        const rm_bvh_node& n = t.nodes[kids[c]];
-        lo[0][c] = n.mn.x; lo[1][c] = n.mn.y; lo[2][c] = n.mn.z;
-        hi[0][c] = n.mx.x; hi[1][c] = n.mx.y; hi[2][c] = n.mx.z;
+        const rm_bvh_node& n = t.nodes[f.kids[c]];
+        f.lo[0][c] = n.mn.x; f.lo[1][c] = n.mn.y; f.lo[2][c] = n.mn.z;
+        f.hi[0][c] = n.mx.x; f.hi[1][c] = n.mx.y; f.hi[2][c] = n.mx.z;
         if (n.count > 0) {
-            ref[c] = n.offset;
-            kind[c] = n.count;
+            f.ref[c] = n.offset;
+            f.kind[c] = n.count;
         } else {
             ++inner;
-            ref[c] = -1;  // patched below
-            kind[c] = 0;
+            f.ref[c] = -1;  // the child's wide node, assigned when it is allocated
+            f.kind[c] = 0;
         }
     }
     // Deepest stack below: every inner child but the first visited is stacked.
-    const int stack_below = stack_here + std::max(0, inner - 1);
-    out.stack_need = std::max(out.stack_need, stack_below);
-    for (int c = 0; c < nk; ++c) {
-        if (kind[c] != 0) continue;
+    f.stack_below = stack_here + std::max(0, inner - 1);
+    out.stack_need = std::max(out.stack_need, f.stack_below);
+}
+
+// Collapses the binary tree into 4-wide nodes, numbered in depth-first
+// preorder (a node's first inner child follows it, each later one follows the
+// previous one's subtree). Iterative, with one frame per level of the wide
+// tree on the heap, so any depth of the caller's tree is handled.
+void collapse(const BinTree& t, PackedScene& out) {
+    std::vector<CollapseFrame> path(1);
+    gather(t, 0, 0, 0, 0, path[0], out);
+    while (!path.empty()) {
+        CollapseFrame& f = path.back();
+        while (f.next < f.nk && f.kind[f.next] != 0) ++f.next;
+        if (f.next == f.nk) {
+            write_node(&out.nodes[size_t(f.dst) * kNodeWords], f.lo, f.hi, f.ref, f.kind);
+            path.pop_back();
+            continue;
+        }
+        const int c = f.next++;
         const int child = int(out.nodes.size() / kNodeWords);
         out.nodes.resize(out.nodes.size() + kNodeWords);
-        out.depth.push_back(depth + 1);
-        ref[c] = child;
-        collapse(t, kids[c], child, depth + 1, stack_below, out);
+        out.depth.push_back(f.depth + 1);
+        f.ref[c] = child;
+        const int b = f.kids[c], depth = f.depth + 1, stack_here = f.stack_below;
+        path.emplace_back();  // may move the frames: f is not used after this
+        gather(t, b, child, depth, stack_here, path.back(), out);
     }
-    write_node(&out.nodes[size_t(dst) * kNodeWords], lo, hi, ref, kind);
 }
 
 using rmpar::parallel_chunks;
@@ -246,7 +273,7 @@ void pack_tree(const rm_scene_desc& s, PackedScene& out) {
     out.nodes.reserve(s.nnode / 2 * kNodeWords);
     out.nodes.resize(kNodeWords);
     out.depth.push_back(0);
-    collapse(bt, 0, 0, 0, 0, out);
+    collapse(bt, out);
 }
 
 void pack_geometry(const rm_scene_desc& s, PackedScene& out) {

@@ -883,7 +883,7 @@ constexpr int kNeeFrac8 = RM_NEE_FRAC8;  // waiting share of the active lanes, i
 template <int S, class E>
 __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_persistent(DevScene sc, FrameParams P, PassIO io,
                                                                       BandQueue q, unsigned long long* cnt) {
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     pt.stage_lights();
     E eng(pt, P, io);
@@ -1224,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(DevScene sc, FrameParams P,
                                                      size_t stride, uint4* __restrict__ hits, unsigned* __restrict__ ctl,
                                                      int cur, unsigned long long* cnt) {
     using W = Slot<typename E::Lane>;
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     const unsigned n = ctl[kWfCount0 + cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1299,7 +1299,7 @@ __global__ __launch_bounds__(kBlock, E::kMinBlocks) void k_wf_finish(DevScene sc
                                                                      unsigned* __restrict__ ctl, int cur,
                                                                      unsigned long long* cnt) {
     using W = Slot<typename E::Lane>;
-    __shared__ uint32_t stack[S * kBlock];
+    __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     Tracer<S> pt(sc, P, stack + threadIdx.x);
     E eng(pt, P, io);
     const unsigned n = ctl[kWfCount0 + cur];
