@@ -4,9 +4,9 @@
 // include/raymotion/runtime.hpp:119-130).
 //
 // 63-bit Morton codes of the triangle centroids (21 bits per axis over the
-// centroid bounds) and a device radix sort put the triangles in spatial order;
-// the binary tree is then built bottom-up by parallel locally-ordered
-// clustering (Meister and Bittner, "Parallel Locally-Ordered Clustering for
+// centroid bounds) and a device radix sort (rm_sort.hip) put the triangles in
+// spatial order; the binary tree is then built bottom-up by parallel
+// locally-ordered clustering (Meister and Bittner, "Parallel Locally-Ordered Clustering for
 // Bounding Volume Hierarchy Construction", IEEE TVCG 2018): every cluster
 // finds, within kPlocRadius neighbours on either side in that order, the one
 // whose merged box has the least surface area; mutual nearest neighbours merge
@@ -18,11 +18,10 @@
 // next to its first child, as in the host tree's preorder) and listed level by
 // level for the refit launches. Traversal returns the closest hit whatever the
 // tree, so images equal those rendered with a host-built tree.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 
 #include "rm_kernels.h"
+#include "rm_sort.h"
 
 namespace rmh {
 namespace {
@@ -434,11 +433,8 @@ struct Scratch {
 Scratch carve(void* base, int n) {
     const size_t N = size_t(n > 1 ? n : 2);  // also bounds the wide node count (n - 1, or 1)
     Scratch s{};
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, s.sort_bytes, static_cast<const uint64_t*>(nullptr),
-                                             static_cast<uint64_t*>(nullptr), static_cast<const uint32_t*>(nullptr),
-                                             static_cast<uint32_t*>(nullptr), int(N), 0, 63);
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, s.scan_bytes, static_cast<const int*>(nullptr),
-                                           static_cast<int*>(nullptr), int(N));
+    s.sort_bytes = sort_tmp_bytes(int(N));
+    s.scan_bytes = scan_tmp_bytes(int(N));
     size_t off = 0;  // with base == nullptr only the size is wanted
     auto take = [&](size_t bytes) {
         void* q = base ? static_cast<void*>(static_cast<char*>(base) + off) : nullptr;
@@ -493,9 +489,7 @@ int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st) {
     hipLaunchKernelGGL(k_centroid_bounds, dim3(bound_blocks), dim3(kB), 0, st, a.geo_scene, n, s.ctl);
     hipLaunchKernelGGL(k_morton, dim3(blocks(size_t(n))), dim3(kB), 0, st, a.geo_scene, n, s.ctl, s.keys_in,
                        s.vals_in);
-    if ((e = hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, s.sort_bytes, s.keys_in, s.keys, s.vals_in, a.perm, n, 0,
-                                                63, st)) != hipSuccess)
-        return int(e);
+    if ((e = sort_pairs_63(s.keys_in, s.vals_in, s.keys, a.perm, n, s.sort_tmp, st)) != hipSuccess) return int(e);
     hipLaunchKernelGGL(k_gather, dim3(blocks(size_t(n) * kTriGeoWords)), dim3(kB), 0, st, a.geo_scene, a.perm, n,
                        a.geo);
     // Clustering. The host reads the cluster count after every round (it
@@ -516,7 +510,8 @@ int gpu_build(const GpuBuildArgs& a, GpuBuildResult& r, hipStream_t st) {
             hipLaunchKernelGGL(k_ploc_nn, g, dim3(kB), 0, st, s.clo[cur], s.chi[cur], cnt, s.nn);
             hipLaunchKernelGGL(k_ploc_merge, g, dim3(kB), 0, st, cnt, s.cid[cur], s.clo[cur], s.chi[cur], s.nn,
                                s.keep, s.ctl + kCtlNextId, s.child, s.box);
-            if ((e = hipcub::DeviceScan::ExclusiveSum(s.scan_tmp, s.scan_bytes, s.keep, s.pos, m, st)) != hipSuccess)
+            if ((e = exclusive_scan(reinterpret_cast<const uint32_t*>(s.keep), reinterpret_cast<uint32_t*>(s.pos), m,
+                                    s.scan_tmp, st)) != hipSuccess)
                 return int(e);
             hipLaunchKernelGGL(k_ploc_compact, g, dim3(kB), 0, st, cnt, s.keep, s.pos, s.cid[cur], s.clo[cur],
                                s.chi[cur], s.cid[cur ^ 1], s.clo[cur ^ 1], s.chi[cur ^ 1], cnt2);

@@ -163,3 +163,51 @@ def test_full_heightfield_oracle(backend):
               f"exact {np.mean(g_hdr == c_hdr):.6f}")
         assert info["path"] == (1 if name == "restir" else 0)
         assert err.max() <= 1e-3
+
+
+def _spread21(v):
+    x = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    for shift, mask in ((32, 0x1F00000000FFFF), (16, 0x1F0000FF0000FF), (8, 0x100F00F00F00F00F),
+                        (4, 0x10C30C30C30C30C3), (2, 0x1249249249249249)):
+        x = (x | (x << np.uint64(shift))) & np.uint64(mask)
+    return x
+
+
+def _morton_order(exported):
+    """The leaf order the build must produce: triangles stably sorted by the
+    63-bit Morton code of their centroid (v0 + v1 + v2, FP32) over the
+    centroid bounds, 21 bits per axis interleaved x, y, z from the top."""
+    t = exported.tris
+    c = (t["v0"] + t["v1"]) + t["v2"]  # float32, the kernel's association
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    q = []
+    for a in range(3):
+        span = hi[a] - lo[a]
+        u = (c[:, a] - lo[a]) / span if span > 0 else np.zeros(len(c), np.float32)
+        u = np.clip(u, np.float32(0), np.float32(1)).astype(np.float32)
+        q.append(np.minimum((u * np.float32(2097152.0)).astype(np.uint32), np.uint32(2097151)))
+    key = (_spread21(q[0]) << np.uint64(2)) | (_spread21(q[1]) << np.uint64(1)) | _spread21(q[2])
+    return np.argsort(key, kind="stable"), key
+
+
+@pytest.mark.parametrize("scene", ["mixed", "heightfield", "coincident"])
+def test_leaf_order_is_stable_morton_sort(backend, scene):
+    """The device radix sort of (Morton code, triangle) pairs: the built
+    tree's leaf-order triangles equal a stable host sort of the same keys,
+    ties (equal codes) in triangle order."""
+    if scene == "mixed":
+        s = scenes.mixed_spheres(seed=5, n_spheres=16, subdivisions=3)
+    elif scene == "heightfield":
+        s = scenes.heightfield(seed=2, n=256)
+    else:
+        s = Scene()
+        for k in range(3000):  # many equal codes: a few distinct centroids
+            a = np.array([k % 3, 0.0, 0.0])
+            tri(s, a, a + [0.1, 0, 0], a + [0, 0.1, 0], (0.5, 0.5, 0.5))
+    e = s.export()
+    backend.set_scene(e, gpu_tree=True, rebuild=True)
+    _, tris, _ = backend.read_tree()
+    leaf_ids = tris[:, 9].view(np.int32)
+    order, key = _morton_order(e)
+    print(f"{scene}: {len(order)} triangles, {len(np.unique(key))} distinct codes", flush=True)
+    assert np.array_equal(leaf_ids, order)
