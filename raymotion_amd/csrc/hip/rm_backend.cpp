@@ -349,9 +349,17 @@ void kpn_forward(rm_ctx* c, int n, int h, int w, const float* rad, const float* 
     const KpnNet net = kpn_net();
     const __bf16* pb = c->kpn_pb.as<__bf16>();
     const float* p = c->kpn_p.as<float>();
+    // An image alone (no loss, no [P][3] output): the head's logits go
+    // straight into the filter inside the head convolution.
+    const bool fused = !ref && !out && out4;
     for (int l = 0; l < kKpnLayers; ++l) {
         const KpnLayer& L = net.layer[l];
         const bool last = l == kKpnLayers - 1;
+        if (last && fused) {
+            launch_conv(L.cin, L.cout, 3, c->kpn_x[l].as<__bf16>(), pb + L.w_off, p + L.b_off, nullptr, out4, n, h, w,
+                        c->stream, rad);
+            return;
+        }
         launch_conv(L.cin, L.cout, last ? 1 : 0, c->kpn_x[l].as<__bf16>(), pb + L.w_off, p + L.b_off, nullptr,
                     last ? c->kpn_logits.ptr : c->kpn_x[l + 1].ptr, n, h, w, c->stream);
     }
@@ -871,12 +879,14 @@ int render_locked(rm_ctx* c, const rm_frame* f, uint64_t seed, uint8_t* rgb8, fl
         mark(3);
         const float4* final_img = c->shade.as<float4>();
         bool final_in_hist = false;
+        // Features: fp32 when the caller reads them back, bf16 straight into
+        // the network's input for a learned frame.
         if (features)
-            launch_features(P, gb, c->shade.as<float4>(), c->kpn_var.as<float>(), c->kpn_feat32.as<float>(),
-                            c->kpn_rad.as<float>(), c->stream);
+            launch_features(P, gb, c->shade.as<float4>(), c->kpn_var.as<float>(),
+                            feat_host ? c->kpn_feat32.as<float>() : nullptr, c->kpn_rad.as<float>(), c->stream,
+                            learned ? c->kpn_x[0].as<__bf16>() : nullptr);
         if (learned) {
             // Kernel-predicting network in place of the a-trous filter.
-            launch_to_bf16(c->kpn_feat32.as<float>(), c->kpn_x[0].as<__bf16>(), px * kKpnFeat, c->stream);
             mark(4);
             kpn_forward(c, 1, f->height, f->width, c->kpn_rad.as<float>(), nullptr, nullptr, c->den_a.as<float4>(),
                         nullptr);

@@ -34,9 +34,11 @@ KpnNet kpn_net();
 
 // conv3x3 (zero padding) as an implicit GEMM on v_mfma_f32_16x16x32_bf16.
 // mode 0: relu(x * w + b) -> bf16; mode 1: x * w + b -> fp32; mode 2 (input
-// gradient): (x * w) masked where act <= 0 -> bf16.
+// gradient): (x * w) masked where act <= 0 -> bf16; mode 3 (the 64 -> 32
+// head at inference): the logits x * w + b go straight into the softmax and
+// 5x5 filter of `rad` (as launch_kpn_apply), out = float4 per pixel (w = 0).
 void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, const float* b, const __bf16* act,
-                 void* out, int n, int h, int wdt, hipStream_t st);
+                 void* out, int n, int h, int wdt, hipStream_t st, const float* rad = nullptr);
 // Weights of the input-gradient convolution: wd[ci][t * cout + co] = w[co][(8 - t) * cin + ci].
 void launch_flip_weights(int cin, int cout, const __bf16* w, __bf16* wd, hipStream_t st);
 // Weight and bias gradients of one layer into grad (at the layer's offsets):
@@ -54,9 +56,10 @@ void launch_kpn_apply(const float* logits, const float* rad, const float* ref, f
 void launch_adam(float* p, const float* g, float* m, float* v, __bf16* pb, size_t count, float lr, float beta1,
                  float beta2, float eps, int step, hipStream_t st);
 void launch_to_bf16(const float* src, __bf16* dst, size_t count, hipStream_t st);
-// Features and noisy radiance of a ReSTIR frame from its G-buffer and shading output.
+// Features and noisy radiance of a ReSTIR frame from its G-buffer and shading
+// output: feat (fp32) and featb (bf16, the network's input) where not null.
 void launch_features(const FrameParams& P, const GBufferSoA& gb, const float4* shade, const float* lum_var,
-                     float* feat, float* rad, hipStream_t st);
+                     float* feat, float* rad, hipStream_t st, __bf16* featb = nullptr);
 // Gathers batch images `idx` (each h * w pixels) from the dataset arrays.
 void launch_gather(const __bf16* feat_all, const float* rad_all, const float* ref_all, const int32_t* idx, int batch,
                    int px_per_image, __bf16* feat, float* rad, float* ref, hipStream_t st);

@@ -90,6 +90,32 @@ __device__ __forceinline__ bf16x8 frag_prev(bf16x8 cur, bf16x8 prv) {
     return __builtin_bit_cast(bf16x8, r);
 }
 
+// Softmax of a pixel's 25 logits lg[0..24] and the 5x5 filter of the noisy
+// radiance around pixel (x, y) of image n (edge-clamped): w receives the
+// normalised weights, o the filtered radiance. Shared by k_kpn_apply and the
+// fused head convolution, so both give the same bits.
+template <class LG>
+__device__ __forceinline__ void kpn_filter(LG lg, const float* __restrict__ rad, int n, int y, int x, int H, int W,
+                                           float w[kKpnTaps], float o[3]) {
+    float mx = lg[0];
+    for (int t = 1; t < kKpnTaps; ++t) mx = fmaxf(mx, lg[t]);
+    float sum = 0.0f;
+    for (int t = 0; t < kKpnTaps; ++t) {
+        w[t] = __expf(lg[t] - mx);
+        sum += w[t];
+    }
+    const float inv = 1.0f / sum;
+    o[0] = o[1] = o[2] = 0.0f;
+    for (int t = 0; t < kKpnTaps; ++t) {
+        w[t] *= inv;
+        const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
+        const float* r = rad + ((size_t(n) * H + yy) * W + xx) * 3;
+        o[0] += w[t] * r[0];
+        o[1] += w[t] * r[1];
+        o[2] += w[t] * r[2];
+    }
+}
+
 // Pixels per wave tile (kConvJ row tiles of 16), k-steps of A fragments
 // loaded ahead (layers with fewer than 64 input channels; those with 64 keep
 // one tap row ahead), and resident blocks per CU. Measured on the 1280x720
@@ -324,12 +350,17 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                                                                 const __bf16* __restrict__ Wt,
                                                                 const float* __restrict__ bias,
                                                                 const __bf16* __restrict__ act,
-                                                                void* __restrict__ out, int N, int H, int W) {
+                                                                void* __restrict__ out, int N, int H, int W,
+                                                                const float* __restrict__ rad) {
     constexpr int CIN = 64;
     constexpr int KP = 9 * CIN;
     constexpr int KS = KP / 32;
     constexpr int NT = COUT / 16;
+    static_assert(MODE != 3 || COUT == 32, "the fused filter takes the 32-channel head");
     __shared__ bf16x8 wl[KS * NT * 64];
+    // MODE 3: each wave's tile of logits (64 pixels x 25), handed from the
+    // MFMA layout (a pixel's channels across 16 lanes) to one lane per pixel.
+    __shared__ float lgs[MODE == 3 ? 4 * 64 * kKpnTaps : 1];
     for (int f = threadIdx.x; f < KS * NT * 64; f += 256) {
         const int l = f & 63, t = (f >> 6) % NT, s = (f >> 6) / NT;
         const int co = t * 16 + (l & 15), k = s * 32 + 8 * (l >> 4);
@@ -420,7 +451,34 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                 const int xx = x0 + 4 * kq + r, yy = y0 + j;
                 return (xx < W && yy < H) ? long(img + size_t(yy)) * W + xx : -1;
             };
-            if constexpr (MODE == 2) {
+            if constexpr (MODE == 3) {
+                // Logits (bias added as in mode 1) into this wave's LDS tile,
+                // then lane q filters pixel (x0 + q % 16, y0 + q / 16) exactly
+                // as k_kpn_apply does from the fp32 logits.
+                float* lw = lgs + wave * 64 * kKpnTaps;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int c = 16 * t + r16;
+                            if (c < kKpnTaps) lw[(16 * j + 4 * kq + r) * kKpnTaps + c] = acc[j][t][r] + bco[t];
+                        }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int xq = x0 + (lane & 15), yq = y0 + (lane >> 4);
+                if (xq < W && yq < H) {
+                    float wt[kKpnTaps], o[3];
+                    kpn_filter(lw + lane * kKpnTaps, rad, n, yq, xq, H, W, wt, o);
+                    static_cast<float4*>(out)[size_t(img + size_t(yq)) * W + xq] = make_float4(o[0], o[1], o[2], 0.0f);
+                }
+                // the next tile's logits overwrite this tile's only after every lane read them
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else if constexpr (MODE == 2) {
                 static_assert(4 * NT * 4 <= 64, "one mask bit per output of the lane");
                 __bf16 av[4][NT][4];
 #pragma unroll
@@ -600,25 +658,8 @@ __global__ __launch_bounds__(256) void k_kpn_apply(const float* __restrict__ log
     float my_loss = 0.0f;
     if (p < P) {
         const int n = p / (H * W), rem = p - n * H * W, y = rem / W, x = rem - y * W;
-        const float* lg = logits + size_t(p) * 32;
-        float w[kKpnTaps];
-        float mx = lg[0];
-        for (int t = 1; t < kKpnTaps; ++t) mx = fmaxf(mx, lg[t]);
-        float sum = 0.0f;
-        for (int t = 0; t < kKpnTaps; ++t) {
-            w[t] = __expf(lg[t] - mx);
-            sum += w[t];
-        }
-        const float inv = 1.0f / sum;
-        float o[3] = {0.0f, 0.0f, 0.0f};
-        for (int t = 0; t < kKpnTaps; ++t) {
-            w[t] *= inv;
-            const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
-            const float* r = rad + ((size_t(n) * H + yy) * W + xx) * 3;
-            o[0] += w[t] * r[0];
-            o[1] += w[t] * r[1];
-            o[2] += w[t] * r[2];
-        }
+        float w[kKpnTaps], o[3];
+        kpn_filter(logits + size_t(p) * 32, rad, n, y, x, H, W, w, o);
         if (out) {
             out[size_t(p) * 3] = o[0];
             out[size_t(p) * 3 + 1] = o[1];
@@ -676,7 +717,8 @@ __global__ void k_to_bf16(const float* __restrict__ s, __bf16* __restrict__ d, s
 }
 
 __global__ void k_features(FrameParams P, GBufferSoA gb, const float4* __restrict__ shade,
-                           const float* __restrict__ lum_var, float* __restrict__ feat, float* __restrict__ rad) {
+                           const float* __restrict__ lum_var, float* __restrict__ feat, float* __restrict__ rad,
+                           __bf16* __restrict__ featb) {
     const int px = blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= P.width * P.height) return;
     const float4 g0 = gb.p0[px], g1 = gb.p1[px], g2 = gb.p2[px];
@@ -708,7 +750,19 @@ __global__ void k_features(FrameParams P, GBufferSoA gb, const float4* __restric
         }
     }
     f[10] = lum_var ? log1pf(fmaxf(lum_var[px], 0.0f)) : 0.0f;
-    for (int i = 0; i < kKpnFeat; ++i) feat[size_t(px) * kKpnFeat + i] = f[i];
+    if (feat)
+        for (int i = 0; i < kKpnFeat; ++i) feat[size_t(px) * kKpnFeat + i] = f[i];
+    if (featb) {  // the network's input, converted as k_to_bf16 converts
+        bf16x8 lo, hi;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            lo[i] = (__bf16)f[i];
+            hi[i] = (__bf16)f[8 + i];
+        }
+        bf16x8* d = reinterpret_cast<bf16x8*>(featb + size_t(px) * kKpnFeat);
+        d[0] = lo;
+        d[1] = hi;
+    }
     rad[size_t(px) * 3] = fmaxf(c.x, 0.0f);
     rad[size_t(px) * 3 + 1] = fmaxf(c.y, 0.0f);
     rad[size_t(px) * 3 + 2] = fmaxf(c.z, 0.0f);
@@ -736,7 +790,7 @@ unsigned blocks_of(size_t n, unsigned b) { return unsigned((n + b - 1) / b); }
 }  // namespace
 
 void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, const float* b, const __bf16* act,
-                 void* out, int n, int h, int wdt, hipStream_t st) {
+                 void* out, int n, int h, int wdt, hipStream_t st, const float* rad) {
     const size_t P = size_t(n) * h * wdt;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -750,10 +804,11 @@ void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, 
 #define RM_CONV(CI, CO, MO) \
     hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
 #define RM_CONV64(CO, MO) \
-    hipLaunchKernelGGL((k_conv64<CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
+    hipLaunchKernelGGL((k_conv64<CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt, rad)
     if (cin == 16 && cout == 64 && mode == 0) RM_CONV(16, 64, 0);
     else if (cin == 64 && cout == 64 && mode == 0) RM_CONV64(64, 0);
     else if (cin == 64 && cout == 32 && mode == 1) RM_CONV64(32, 1);
+    else if (cin == 64 && cout == 32 && mode == 3) RM_CONV64(32, 3);
     else if (cin == 64 && cout == 64 && mode == 2) RM_CONV64(64, 2);
     else if (cin == 32 && cout == 64 && mode == 2) RM_CONV(32, 64, 2);
 #undef RM_CONV
@@ -811,9 +866,10 @@ void launch_to_bf16(const float* src, __bf16* dst, size_t count, hipStream_t st)
 }
 
 void launch_features(const FrameParams& P, const GBufferSoA& gb, const float4* shade, const float* lum_var,
-                     float* feat, float* rad, hipStream_t st) {
+                     float* feat, float* rad, hipStream_t st, __bf16* featb) {
     const size_t npx = size_t(P.width) * P.height;
-    hipLaunchKernelGGL(k_features, dim3(blocks_of(npx, 256)), dim3(256), 0, st, P, gb, shade, lum_var, feat, rad);
+    hipLaunchKernelGGL(k_features, dim3(blocks_of(npx, 256)), dim3(256), 0, st, P, gb, shade, lum_var, feat, rad,
+                       featb);
 }
 
 void launch_gather(const __bf16* feat_all, const float* rad_all, const float* ref_all, const int32_t* idx, int batch,

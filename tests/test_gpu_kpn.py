@@ -82,21 +82,24 @@ def test_training_step_reduces_loss(backend):
     assert losses[-5:].mean() < losses[:5].mean()
 
 
-def test_learned_denoise_in_frame(backend):
-    """denoise = 2 inside rm_render equals the network applied to the frame's
-    exported features."""
+@pytest.mark.parametrize("width,height", [(96, 64), (90, 61)])
+def test_learned_denoise_in_frame(backend, width, height):
+    """denoise = 2 inside rm_render (the head convolution filters in its
+    epilogue) equals the network applied to the frame's exported features
+    (logits through memory, then the filter kernel), bit for bit, also on an
+    image that ends inside a 16 x 4 tile."""
     s = scenes.mixed_spheres(seed=3, n_spheres=12, subdivisions=2)
     s.build_bvh()
     backend.set_scene(s)
     backend.kpn_set_weights(_params(9))
     cam = scenes.mixed_camera(2)
-    f = denoise.restir_frame(96, 64, 4, cam, denoise=0, temporal=0)
+    f = denoise.restir_frame(width, height, 4, cam, denoise=0, temporal=0)
     feat, noisy = backend.render_features(f, 77)
     expect = backend.kpn_denoise(feat, noisy)
-    f2 = denoise.restir_frame(96, 64, 4, cam, denoise=2, temporal=0)
+    f2 = denoise.restir_frame(width, height, 4, cam, denoise=2, temporal=0)
     _, hdr = backend.render(f2, 77)
     assert backend.stats().path == 1
-    np.testing.assert_array_equal(hdr.reshape(64, 96, 3), expect)
+    np.testing.assert_array_equal(hdr.reshape(height, width, 3), expect)
     # features: radiance channels are log(1 + noisy), normals unit or zero
     np.testing.assert_allclose(feat[..., :3], np.log1p(noisy), rtol=1e-6, atol=1e-6)
     nrm = np.linalg.norm(feat[..., 6:9], axis=-1)
