@@ -90,13 +90,13 @@ __device__ __forceinline__ bf16x8 frag_prev(bf16x8 cur, bf16x8 prv) {
     return __builtin_bit_cast(bf16x8, r);
 }
 
-// Softmax of a pixel's 25 logits lg[0..24] and the 5x5 filter of the noisy
-// radiance around pixel (x, y) of image n (edge-clamped): w receives the
-// normalised weights, o the filtered radiance. Shared by k_kpn_apply and the
-// fused head convolution, so both give the same bits.
-template <class LG>
-__device__ __forceinline__ void kpn_filter(LG lg, const float* __restrict__ rad, int n, int y, int x, int H, int W,
-                                           float w[kKpnTaps], float o[3]) {
+// Softmax of a pixel's 25 logits lg[0..24] and the 5x5 filter of its noisy
+// radiance: tap(t) points at the 3 radiance values of tap t (row t / 5 - 2,
+// column t % 5 - 2, edge-clamped). w receives the normalised weights, o the
+// filtered radiance. Shared by k_kpn_apply and the fused head convolution, so
+// both give the same bits.
+template <class LG, class TAP>
+__device__ __forceinline__ void kpn_filter(LG lg, TAP tap, float w[kKpnTaps], float o[3]) {
     float mx = lg[0];
     for (int t = 1; t < kKpnTaps; ++t) mx = fmaxf(mx, lg[t]);
     float sum = 0.0f;
@@ -108,11 +108,47 @@ __device__ __forceinline__ void kpn_filter(LG lg, const float* __restrict__ rad,
     o[0] = o[1] = o[2] = 0.0f;
     for (int t = 0; t < kKpnTaps; ++t) {
         w[t] *= inv;
-        const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
-        const float* r = rad + ((size_t(n) * H + yy) * W + xx) * 3;
+        const float* r = tap(t);
         o[0] += w[t] * r[0];
         o[1] += w[t] * r[1];
         o[2] += w[t] * r[2];
+    }
+}
+
+// Wave-level ordering of LDS accesses between the lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stores a wave's 2-D output tile (pixel (x0 + q % 16, y0 + q / 16) for
+// q = 0..63, COUT bf16 channels each) from the MFMA layout (value(j, t, r):
+// pixel q = 16 j + 4 kq + r, channel 16 t + r16) as 16-byte vectors: one
+// 16-channel group at a time goes through the wave's 2 KiB LDS buffer `tw`,
+// then lane l writes the two 8-channel halves of pixels l / 2 and 32 + l / 2
+// (half l % 2). Pixels outside the W x H image are not written.
+template <int NT, class VAL>
+__device__ __forceinline__ void store_tile_bf16(__bf16* tw, VAL value, __bf16* __restrict__ out, int x0, int y0,
+                                                size_t img, int W, int H) {
+    constexpr int COUT = 16 * NT;
+    const int lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tw[(16 * j + 4 * kq + r) * 16 + r16] = value(j, t, r);
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int q = 32 * c + (lane >> 1), hh = lane & 1;
+            const int xx = x0 + (q & 15), yy = y0 + (q >> 4);
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(tw + q * 16 + 8 * hh);
+            if (xx < W && yy < H)
+                *reinterpret_cast<bf16x8*>(out + ((img + size_t(yy)) * W + xx) * COUT + 16 * t + 8 * hh) = v;
+        }
+        wave_lds_sync();  // the next group overwrites tw only after every lane read it
     }
 }
 
@@ -359,8 +395,17 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
     static_assert(MODE != 3 || COUT == 32, "the fused filter takes the 32-channel head");
     __shared__ bf16x8 wl[KS * NT * 64];
     // MODE 3: each wave's tile of logits (64 pixels x 25), handed from the
-    // MFMA layout (a pixel's channels across 16 lanes) to one lane per pixel.
-    __shared__ float lgs[MODE == 3 ? 4 * 64 * kKpnTaps : 1];
+    // MFMA layout (a pixel's channels across 16 lanes) to one lane per pixel,
+    // and its radiance window, (16 + 4) x (4 + 4) pixels around the tile,
+    // edge-clamped as the filter's taps are. Modes 0 and 2: each wave's 2 KiB
+    // staging buffer of store_tile_bf16 (with the 72 KiB of 64 -> 64 weights,
+    // two blocks take the 160 KiB of LDS exactly).
+    constexpr int kWinW = 20, kWinH = 8;
+    constexpr int kStageWords = MODE == 3 ? 4 * (64 * kKpnTaps + kWinW * kWinH * 3) : 4 * 64 * 16 / 2;
+    __shared__ __align__(16) float stage[kStageWords];
+    float* const lgs = stage;
+    float* const rws = stage + 4 * 64 * kKpnTaps;
+    __bf16* const tws = reinterpret_cast<__bf16*>(stage);
     for (int f = threadIdx.x; f < KS * NT * 64; f += 256) {
         const int l = f & 63, t = (f >> 6) % NT, s = (f >> 6) / NT;
         const int co = t * 16 + (l & 15), k = s * 32 + 8 * (l >> 4);
@@ -456,6 +501,15 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                 // then lane q filters pixel (x0 + q % 16, y0 + q / 16) exactly
                 // as k_kpn_apply does from the fp32 logits.
                 float* lw = lgs + wave * 64 * kKpnTaps;
+                float* rw = rws + wave * kWinW * kWinH * 3;
+                for (int q = lane; q < kWinW * kWinH; q += 64) {
+                    const int xx = min(max(x0 - 2 + q % kWinW, 0), W - 1);
+                    const int yy = min(max(y0 - 2 + q / kWinW, 0), H - 1);
+                    const float* src = rad + ((img + size_t(yy)) * W + xx) * 3;
+                    rw[3 * q] = src[0];
+                    rw[3 * q + 1] = src[1];
+                    rw[3 * q + 2] = src[2];
+                }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -471,7 +525,9 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                 const int xq = x0 + (lane & 15), yq = y0 + (lane >> 4);
                 if (xq < W && yq < H) {
                     float wt[kKpnTaps], o[3];
-                    kpn_filter(lw + lane * kKpnTaps, rad, n, yq, xq, H, W, wt, o);
+                    const int lx = lane & 15, ly = lane >> 4;
+                    kpn_filter(lw + lane * kKpnTaps,
+                               [&](int t) { return rw + ((ly + t / 5) * kWinW + lx + t % 5) * 3; }, wt, o);
                     static_cast<float4*>(out)[size_t(img + size_t(yq)) * W + xq] = make_float4(o[0], o[1], o[2], 0.0f);
                 }
                 // the next tile's logits overwrite this tile's only after every lane read them
@@ -498,17 +554,17 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             on |= (unsigned long long)(float(av[j][t][r]) > 0.0f) << ((j * NT + t) * 4 + r);
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                    for (int t = 0; t < NT; ++t)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const long m = pixel_of(j, r);
-                            if (m < 0) continue;
-                            static_cast<__bf16*>(out)[size_t(m) * COUT + 16 * t + r16] =
-                                (on >> ((j * NT + t) * 4 + r)) & 1ull ? (__bf16)acc[j][t][r] : (__bf16)0.0f;
-                        }
+                store_tile_bf16<NT>(
+                    tws + wave * 64 * 16,
+                    [&](int j, int t, int r) {
+                        return (on >> ((j * NT + t) * 4 + r)) & 1ull ? (__bf16)acc[j][t][r] : (__bf16)0.0f;
+                    },
+                    static_cast<__bf16*>(out), x0, y0, img, W, H);
+            } else if constexpr (MODE == 0) {
+                store_tile_bf16<NT>(
+                    tws + wave * 64 * 16,
+                    [&](int j, int t, int r) { return (__bf16)fmaxf(acc[j][t][r] + bco[t], 0.0f); },
+                    static_cast<__bf16*>(out), x0, y0, img, W, H);
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -518,13 +574,100 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                         for (int r = 0; r < 4; ++r) {
                             const long m = pixel_of(j, r);
                             if (m < 0) continue;
-                            const size_t o = size_t(m) * COUT + 16 * t + r16;
-                            const float v = acc[j][t][r] + bco[t];
-                            if (MODE == 0) static_cast<__bf16*>(out)[o] = (__bf16)fmaxf(v, 0.0f);
-                            else static_cast<float*>(out)[o] = v;
+                            static_cast<float*>(out)[size_t(m) * COUT + 16 * t + r16] = acc[j][t][r] + bco[t];
                         }
             }
         }
+    }
+}
+
+// The 16-input-channel first layer on the same 2-D tiles (4 image rows x 16
+// pixels per wave). K = 9 taps x 16 channels, padded to 5 k-steps of 32: k-step
+// s holds tap 2s in lanes kq = 0, 1 (channels 0-7, 8-15) and tap 2s + 1 in
+// lanes kq = 2, 3, as in k_conv, so every output sums the same MFMAs in the
+// same order and the results equal k_conv's bit for bit. Each of the 6 image
+// rows a tile needs is loaded once per lane (the lane's 8 channels of its
+// pixel, and of the row's end pixels), where k_conv loads every tap's pixel:
+// a k-step's fragment for row tile j is the input row j + dy + 1 shifted by
+// dx across lanes, formed for both of the k-step's taps and selected by kq.
+template <int COUT, int MODE>
+__global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv16(const __bf16* __restrict__ X,
+                                                                const __bf16* __restrict__ Wt,
+                                                                const float* __restrict__ bias,
+                                                                void* __restrict__ out, int N, int H, int W) {
+    static_assert(MODE == 0, "forward (bias + ReLU) only");
+    constexpr int CIN = 16;
+    constexpr int KP = (9 * CIN + 31) / 32 * 32;
+    constexpr int KS = KP / 32;
+    constexpr int NT = COUT / 16;
+    __shared__ bf16x8 wl[KS * NT * 64];
+    __shared__ __align__(16) __bf16 tws[4 * 64 * 16];  // store_tile_bf16 staging, per wave
+    for (int f = threadIdx.x; f < KS * NT * 64; f += 256) {
+        const int l = f & 63, t = (f >> 6) % NT, s = (f >> 6) / NT;
+        const int co = t * 16 + (l & 15), k = s * 32 + 8 * (l >> 4);
+        wl[f] = k < 9 * CIN ? *reinterpret_cast<const bf16x8*>(Wt + size_t(co) * KP + k) : zero8();
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int half = kq & 1, upper = kq >> 1;  // channel half, and which of the k-step's two taps
+    const int segs = (W + 15) / 16, groups = (H + 3) / 4;
+    const int tiles = N * groups * segs;
+    float bco[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bco[t] = bias[16 * t + r16];
+    for (int tile = int(blockIdx.x) * 4 + wave; tile < tiles; tile += int(gridDim.x) * 4) {
+        const int seg = tile % segs, rest = tile / segs;
+        const int rg = rest % groups, n = rest / groups;
+        const int x0 = seg * 16, y0 = rg * 4;
+        const int x = x0 + r16;
+        const size_t img = size_t(n) * H;
+        // rc[i]: the lane's channel half of pixel x in image row y0 - 1 + i;
+        // re[i]: the row's end pixels (lane 15: x0 - 1, lane 0: x0 + 16).
+        bf16x8 rc[6], re[6];
+        const __bf16* zero = reinterpret_cast<const __bf16*>(kZeroPixel);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const int yy = y0 - 1 + i;
+            const bool row = yy >= 0 && yy < H;
+            const __bf16* line = X + (img + size_t(row ? yy : 0)) * size_t(W) * CIN;
+            const bool okc = row && x < W;
+            const int xe = r16 == 15 ? x0 - 1 : x0 + 16;
+            const bool oke = row && ((r16 == 15 && x0 > 0) || (r16 == 0 && x0 + 16 < W));
+            rc[i] = load_global((okc ? line + size_t(x) * CIN : zero) + 8 * half);
+            re[i] = load_global((oke ? line + size_t(xe) * CIN : zero) + 8 * half);
+        }
+        f32x4 acc[4][NT];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[j][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const int ta = 2 * s, tb = 2 * s + 1;  // the k-step's taps (tb = 9 is padding)
+            bf16x8 a[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                auto shifted = [&](int tap) -> bf16x8 {
+                    const int dyi = tap / 3, dxi = tap % 3;
+                    const bf16x8 c = rc[j + dyi], e = re[j + dyi];
+                    return dxi == 1 ? c : dxi == 0 ? frag_prev(c, e) : frag_next(c, e);
+                };
+                const bf16x8 fa = shifted(ta);
+                const bf16x8 fb = tb < 9 ? shifted(tb) : zero8();
+                a[j] = upper ? fb : fa;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bf16x8 b = wl[(s * NT + t) * 64 + lane];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j][t] = mfma(a[j], b, acc[j][t]);
+            }
+        }
+        // acc[j][t][r] is pixel (x0 + 4 kq + r, y0 + j), channel 16 t + r16.
+        store_tile_bf16<NT>(
+            tws + wave * 64 * 16, [&](int j, int t, int r) { return (__bf16)fmaxf(acc[j][t][r] + bco[t], 0.0f); },
+            static_cast<__bf16*>(out), x0, y0, img, W, H);
     }
 }
 
@@ -659,7 +802,12 @@ __global__ __launch_bounds__(256) void k_kpn_apply(const float* __restrict__ log
     if (p < P) {
         const int n = p / (H * W), rem = p - n * H * W, y = rem / W, x = rem - y * W;
         float w[kKpnTaps], o[3];
-        kpn_filter(logits + size_t(p) * 32, rad, n, y, x, H, W, w, o);
+        kpn_filter(logits + size_t(p) * 32,
+                   [&](int t) {
+                       const int yy = min(max(y + t / 5 - 2, 0), H - 1), xx = min(max(x + t % 5 - 2, 0), W - 1);
+                       return rad + ((size_t(n) * H + yy) * W + xx) * 3;
+                   },
+                   w, o);
         if (out) {
             out[size_t(p) * 3] = o[0];
             out[size_t(p) * 3 + 1] = o[1];
@@ -796,16 +944,19 @@ void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, 
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    // 64 input channels: 2-D tiles of 4 rows x 16 pixels (k_conv64); else
-    // strips of 16 * kConvJ pixels (k_conv).
-    const size_t tiles = cin == 64 ? size_t(n) * size_t((h + 3) / 4) * size_t((wdt + 15) / 16)
-                                   : (P + 16 * kConvJ - 1) / (16 * kConvJ);
+    // 64 input channels, and the 16-channel forward layer: 2-D tiles of 4
+    // rows x 16 pixels (k_conv64, k_conv16); else strips of 16 * kConvJ
+    // pixels (k_conv).
+    const bool tiles2d = cin == 64 || (cin == 16 && mode == 0);
+    const size_t tiles = tiles2d ? size_t(n) * size_t((h + 3) / 4) * size_t((wdt + 15) / 16)
+                                 : (P + 16 * kConvJ - 1) / (16 * kConvJ);
     const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * kConvBlocksPerCU))), block(256);
 #define RM_CONV(CI, CO, MO) \
     hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
 #define RM_CONV64(CO, MO) \
     hipLaunchKernelGGL((k_conv64<CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt, rad)
-    if (cin == 16 && cout == 64 && mode == 0) RM_CONV(16, 64, 0);
+    if (cin == 16 && cout == 64 && mode == 0)
+        hipLaunchKernelGGL((k_conv16<64, 0>), grid, block, 0, st, x, w, b, out, n, h, wdt);
     else if (cin == 64 && cout == 64 && mode == 0) RM_CONV64(64, 0);
     else if (cin == 64 && cout == 32 && mode == 1) RM_CONV64(32, 1);
     else if (cin == 64 && cout == 32 && mode == 3) RM_CONV64(32, 3);
