@@ -579,7 +579,7 @@ def run_denoiser_infer(args) -> dict:
             "data": "synthetic (procedural scene, seed 0)",
             "config": {"workload": "mixed-scene orbit, ReSTIR 4 spp + learned KPN denoiser, 1280x720",
                        "network_ms": round(ms, 3)},
-            "roofline": {"kernel": "kpn network (6 k_conv + k_kpn_apply)", "bound": "mfma", "achieved": round(tflops, 1),
+            "roofline": {"kernel": "kpn network (6 convolutions, the 5x5 filter in the head's epilogue)", "bound": "mfma", "achieved": round(tflops, 1),
                          "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
                          "traffic": None, "flop_per_launch": flop}}
 
