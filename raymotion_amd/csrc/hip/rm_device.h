@@ -303,11 +303,15 @@ struct Tracer {
         }
         // Sort the slots by entry distance (misses last): the optimal
         // 5- and 19-comparator networks for four and eight inputs.
-#define RM_CX(a, b)                                                   \
-    if (tn[b] < tn[a]) {                                              \
-        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;            \
-        const int rr = ref[a]; ref[a] = ref[b]; ref[b] = rr;          \
-        const int kk = kind[a]; kind[a] = kind[b]; kind[b] = kk;      \
+        // Each compare-exchange as selects (no branch): the same swaps.
+#define RM_CX(a, b)                                                                        \
+    {                                                                                      \
+        const bool sw = tn[b] < tn[a];                                                     \
+        const float ta = tn[a], tb = tn[b];                                                \
+        const int ra = ref[a], rb = ref[b], ka = kind[a], kb = kind[b];                    \
+        tn[a] = sw ? tb : ta; tn[b] = sw ? ta : tb;                                        \
+        ref[a] = sw ? rb : ra; ref[b] = sw ? ra : rb;                                      \
+        kind[a] = sw ? kb : ka; kind[b] = sw ? ka : kb;                                    \
     }
         if constexpr (kWide == 4) {
             RM_CX(0, 1) RM_CX(2, 3) RM_CX(0, 2) RM_CX(1, 3) RM_CX(1, 2)
@@ -319,21 +323,19 @@ struct Tracer {
 #undef RM_CX
 #pragma unroll
         for (int c = 0; c < kWide; ++c) T.pend = T.pend || kind[c] > 0;
-        // Inner children: continue with the nearest, stack the rest.
+        // Inner children: continue with the nearest, stack the rest (farthest
+        // first). Without branches: every step stores `next` at the first free
+        // entry (above the live ones, so a store that is not a push is
+        // overwritten by the next one) and advances the stack only on a push.
         int next = -1;
 #pragma unroll
         for (int c = kWide - 1; c >= 0; --c) {
-            if (kind[c] == 0) {
-                if (next >= 0) {
-                    if (T.sp < S) {
-                        stack_put(T.sp, uint32_t(next));
-                        ++T.sp;
-                    } else {
-                        tl.overflow = 1;
-                    }
-                }
-                next = ref[c];
-            }
+            const bool inner = kind[c] == 0;
+            const bool push = inner && next >= 0;
+            if (T.sp < S) stack_put(T.sp, uint32_t(next));
+            if (push && T.sp >= S) tl.overflow = 1;
+            T.sp += push && T.sp < S ? 1 : 0;
+            next = inner ? ref[c] : next;
         }
         if (next < 0) {
             if (T.sp == 0) {
