@@ -269,8 +269,23 @@ struct Tracer {
     __device__ __forceinline__ void trav_node(Trav& T) {
         const float4* n = sc.nodes + size_t(T.node) * kNodeWords;
         float4 w[kNodeWords];
-#pragma unroll
-        for (int k = 0; k < kNodeWords; ++k) w[k] = n[k];
+        // Each lane loads, per axis, the near-plane word (lower bounds for a
+        // positive direction component, upper bounds for a negative one) into
+        // the lo slot and the far-plane word into the hi slot: with a fixed
+        // sign of the inverse direction, fma(lo, i, o) <= fma(hi, i, o) or the
+        // reverse holds for every non-empty child box (rounding is monotonic),
+        // so the slab minima and maxima below are known without min / max.
+        static_assert(kGroups == 1, "one group of four children");
+        const int sx = int(__float_as_uint(T.ix) >> 31), sy = int(__float_as_uint(T.iy) >> 31),
+                  sz = int(__float_as_uint(T.iz) >> 31);
+        w[node_lo_word(0, 0)] = n[node_lo_word(0, 0) + sx];
+        w[node_hi_word(0, 0)] = n[node_hi_word(0, 0) - sx];
+        w[node_lo_word(1, 0)] = n[node_lo_word(1, 0) + sy];
+        w[node_hi_word(1, 0)] = n[node_hi_word(1, 0) - sy];
+        w[node_lo_word(2, 0)] = n[node_lo_word(2, 0) + sz];
+        w[node_hi_word(2, 0)] = n[node_hi_word(2, 0) - sz];
+        w[kNodeRefWord] = n[kNodeRefWord];
+        w[kNodeKindWord] = n[kNodeKindWord];
         ++tl.nodes;
         const float tc = T.h.t + T.h.t * (1.0f / 4096.0f);
         float* tn = T.tn;
@@ -293,8 +308,11 @@ struct Tracer {
                 const float ax = __builtin_fmaf(lx[j], T.ix, T.ox), bx = __builtin_fmaf(hx[j], T.ix, T.ox);
                 const float ay = __builtin_fmaf(ly[j], T.iy, T.oy), by = __builtin_fmaf(hy[j], T.iy, T.oy);
                 const float az = __builtin_fmaf(lz[j], T.iz, T.oz), bz = __builtin_fmaf(hz[j], T.iz, T.oz);
-                const float nr = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
-                const float fr = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), tc));
+                // a* is the near plane's distance and b* the far plane's (an
+                // empty slot, lo = +inf and hi = -inf, fails either way): the
+                // values min(a, b) and max(a, b) would give
+                const float nr = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+                const float fr = fminf(fminf(bx, by), fminf(bz, tc));
                 const bool ok = k4[j] >= 0 && nr <= fr;
                 tn[c] = ok ? nr : __builtin_inff();
                 ref[c] = r4[j];
