@@ -286,6 +286,10 @@ struct Tracer {
         w[node_hi_word(2, 0)] = n[node_hi_word(2, 0) - sz];
         w[kNodeRefWord] = n[kNodeRefWord];
         w[kNodeKindWord] = n[kNodeKindWord];
+        // The stack's top entry, read while the node is in flight: it is the
+        // next node when this one has no inner child to descend into, and then
+        // nothing is pushed below (entry 0 of an empty stack: unused).
+        const int popped = int(stack_get(min(max(T.sp - 1, 0), S - 1)));
         ++tl.nodes;
         const float tc = T.h.t + T.h.t * (1.0f / 4096.0f);
         float* tn = T.tn;
@@ -345,25 +349,23 @@ struct Tracer {
         // first). Without branches: every step stores `next` at the first free
         // entry (above the live ones, so a store that is not a push is
         // overwritten by the next one) and advances the stack only on a push.
+        // The instance has more entries than the tree's stack bound (launchers
+        // pick S > bound), so that entry exists; the index is clamped anyway.
         int next = -1;
 #pragma unroll
         for (int c = kWide - 1; c >= 0; --c) {
             const bool inner = kind[c] == 0;
             const bool push = inner && next >= 0;
-            if (T.sp < S) stack_put(T.sp, uint32_t(next));
-            if (push && T.sp >= S) tl.overflow = 1;
-            T.sp += push && T.sp < S ? 1 : 0;
+            stack_put(min(T.sp, S - 1), uint32_t(next));
+            T.sp += push ? 1 : 0;
             next = inner ? ref[c] : next;
         }
-        if (next < 0) {
-            if (T.sp == 0) {
-                T.last = true;
-            } else {
-                --T.sp;
-                next = int(stack_get(T.sp));
-            }
-        }
-        T.node = next;
+        if (T.sp >= S) tl.overflow = 1;  // only a tree deeper than its stated bound
+        // No inner child: the top of the stack, or the end.
+        const bool pop = next < 0 && T.sp > 0;
+        T.last = next < 0 && T.sp == 0;
+        T.sp -= pop ? 1 : 0;
+        T.node = pop ? popped : next;
     }
     __device__ __forceinline__ void stack_put(int i, uint32_t v) {
         if constexpr (S > kLdsStack) {

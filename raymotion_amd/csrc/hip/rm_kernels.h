@@ -14,16 +14,18 @@ namespace rmh {
 // LDS, or for deeper trees kSpillStack entries of which those past kLdsStack
 // live in per-lane scratch memory. (A 48-entry LDS stack with deeper entries
 // in scratch was measured 5-7 % slower on restir_shade, although rays rarely
-// reach it, so trees that fit in LDS keep the LDS-only instances.)
-constexpr int kMaxStack = kSpillStack;
+// reach it, so trees that fit in LDS keep the LDS-only instances.) An
+// instance holds at least one entry more than the depth: the traversal stores
+// into the entry above the live ones on every step (Tracer::trav_node).
+constexpr int kMaxStack = kSpillStack - 1;
 
 // Calls f(std::integral_constant<int, S>{}) with the smallest stack capacity
-// S in {32, 48, 64, kSpillStack} that holds `stack` entries.
+// S in {32, 48, 64, kSpillStack} that exceeds `stack`.
 template <class F>
 inline auto with_stack(int stack, F&& f) {
-    if (stack <= 32) return f(std::integral_constant<int, 32>{});
-    if (stack <= 48) return f(std::integral_constant<int, 48>{});
-    if (stack <= kLdsStack) return f(std::integral_constant<int, kLdsStack>{});
+    if (stack < 32) return f(std::integral_constant<int, 32>{});
+    if (stack < 48) return f(std::integral_constant<int, 48>{});
+    if (stack < kLdsStack) return f(std::integral_constant<int, kLdsStack>{});
     return f(std::integral_constant<int, kSpillStack>{});
 }
 

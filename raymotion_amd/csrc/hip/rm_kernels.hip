@@ -362,13 +362,13 @@ inline unsigned blocks_for(size_t n) { return unsigned((n + kBlock - 1) / kBlock
 
 #define RM_STACK_DISPATCH(stack, launch) \
     do {                                 \
-        if ((stack) <= 32) {             \
+        if ((stack) < 32) {              \
             constexpr int S = 32;        \
             launch;                      \
-        } else if ((stack) <= 48) {      \
+        } else if ((stack) < 48) {       \
             constexpr int S = 48;        \
             launch;                      \
-        } else if ((stack) <= 64) {      \
+        } else if ((stack) < 64) {       \
             constexpr int S = 64;        \
             launch;                      \
         } else {                         \
