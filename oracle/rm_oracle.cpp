@@ -25,6 +25,7 @@
 // Everything is templated on the real type R: R=float is the FP32 parity
 // oracle, R=double the FP64 reference renderer.
 #include "oracle_abi.h"
+#include "../raymotion_amd/csrc/common/rm_fdiv.h"
 #include "../raymotion_amd/csrc/common/rm_sincos.h"
 
 #include <algorithm>
@@ -34,6 +35,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <thread>
 #include <vector>
 
@@ -254,6 +256,14 @@ template <class R> struct Tracer {
         hx *= d; hy *= d; hz *= d;
     }
 
+    // x / pi: in FP32 the two-term-reciprocal division both sides share
+    // (rm_fdiv.h; correctly rounded for 2^-100 <= |x| < 2^100), in FP64 the
+    // IEEE division.
+    static R div_pi(R x) {
+        if constexpr (std::is_same<R, float>::value) return rm_div_pi(x);
+        else return x / R(3.141592653589793);
+    }
+
     static Bsdf evaluate_opaque_bsdf(R wi_x, R wi_y, R wi_z, R wo_x, R wo_y, R wo_z, R n_x, R n_y, R n_z,
                               R alpha2, R metallic, R spec_prob, R ar, R ag, R ab) {
         const R kpi = R(3.141592653589793);
@@ -284,10 +294,10 @@ template <class R> struct Tracer {
         const R fr = f0r + (1 - f0r) * om, fg = f0g + (1 - f0g) * om, fb = f0b + (1 - f0b) * om;
         const R kdr = ar * di * (1 - fr), kdg = ag * di * (1 - fg), kdb = ab * di * (1 - fb);
         const R base = D * G / (4 * cos_i * cos_o);
-        out.r = kdr / kpi + base * fr;
-        out.g = kdg / kpi + base * fg;
-        out.b = kdb / kpi + base * fb;
-        const R pdf_d = cos_i / kpi;
+        out.r = div_pi(kdr) + base * fr;
+        out.g = div_pi(kdg) + base * fg;
+        out.b = div_pi(kdb) + base * fb;
+        const R pdf_d = div_pi(cos_i);
         const R pdf_s = D * cos_h / (4 * vo_h);
         out.pdf = (1 - spec_prob) * pdf_d + spec_prob * pdf_s;
         return out;
@@ -335,7 +345,7 @@ template <class R> struct Tracer {
         for (int c = 0; c < 3; ++c) {
             const R f0 = R(0.04) * di + a[c] * metallic;
             const R F = f0 + (1 - f0) * om;
-            out.da[c] = (di * (1 - F) - a[c] * di * dF) / kpi + base * dF;
+            out.da[c] = div_pi(di * (1 - F) - a[c] * di * dF) + base * dF;
             out.dl[c] = dbase * F;
         }
         return out;
@@ -2021,6 +2031,32 @@ void orc_rng_draws(uint64_t s, int n, float* out, uint32_t* states) {
 
 void orc_sincos(const float* x, size_t n, float* s, float* c) {
     for (size_t i = 0; i < n; ++i) rm_sincosf(x[i], &s[i], &c[i]);
+}
+
+// Every float whose bit pattern with the sign cleared lies in [lo, hi):
+// the number of them (both signs) for which rm_div_pi differs from the
+// IEEE quotient x / pi (a pair of NaNs counts as equal).
+uint64_t orc_check_div_pi(uint32_t lo, uint32_t hi, int threads) {
+    const float kpi = 3.14159265358979323846f;
+    std::atomic<uint64_t> bad{0};
+    const uint32_t n = hi > lo ? hi - lo : 0, chunk = 1u << 20;
+    orc::parallel_for((n + chunk - 1) / chunk, threads, [&](uint32_t c) {
+        uint64_t local = 0;
+        const uint32_t a = lo + c * chunk, b = std::min(hi, a + chunk);
+        for (uint32_t u = a; u < b; ++u)
+            for (uint32_t sign = 0; sign < 2; ++sign) {
+                const uint32_t bits = u | (sign << 31);
+                float x;
+                std::memcpy(&x, &bits, 4);
+                const float want = x / kpi, got = rm_div_pi(x);
+                uint32_t wb, gb;
+                std::memcpy(&wb, &want, 4);
+                std::memcpy(&gb, &got, 4);
+                if (wb != gb && !(want != want && got != got)) ++local;
+            }
+        bad += local;
+    });
+    return bad.load();
 }
 
 // Closed-form test hooks (FP32 oracle functions).

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../common/rm_fdiv.h"
 #include "../common/rm_sincos.h"
 #include "rm_layout.h"
 
@@ -639,9 +640,9 @@ struct Tracer {
         const float fr = f0r + (1.0f - f0r) * om, fg = f0g + (1.0f - f0g) * om, fb = f0b + (1.0f - f0b) * om;
         const float base = D * G / (4.0f * cos_i * cos_o);
         const float dF = metallic * (1.0f - om);
-        out.da_r = (di * (1.0f - fr) - ar * di * dF) / kpi + base * dF;
-        out.da_g = (di * (1.0f - fg) - ag * di * dF) / kpi + base * dF;
-        out.da_b = (di * (1.0f - fb) - ab * di * dF) / kpi + base * dF;
+        out.da_r = rm_div_pi(di * (1.0f - fr) - ar * di * dF) + base * dF;
+        out.da_g = rm_div_pi(di * (1.0f - fg) - ag * di * dF) + base * dF;
+        out.da_b = rm_div_pi(di * (1.0f - fb) - ab * di * dF) + base * dF;
         const float dD = (dd - 2.0f * alpha2 * cos_h * cos_h) / (kpi * dd * dd * dd);
         const float dg1 = -cos_o * (1.0f - cos_o * cos_o) / (so * (cos_o + so) * (cos_o + so));
         const float dg2 = -cos_i * (1.0f - cos_i * cos_i) / (si * (cos_i + si) * (cos_i + si));
@@ -683,10 +684,10 @@ struct Tracer {
         const float fr = f0r + (1.0f - f0r) * om, fg = f0g + (1.0f - f0g) * om, fb = f0b + (1.0f - f0b) * om;
         const float kdr = ar * di * (1.0f - fr), kdg = ag * di * (1.0f - fg), kdb = ab * di * (1.0f - fb);
         const float base = D * G / (4.0f * cos_i * cos_o);
-        out.r = kdr / kpi + base * fr;
-        out.g = kdg / kpi + base * fg;
-        out.b = kdb / kpi + base * fb;
-        const float pdf_d = cos_i / kpi;
+        out.r = rm_div_pi(kdr) + base * fr;
+        out.g = rm_div_pi(kdg) + base * fg;
+        out.b = rm_div_pi(kdb) + base * fb;
+        const float pdf_d = rm_div_pi(cos_i);
         const float pdf_s = D * cos_h / (4.0f * vo_h);
         out.pdf = (1.0f - spec_prob) * pdf_d + spec_prob * pdf_s;
         return out;

@@ -51,6 +51,7 @@ def orc() -> ctypes.CDLL:
                                                        ctypes.c_int, _P]),
             ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
+            ("orc_check_div_pi", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
             ("orc_trace_wide", ctypes.c_int, [_P, _SZ, _P, _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
                                               _P]),
             ("orc_restir_merge", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame), ctypes.c_uint64,
@@ -255,6 +256,12 @@ def sincos(x):
     c = np.empty_like(x)
     orc().orc_sincos(ptr(x), len(x), ptr(s), ptr(c))
     return s, c
+
+
+def div_pi_mismatches(lo_bits: int, hi_bits: int, threads: int = 8) -> int:
+    """Floats with |x|'s bits in [lo_bits, hi_bits), both signs, for which the
+    shared division by pi differs from the IEEE quotient x / pi."""
+    return int(orc().orc_check_div_pi(lo_bits, hi_bits, threads))
 
 
 class RefScene:

@@ -71,6 +71,17 @@ def test_sincos_within_one_ulp_of_libm():
         assert np.mean(got == want.astype(np.float32)) > 0.9999
 
 
+def test_div_pi_correctly_rounded():
+    """The BSDF's division by pi (a product with a two-term reciprocal, shared
+    by the kernels and the oracle) equals the IEEE quotient x / pi for every
+    float with 2^-100 <= |x| < 2^100 (exhaustive, both signs), and for 0,
+    infinities and NaN."""
+    lo, hi = (127 - 100) << 23, (127 + 100) << 23
+    assert ob.div_pi_mismatches(lo, hi) == 0
+    assert ob.div_pi_mismatches(0, 1) == 0            # +-0
+    assert ob.div_pi_mismatches(0x7F800000, 0x7F800002) == 0  # inf, a NaN
+
+
 def _unit(v):
     v = np.asarray(v, np.float64)
     return v / np.linalg.norm(v, axis=-1, keepdims=True)
