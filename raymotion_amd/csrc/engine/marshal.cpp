@@ -2,6 +2,7 @@
 #include "marshal.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <cstdlib>
 #include <string>
 #include <unordered_map>
@@ -36,20 +37,42 @@ rm_scene_desc MarshalledScene::desc(const Scene& scene) const {
     return d;
 }
 
+rmkeys::RecordKeys MarshalledScene::keys() const {
+    rmkeys::RecordKeys k;
+    k.kp = kp.data();
+    k.kn = kn.data();
+    k.km = km.data();
+    k.chunks = kp.size();
+    return k;
+}
+
 void marshal_scene(const Scene& scene, MarshalledScene& out) {
     const size_t n = scene.tris.size();
+    const size_t kept = std::min(out.tris.size(), n);  // records of the last marshal still in place
     out.tris.resize(n);
     out.texels.clear();
-    // Field conversion in parallel chunks; each chunk lists its textured
-    // triangles, which then get atlas offsets in triangle order.
-    constexpr size_t kChunk = size_t(1) << 15;
-    const size_t chunks = (n + kChunk - 1) / kChunk;
+    // Field conversion in parallel chunks (the records' key chunks). A record
+    // equal to the one already in place is not written again, so re-marshalling
+    // an unchanged scene only reads; each chunk's keys are taken from the
+    // records as they are converted. Each chunk lists its textured triangles,
+    // which then get atlas offsets in triangle order (their chunks' keys are
+    // taken after that).
+    constexpr size_t kChunk = rmkeys::kKeyChunk;
+    const size_t chunks = rmkeys::key_chunks(n);
+    out.kp.resize(chunks);
+    out.kn.resize(chunks);
+    out.km.resize(chunks);
     std::vector<std::vector<uint32_t>> textured(chunks);
+    auto store = [&](size_t i, const rm_triangle& t) {
+        rm_triangle& d = out.tris[i];
+        if (i >= kept || std::memcmp(&d, &t, sizeof t) != 0) d = t;
+    };
     rmpar::parallel_chunks(chunks, [&](size_t c) {
         const size_t a = c * kChunk, b = std::min(n, a + kChunk);
+        rmkeys::RecordMix mix(c);
         for (size_t i = a; i < b; ++i) {
             const Triangle& s = scene.tris[i];
-            rm_triangle& t = out.tris[i];
+            rm_triangle t;
             t.v0 = pack(s.v0); t.v1 = pack(s.v1); t.v2 = pack(s.v2);
             t.n = pack(s.n); t.nn = pack(s.nn);
             t.n0 = pack(s.n0); t.n1 = pack(s.n1); t.n2 = pack(s.n2);
@@ -63,14 +86,27 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
             t.texture_offset = -1;
             t.texture_width = 0;
             t.texture_height = 0;
-            if (s.texture) textured[c].push_back(uint32_t(i));
+            if (s.texture) {
+                // atlas fields as in place, until the atlas pass below
+                if (i < kept) {
+                    const rm_triangle& d = out.tris[i];
+                    t.texture_offset = d.texture_offset;
+                    t.texture_width = d.texture_width;
+                    t.texture_height = d.texture_height;
+                }
+                textured[c].push_back(uint32_t(i));
+            }
+            store(i, t);
+            mix.add(reinterpret_cast<const unsigned char*>(&t));
         }
+        out.kp[c] = mix.p.done();
+        out.kn[c] = mix.n.done();
+        out.km[c] = mix.m.done();
     });
     std::unordered_map<const Texture*, int> atlas;
     for (const auto& list : textured)
         for (uint32_t i : list) {
             const Triangle& s = scene.tris[i];
-            rm_triangle& t = out.tris[i];
             auto ins = atlas.emplace(s.texture.get(), int(out.texels.size() / 3));
             if (ins.second)
                 for (const Vec3& p : s.texture->pixels) {
@@ -78,10 +114,21 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
                     out.texels.push_back(float(p.y));
                     out.texels.push_back(float(p.z));
                 }
+            rm_triangle t = out.tris[i];
             t.texture_offset = ins.first->second;
             t.texture_width = s.texture->width;
             t.texture_height = s.texture->height;
+            store(i, t);
         }
+    rmpar::parallel_chunks(chunks, [&](size_t c) {
+        if (textured[c].empty()) return;
+        const size_t a = c * kChunk, b = std::min(n, a + kChunk);
+        rmkeys::RecordMix mix(c);
+        for (size_t i = a; i < b; ++i) mix.add(reinterpret_cast<const unsigned char*>(&out.tris[i]));
+        out.kp[c] = mix.p.done();
+        out.kn[c] = mix.n.done();
+        out.km[c] = mix.m.done();
+    });
     // Preorder with escape links: a node is followed by its left subtree, then
     // its right subtree; `right` holds the index just past its own subtree.
     out.nodes.clear();

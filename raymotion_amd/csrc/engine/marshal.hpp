@@ -8,6 +8,7 @@
 
 #include "raymotion/pathtrace.hpp"
 #include "rm_hip.h"
+#include "rm_keys.h"
 
 namespace raymotion {
 
@@ -38,12 +39,16 @@ struct MarshalledScene {
     std::vector<rm_bvh_node> nodes;
     std::vector<float> cdf;
     std::vector<float> texels;
+    // per-chunk key values of tris (rm_keys.h), taken while marshalling
+    std::vector<uint64_t> kp, kn, km;
     rm_scene_desc desc(const Scene& scene) const;
+    rmkeys::RecordKeys keys() const;
 };
 
 // FP64 triangles -> 176-byte FP32 records with a deduplicated texture atlas
 // (metal.mm:109-128), binary BVH -> preorder array with escape links
-// (metal.mm:129-144), light CDF -> FP32 (metal.mm:145).
+// (metal.mm:129-144), light CDF -> FP32 (metal.mm:145). `out` may hold the
+// previous marshal's arrays: records that did not change are not rewritten.
 void marshal_scene(const Scene& scene, MarshalledScene& out);
 
 // Reference host-side parameter handling (metal.mm:146-170, 251-252, 269-271):

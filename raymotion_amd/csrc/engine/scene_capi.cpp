@@ -7,6 +7,7 @@
 #include "rm_parallel.h"
 #include "raymotion/bvh.hpp"
 #include "raymotion/renderer.hpp"
+#include "rm_internal.h"
 #include "rm_scene.h"
 
 struct rm_scene {
@@ -169,7 +170,7 @@ int rm_scene_upload(rm_ctx* ctx, const rm_scene* s) {
     raymotion::MarshalledScene& m = s->marshalled;
     raymotion::marshal_scene(s->scene, m);
     const rm_scene_desc d = m.desc(s->scene);
-    return rm_set_scene(ctx, &d);
+    return rm_set_scene_keyed(ctx, &d, m.keys());
 }
 
 int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild) {
@@ -177,7 +178,7 @@ int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild) {
     raymotion::MarshalledScene& m = s->marshalled;
     raymotion::marshal_scene(s->scene, m);
     const rm_scene_desc d = m.desc(s->scene);
-    return rm_set_scene_gpu_tree(ctx, &d, rebuild);
+    return rm_set_scene_gpu_tree_keyed(ctx, &d, rebuild, m.keys());
 }
 
 static rm_camera to_c(const raymotion::Camera& c) {

@@ -18,6 +18,7 @@
 #include "rm_kpn.h"
 #include "rm_layout.h"
 #include "rm_pack.h"
+#include "rm_internal.h"
 
 // Work space of rm_trace_device, the entry point that allocates nothing: the
 // traversal counters of one call (kNumCounters words, at most kScratchWords =
@@ -536,13 +537,21 @@ int rm_get_stats(rm_ctx* c, rm_stats* out) {
     return RM_OK;
 }
 
-int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
+// The records' keys: from the chunk values the caller took while writing
+// them (pre), else hashed here.
+static SceneKeys keys_of(const rm_scene_desc& s, const rmkeys::RecordKeys* pre) {
+    if (pre && pre->chunks == rmkeys::key_chunks(s.ntri) && (pre->chunks == 0 || (pre->kp && pre->kn && pre->km)))
+        return scene_keys_from_chunks(s, *pre);
+    return scene_keys(s);
+}
+
+static int set_scene(rm_ctx* c, const rm_scene_desc* s, const rmkeys::RecordKeys* pre) {
     if (!c || !s) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
     std::string perr;
     if (!validate_scene(*s, false, false, false, perr)) return c->fail(RM_ERR_INVALID, perr);
-    const SceneKeys keys = scene_keys(*s);
+    const SceneKeys keys = keys_of(*s, pre);
     const bool had = c->has_scene;
     if (had && keys == c->keys && !c->gpu_tree) {
         c->last_refit = 0;
@@ -642,7 +651,10 @@ int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) {
     return RM_OK;
 }
 
-int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_request) {
+int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) { return set_scene(c, s, nullptr); }
+
+static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_request,
+                              const rmkeys::RecordKeys* pre) {
     if (!c || !s_in) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
@@ -654,7 +666,7 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
     std::string perr;
     if (!validate_scene(*s, false, false, false, perr)) return c->fail(RM_ERR_INVALID, perr);
     if (s->ntri == 0 || s->ntri > size_t(INT32_MAX / 4)) return c->fail(RM_ERR_INVALID, "GPU tree build needs 1 to 2^29 triangles");
-    const SceneKeys keys = scene_keys(*s);
+    const SceneKeys keys = keys_of(*s, pre);
     const bool had = c->has_scene && c->gpu_tree;
     if (had && keys == c->keys && !rebuild_request) {
         c->last_refit = 0;
@@ -764,7 +776,20 @@ int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_requ
     return RM_OK;
 }
 
+int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s, int rebuild_request) {
+    return set_scene_gpu_tree(c, s, rebuild_request, nullptr);
+}
+
 }  // extern "C"
+
+int rm_set_scene_keyed(rm_ctx* c, const rm_scene_desc* s, const rmkeys::RecordKeys& keys) {
+    return set_scene(c, s, &keys);
+}
+
+int rm_set_scene_gpu_tree_keyed(rm_ctx* c, const rm_scene_desc* s, int rebuild_request,
+                                const rmkeys::RecordKeys& keys) {
+    return set_scene_gpu_tree(c, s, rebuild_request, &keys);
+}
 
 namespace {
 

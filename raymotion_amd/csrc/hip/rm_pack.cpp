@@ -123,30 +123,8 @@ void collapse(const BinTree& t, PackedScene& out) {
 
 using rmpar::parallel_chunks;
 
-constexpr size_t kChunk = size_t(1) << 15;  // triangles per parallel chunk
-constexpr uint64_t kMul = 0x100000001B3ULL;
-
-// Four independent multiply-xor lanes over 8-byte words, so consecutive words
-// do not wait for each other's multiply.
-struct Mix4 {
-    uint64_t h[4];
-    explicit Mix4(uint64_t seed) {
-        for (int i = 0; i < 4; ++i) h[i] = seed + 0x9E3779B97F4A7C15ULL * uint64_t(i + 1);
-    }
-    void words(const unsigned char* p, size_t nwords) {
-        for (size_t k = 0; k < nwords; ++k) {
-            uint64_t w;
-            std::memcpy(&w, p + 8 * k, 8);
-            const uint64_t x = (h[k & 3] ^ w) * kMul;
-            h[k & 3] = x ^ (x >> 29);
-        }
-    }
-    uint64_t done() const { return fold(fold(fold(h[0], h[1]), h[2]), h[3]); }
-    static uint64_t fold(uint64_t a, uint64_t b) {
-        const uint64_t x = (a ^ b) * 0xBF58476D1CE4E5B9ULL;
-        return x ^ (x >> 31);
-    }
-};
+constexpr size_t kChunk = rmkeys::kKeyChunk;  // triangles per parallel chunk
+using rmkeys::Mix4;
 
 // Hash of a byte array: 1 MiB pieces hashed in parallel, folded in order.
 uint64_t hash_span(const void* data, size_t n, uint64_t seed) {
@@ -169,39 +147,17 @@ uint64_t hash_span(const void* data, size_t n, uint64_t seed) {
 
 }  // namespace
 
-SceneKeys scene_keys(const rm_scene_desc& s) {
+SceneKeys scene_keys_from_chunks(const rm_scene_desc& s, const rmkeys::RecordKeys& rk) {
     SceneKeys k;
     k.ntri = s.ntri;
     k.nnode = s.nnode;
     k.nlight = s.nlight;
     k.ntexel = s.ntexel;
-    static_assert(sizeof(rm_triangle) == 176, "triangle record");
-    const size_t chunks = (s.ntri + kChunk - 1) / kChunk;
-    std::vector<uint64_t> kp(chunks), kn(chunks), km(chunks);
-    const unsigned char* base = reinterpret_cast<const unsigned char*>(s.tris);
-    parallel_chunks(chunks, [&](size_t c) {
-        Mix4 mp(0x5EED01 ^ c), mn(0x5EED02 ^ c), mm(0x5EED03 ^ c);
-        const size_t a = c * kChunk, b = std::min(s.ntri, a + kChunk);
-        for (size_t i = a; i < b; ++i) {
-            const unsigned char* r = base + i * sizeof(rm_triangle);
-            uint32_t split[2];  // bytes 32..35 (v2.z) and 36..39 (n.x)
-            std::memcpy(split, r + 32, 8);
-            const uint64_t w_pos = split[0], w_nrm = split[1];
-            mp.words(r, 4);  // bytes 0..31
-            mp.words(reinterpret_cast<const unsigned char*>(&w_pos), 1);
-            mn.words(reinterpret_cast<const unsigned char*>(&w_nrm), 1);
-            mn.words(r + 40, 7);   // bytes 40..95
-            mm.words(r + 96, 10);  // bytes 96..175
-        }
-        kp[c] = mp.done();
-        kn[c] = mn.done();
-        km[c] = mm.done();
-    });
     uint64_t hp = 0x51ED0001 ^ s.ntri, hn = 0x51ED0002 ^ s.ntri, hm = 0x51ED0003 ^ s.ntri;
-    for (size_t c = 0; c < chunks; ++c) {
-        hp = Mix4::fold(hp, kp[c]);
-        hn = Mix4::fold(hn, kn[c]);
-        hm = Mix4::fold(hm, km[c]);
+    for (size_t c = 0; c < rk.chunks; ++c) {
+        hp = Mix4::fold(hp, rk.kp[c]);
+        hn = Mix4::fold(hn, rk.kn[c]);
+        hm = Mix4::fold(hm, rk.km[c]);
     }
     k.positions = hp;
     k.normals = hn;
@@ -213,6 +169,27 @@ SceneKeys scene_keys(const rm_scene_desc& s) {
     k.lights = hash_span(s.light_cdf, s.nlight * sizeof(float),
                          hash_span(s.light_tri, s.nlight * sizeof(int32_t), 0x11647 ^ s.nlight) ^ lt);
     return k;
+}
+
+SceneKeys scene_keys(const rm_scene_desc& s) {
+    static_assert(sizeof(rm_triangle) == 176, "triangle record");
+    const size_t chunks = rmkeys::key_chunks(s.ntri);
+    std::vector<uint64_t> kp(chunks), kn(chunks), km(chunks);
+    const unsigned char* base = reinterpret_cast<const unsigned char*>(s.tris);
+    parallel_chunks(chunks, [&](size_t c) {
+        rmkeys::RecordMix mix(c);
+        const size_t a = c * kChunk, b = std::min(s.ntri, a + kChunk);
+        for (size_t i = a; i < b; ++i) mix.add(base + i * sizeof(rm_triangle));
+        kp[c] = mix.p.done();
+        kn[c] = mix.n.done();
+        km[c] = mix.m.done();
+    });
+    rmkeys::RecordKeys rk;
+    rk.kp = kp.data();
+    rk.kn = kn.data();
+    rk.km = km.data();
+    rk.chunks = chunks;
+    return scene_keys_from_chunks(s, rk);
 }
 
 uint64_t history_key(const SceneKeys& k, float light_total) {

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "rm_hip.h"
+#include "rm_keys.h"
 
 namespace rmh {
 
@@ -87,6 +88,9 @@ struct SceneKeys {
     }
 };
 SceneKeys scene_keys(const rm_scene_desc& s);
+// The same keys from the records' per-chunk values (taken by the marshal
+// while it wrote s.tris); rk.chunks must be rmkeys::key_chunks(s.ntri).
+SceneKeys scene_keys_from_chunks(const rm_scene_desc& s, const rmkeys::RecordKeys& rk);
 
 // Temporal-history key: the whole triangle records, the triangle and light
 // counts and the light total (the fields metal.mm:241-248 hashes).
