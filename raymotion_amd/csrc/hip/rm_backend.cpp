@@ -268,6 +268,27 @@ bool upload(rm_ctx* c, DevBuf& b, const V& v) {
     return hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;
 }
 
+// Packs the triangles' geometry (pack_geometry) and copies it to the device
+// in kGeometrySlices ranges: each range goes to the device from the pinned
+// packing buffers while the next one is packed. geo_dev receives the vertex
+// records only when `positions`; nrm and nn always.
+constexpr size_t kGeometrySlices = 8;
+bool pack_upload_geometry(rm_ctx* c, const rm_scene_desc& s, PackedScene& ps, DevBuf& geo_dev, bool positions) {
+    const size_t n = s.ntri;
+    if (!geo_dev.ensure(n * kTriGeoWords * sizeof(F4)) || !c->nrm.ensure(n * kTriNrmWords * sizeof(F4)) ||
+        !c->nn.ensure(n * sizeof(F4)))
+        return false;
+    auto copy = [&](DevBuf& d, const F4* src, size_t per_tri, size_t first, size_t count) {
+        const size_t bytes = per_tri * sizeof(F4);
+        return hipMemcpyAsync(static_cast<char*>(d.ptr) + first * bytes, src + first * per_tri, count * bytes,
+                              hipMemcpyHostToDevice, c->stream) == hipSuccess;
+    };
+    return pack_geometry_sliced(s, ps, kGeometrySlices, [&](size_t first, size_t count) {
+        return (!positions || copy(geo_dev, ps.geo.data(), kTriGeoWords, first, count)) &&
+               copy(c->nrm, ps.nrm.data(), kTriNrmWords, first, count) && copy(c->nn, ps.nn.data(), 1, first, count);
+    });
+}
+
 // Wavefront buffers for `pixels` slots.
 bool wave_ensure(rm_ctx* c, size_t pixels, WaveBufs& wb) {
     const size_t bytes = wave_slot_words(true) * pixels * sizeof(uint32_t);
@@ -580,15 +601,13 @@ static int set_scene(rm_ctx* c, const rm_scene_desc* s, const rmkeys::RecordKeys
                                                std::to_string(ps.stack_need) + " entries (at most " +
                                                std::to_string(kMaxStack) + ")");
     }
-    if (geometry) pack_geometry(*s, ps);
-    pack_lights(*s, ps);
-    if (materials) pack_materials(*s, ps);
-
     c->has_scene = false;  // until every part is resident again
     const bool prof = c->profiling;
     if (prof) (void)hipEventRecord(c->ev[8], c->stream);
-    bool ok = upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
-    if (ok && geometry) ok = upload(c, c->geo, ps.geo) && upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn);
+    bool ok = !geometry || pack_upload_geometry(c, *s, ps, c->geo, true);
+    pack_lights(*s, ps);
+    if (materials) pack_materials(*s, ps);
+    ok = ok && upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
     if (ok && materials)
         ok = upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
              upload(c, c->texels, ps.texels);
@@ -684,18 +703,17 @@ static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_
     if (!validate_scene(*s, false, materials, true, perr)) return c->fail(RM_ERR_INVALID, perr);
 
     PackedScene& ps = c->pack;
-    if (moved || normals) pack_geometry(*s, ps);  // no tree: records in scene order
-    pack_lights(*s, ps);
-    if (materials) pack_materials(*s, ps);
     const int n = int(s->ntri);
     const size_t max_nodes = size_t(std::max(n - 1, 1));
 
     c->has_scene = false;
     const bool prof = c->profiling;
     if (prof) (void)hipEventRecord(c->ev[8], c->stream);
-    bool ok = upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
-    if (ok && moved) ok = upload(c, c->geo_scene, ps.geo);
-    if (ok && (moved || normals)) ok = upload(c, c->nrm, ps.nrm) && upload(c, c->nn, ps.nn);
+    // no tree: records in scene order
+    bool ok = !(moved || normals) || pack_upload_geometry(c, *s, ps, c->geo_scene, moved);
+    pack_lights(*s, ps);
+    if (materials) pack_materials(*s, ps);
+    ok = ok && upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
     if (ok && materials)
         ok = upload(c, c->matid, ps.matid) && upload(c, c->mats, ps.mats) && upload(c, c->uv, ps.uv) &&
              upload(c, c->texels, ps.texels);

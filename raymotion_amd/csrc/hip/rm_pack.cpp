@@ -3,8 +3,10 @@
 #include "rm_pack.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 
 #include "rm_layout.h"
@@ -253,28 +255,46 @@ void pack_tree(const rm_scene_desc& s, PackedScene& out) {
     collapse(bt, out);
 }
 
-void pack_geometry(const rm_scene_desc& s, PackedScene& out) {
+bool pack_geometry_sliced(const rm_scene_desc& s, PackedScene& out, size_t slices,
+                          const std::function<bool(size_t, size_t)>& sink) {
     out.geo.resize(s.ntri * kTriGeoWords);
     out.nrm.resize(s.ntri * kTriNrmWords);
     out.nn.resize(s.ntri);
-    parallel_chunks((s.ntri + kChunk - 1) / kChunk, [&](size_t c) {
-        const size_t a = c * kChunk, b = std::min(s.ntri, a + kChunk);
+    constexpr size_t kTask = 8192;  // triangles per parallel task
+    const size_t tasks = (s.ntri + kTask - 1) / kTask;
+    slices = std::max<size_t>(1, std::min(slices, tasks));
+    const size_t per_slice = tasks > 0 ? (tasks + slices - 1) / slices : 1;
+    // Tasks are claimed in order; the worker that finishes a slice's last task
+    // hands the slice to the sink (which must be callable from any thread).
+    std::vector<std::atomic<size_t>> done((tasks + per_slice - 1) / per_slice);
+    for (auto& d : done) d.store(0);
+    std::atomic<bool> ok{true};
+    parallel_chunks(tasks, [&](size_t t) {
+        const size_t a = t * kTask, b = std::min(s.ntri, a + kTask);
         for (size_t k = a; k < b; ++k) {  // vertices in leaf order for traversal
             const int32_t j = s.nnode > 0 ? s.bvh_tri[k] : int32_t(k);
-            const rm_triangle& t = s.tris[j];
-            out.geo[3 * k] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
-            out.geo[3 * k + 1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
-            out.geo[3 * k + 2] = {t.v2.z, ibf(j), 0.0f, 0.0f};
+            const rm_triangle& tr = s.tris[j];
+            out.geo[3 * k] = {tr.v0.x, tr.v0.y, tr.v0.z, tr.v1.x};
+            out.geo[3 * k + 1] = {tr.v1.y, tr.v1.z, tr.v2.x, tr.v2.y};
+            out.geo[3 * k + 2] = {tr.v2.z, ibf(j), 0.0f, 0.0f};
         }
         for (size_t i = a; i < b; ++i) {  // shading normals in scene order
-            const rm_triangle& t = s.tris[i];
-            out.nrm[3 * i] = {t.n0.x, t.n0.y, t.n0.z, t.n.x};
-            out.nrm[3 * i + 1] = {t.n1.x, t.n1.y, t.n1.z, t.n.y};
-            out.nrm[3 * i + 2] = {t.n2.x, t.n2.y, t.n2.z, t.n.z};
-            out.nn[i] = {t.nn.x, t.nn.y, t.nn.z, 0.0f};
+            const rm_triangle& tr = s.tris[i];
+            out.nrm[3 * i] = {tr.n0.x, tr.n0.y, tr.n0.z, tr.n.x};
+            out.nrm[3 * i + 1] = {tr.n1.x, tr.n1.y, tr.n1.z, tr.n.y};
+            out.nrm[3 * i + 2] = {tr.n2.x, tr.n2.y, tr.n2.z, tr.n.z};
+            out.nn[i] = {tr.nn.x, tr.nn.y, tr.nn.z, 0.0f};
+        }
+        const size_t sl = t / per_slice, t0 = sl * per_slice, nt = std::min(per_slice, tasks - t0);
+        if (done[sl].fetch_add(1) + 1 == nt && sink) {
+            const size_t first = t0 * kTask, last = std::min(s.ntri, (t0 + nt) * kTask);
+            if (!sink(first, last - first)) ok.store(false);
         }
     });
+    return ok.load();
 }
+
+void pack_geometry(const rm_scene_desc& s, PackedScene& out) { (void)pack_geometry_sliced(s, out, 1, nullptr); }
 
 void pack_lights(const rm_scene_desc& s, PackedScene& out) {
     out.lights.resize(s.nlight * kLightWords);

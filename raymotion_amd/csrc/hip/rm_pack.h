@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <new>
 #include <utility>
 #include <string>
@@ -103,6 +104,12 @@ bool validate_scene(const rm_scene_desc& s, bool tree, bool materials, bool ligh
 
 void pack_tree(const rm_scene_desc& s, PackedScene& out);       // nodes, depth, stack_need
 void pack_geometry(const rm_scene_desc& s, PackedScene& out);   // geo (leaf order: needs the tree's bvh_tri), nrm, nn
+// pack_geometry in `slices` consecutive triangle ranges, calling
+// sink(first, count) once the geo, nrm and nn entries of triangles
+// [first, first + count) are written (the caller copies them to the device
+// while the next range is packed). False if sink returned false.
+bool pack_geometry_sliced(const rm_scene_desc& s, PackedScene& out, size_t slices,
+                          const std::function<bool(size_t, size_t)>& sink);
 void pack_lights(const rm_scene_desc& s, PackedScene& out);     // lights, cdf, light_total
 void pack_materials(const rm_scene_desc& s, PackedScene& out);  // matid, mats, uv, texels, opaque
 
