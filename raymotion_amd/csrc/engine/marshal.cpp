@@ -2,11 +2,14 @@
 #include "marshal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <cstdlib>
 #include <string>
 #include <unordered_map>
 
+#include "rm_internal.h"
+#include "rm_layout.h"
 #include "rm_parallel.h"
 
 namespace raymotion {
@@ -46,11 +49,17 @@ rmkeys::RecordKeys MarshalledScene::keys() const {
     return k;
 }
 
-void marshal_scene(const Scene& scene, MarshalledScene& out) {
-    const size_t n = scene.tris.size();
-    const size_t kept = std::min(out.tris.size(), n);  // records of the last marshal still in place
+bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink) {
+    const size_t n = scene.tris.size(), old_n = out.tris.size();
+    const size_t kept = std::min(old_n, n);  // records of the last marshal still in place
     out.tris.resize(n);
     out.texels.clear();
+    if (sink) {
+        out.geo.resize(n * rmh::kTriGeoWords);
+        out.nrm.resize(n * rmh::kTriNrmWords);
+        out.nn.resize(n);
+    }
+    std::atomic<bool> changed{old_n != n}, sink_ok{true};
     // Field conversion in parallel chunks (the records' key chunks). A record
     // equal to the one already in place is not written again, so re-marshalling
     // an unchanged scene only reads; each chunk's keys are taken from the
@@ -65,11 +74,16 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
     std::vector<std::vector<uint32_t>> textured(chunks);
     auto store = [&](size_t i, const rm_triangle& t) {
         rm_triangle& d = out.tris[i];
-        if (i >= kept || std::memcmp(&d, &t, sizeof t) != 0) d = t;
+        if (i >= kept || std::memcmp(&d, &t, sizeof t) != 0) {
+            d = t;
+            return true;
+        }
+        return false;
     };
     rmpar::parallel_chunks(chunks, [&](size_t c) {
         const size_t a = c * kChunk, b = std::min(n, a + kChunk);
         rmkeys::RecordMix mix(c);
+        bool dirty = false;
         for (size_t i = a; i < b; ++i) {
             const Triangle& s = scene.tris[i];
             rm_triangle t;
@@ -96,12 +110,20 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
                 }
                 textured[c].push_back(uint32_t(i));
             }
-            store(i, t);
+            dirty |= store(i, t);
             mix.add(reinterpret_cast<const unsigned char*>(&t));
+            if (sink) {
+                rmh::pack_tri_geo(t, int32_t(i), &out.geo[rmh::kTriGeoWords * i]);
+                rmh::pack_tri_nrm(t, &out.nrm[rmh::kTriNrmWords * i], &out.nn[i]);
+            }
         }
         out.kp[c] = mix.p.done();
         out.kn[c] = mix.n.done();
         out.km[c] = mix.m.done();
+        if (dirty) changed.store(true);
+        if (sink && !sink->put(a, b - a, &out.geo[rmh::kTriGeoWords * a], &out.nrm[rmh::kTriNrmWords * a],
+                               &out.nn[a]))
+            sink_ok.store(false);
     });
     std::unordered_map<const Texture*, int> atlas;
     for (const auto& list : textured)
@@ -118,7 +140,7 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
             t.texture_offset = ins.first->second;
             t.texture_width = s.texture->width;
             t.texture_height = s.texture->height;
-            store(i, t);
+            if (store(i, t)) changed.store(true);
         }
     rmpar::parallel_chunks(chunks, [&](size_t c) {
         if (textured[c].empty()) return;
@@ -158,6 +180,34 @@ void marshal_scene(const Scene& scene, MarshalledScene& out) {
         }
     }
     out.cdf.assign(scene.light_cdf.begin(), scene.light_cdf.end());
+    out.changed = changed.load();
+    return sink_ok.load();
+}
+
+namespace {
+
+struct StageSink final : GeometrySink {
+    rm_ctx* ctx;
+    explicit StageSink(rm_ctx* c) : ctx(c) {}
+    bool put(size_t first, size_t count, const rmh::F4* geo, const rmh::F4* nrm, const rmh::F4* nn) override {
+        return rm_geo_stage_put(ctx, first, count, geo, nrm, nn);
+    }
+};
+
+}  // namespace
+
+int marshal_and_upload(rm_ctx* ctx, const Scene& scene, MarshalledScene& m, bool gpu_tree, int rebuild) {
+    if (!ctx) return RM_ERR_INVALID;
+    bool staged = false;
+    if (gpu_tree && m.changed && !scene.tris.empty() && rm_geo_stage_begin(ctx, scene.tris.size()) == RM_OK) {
+        StageSink sink(ctx);
+        staged = marshal_scene(scene, m, &sink);
+    } else {
+        marshal_scene(scene, m);
+    }
+    const rm_scene_desc d = m.desc(scene);
+    return gpu_tree ? rm_set_scene_gpu_tree_keyed(ctx, &d, rebuild, m.keys(), staged)
+                    : rm_set_scene_keyed(ctx, &d, m.keys());
 }
 
 rm_frame make_frame(const Camera& cam, const RenderConfig& cfg) {

@@ -7,7 +7,6 @@
 #include "rm_parallel.h"
 #include "raymotion/bvh.hpp"
 #include "raymotion/renderer.hpp"
-#include "rm_internal.h"
 #include "rm_scene.h"
 
 struct rm_scene {
@@ -167,18 +166,12 @@ int rm_scene_export(const rm_scene* s, rm_triangle* tris, rm_bvh_node* nodes, in
 
 int rm_scene_upload(rm_ctx* ctx, const rm_scene* s) {
     if (!ctx || !s) return RM_ERR_INVALID;
-    raymotion::MarshalledScene& m = s->marshalled;
-    raymotion::marshal_scene(s->scene, m);
-    const rm_scene_desc d = m.desc(s->scene);
-    return rm_set_scene_keyed(ctx, &d, m.keys());
+    return raymotion::marshal_and_upload(ctx, s->scene, s->marshalled, false, 0);
 }
 
 int rm_scene_upload_gpu_tree(rm_ctx* ctx, const rm_scene* s, int rebuild) {
     if (!ctx || !s) return RM_ERR_INVALID;
-    raymotion::MarshalledScene& m = s->marshalled;
-    raymotion::marshal_scene(s->scene, m);
-    const rm_scene_desc d = m.desc(s->scene);
-    return rm_set_scene_gpu_tree_keyed(ctx, &d, rebuild, m.keys());
+    return raymotion::marshal_and_upload(ctx, s->scene, s->marshalled, true, rebuild);
 }
 
 static rm_camera to_c(const raymotion::Camera& c) {

@@ -5,7 +5,6 @@
 #include <mutex>
 
 #include "marshal.hpp"
-#include "rm_internal.h"
 #include "raymotion/metal.hpp"
 
 namespace raymotion {
@@ -46,11 +45,7 @@ bool render_image_hip_hdr(const Scene& scene, const Camera& cam, const RenderCon
     static std::mutex mu;
     std::lock_guard<std::mutex> lock(mu);
     static MarshalledScene marshalled;
-    marshal_scene(scene, marshalled);
-    const rm_scene_desc desc = marshalled.desc(scene);
-    const int rc = cfg.gpu_tree > 0
-                       ? rm_set_scene_gpu_tree_keyed(b.ctx, &desc, cfg.gpu_tree == 2 ? 1 : 0, marshalled.keys())
-                       : rm_set_scene_keyed(b.ctx, &desc, marshalled.keys());
+    const int rc = marshal_and_upload(b.ctx, scene, marshalled, cfg.gpu_tree > 0, cfg.gpu_tree == 2 ? 1 : 0);
     if (rc != RM_OK) {
         error = rm_last_error(b.ctx);
         return false;

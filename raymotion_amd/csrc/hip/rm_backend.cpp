@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -112,6 +113,13 @@ struct rm_ctx {
     // permutation, build scratch and a pinned read-back of level sizes.
     bool gpu_tree = false;
     DevBuf geo_scene, perm, build_scratch;
+    // Geometry staged by rm_geo_stage_put for the next GPU-tree upload
+    // (device format, scene order): triangles expected, triangles staged, and
+    // whether every staging copy was queued.
+    DevBuf geo_stage, nrm_stage, nn_stage;
+    size_t stage_ntri = 0;
+    std::atomic<size_t> staged_tris{0};
+    std::atomic<bool> stage_ok{false};
     uint32_t* build_host = nullptr;
     std::vector<int32_t> level_start;  // refit launch ranges into level_nodes, deepest level first
     int32_t nnodes = 0, nlights = 0, opaque = 1, stack_need = 0, ntri = 0;
@@ -515,6 +523,9 @@ void rm_destroy(rm_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     c->geo_scene.release();
+    c->geo_stage.release();
+    c->nrm_stage.release();
+    c->nn_stage.release();
     c->perm.release();
     c->build_scratch.release();
     if (c->build_host) (void)hipHostFree(c->build_host);
@@ -673,7 +684,7 @@ static int set_scene(rm_ctx* c, const rm_scene_desc* s, const rmkeys::RecordKeys
 int rm_set_scene(rm_ctx* c, const rm_scene_desc* s) { return set_scene(c, s, nullptr); }
 
 static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_request,
-                              const rmkeys::RecordKeys* pre) {
+                              const rmkeys::RecordKeys* pre, bool staged) {
     if (!c || !s_in) return RM_ERR_INVALID;
     std::lock_guard<std::mutex> lock(c->mu);
     if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
@@ -687,6 +698,9 @@ static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_
     if (s->ntri == 0 || s->ntri > size_t(INT32_MAX / 4)) return c->fail(RM_ERR_INVALID, "GPU tree build needs 1 to 2^29 triangles");
     const SceneKeys keys = keys_of(*s, pre);
     const bool had = c->has_scene && c->gpu_tree;
+    // staged geometry is taken by this call or dropped
+    const bool use_stage = staged && c->stage_ok.load() && c->stage_ntri == s->ntri && c->staged_tris.load() == s->ntri;
+    c->stage_ntri = 0;
     if (had && keys == c->keys && !rebuild_request) {
         c->last_refit = 0;
         c->stats.refit = 0;
@@ -709,8 +723,18 @@ static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_
     c->has_scene = false;
     const bool prof = c->profiling;
     if (prof) (void)hipEventRecord(c->ev[8], c->stream);
-    // no tree: records in scene order
-    bool ok = !(moved || normals) || pack_upload_geometry(c, *s, ps, c->geo_scene, moved);
+    // no tree: records in scene order (the staged copies already sit in the
+    // staging buffers, queued on c->stream ahead of everything below)
+    bool ok = true;
+    if (moved || normals) {
+        if (use_stage) {
+            std::swap(c->geo_scene, c->geo_stage);
+            std::swap(c->nrm, c->nrm_stage);
+            std::swap(c->nn, c->nn_stage);
+        } else {
+            ok = pack_upload_geometry(c, *s, ps, c->geo_scene, moved);
+        }
+    }
     pack_lights(*s, ps);
     if (materials) pack_materials(*s, ps);
     ok = ok && upload(c, c->lights, ps.lights) && upload(c, c->cdf, ps.cdf);
@@ -795,7 +819,7 @@ static int set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s_in, int rebuild_
 }
 
 int rm_set_scene_gpu_tree(rm_ctx* c, const rm_scene_desc* s, int rebuild_request) {
-    return set_scene_gpu_tree(c, s, rebuild_request, nullptr);
+    return set_scene_gpu_tree(c, s, rebuild_request, nullptr, false);
 }
 
 }  // extern "C"
@@ -805,8 +829,40 @@ int rm_set_scene_keyed(rm_ctx* c, const rm_scene_desc* s, const rmkeys::RecordKe
 }
 
 int rm_set_scene_gpu_tree_keyed(rm_ctx* c, const rm_scene_desc* s, int rebuild_request,
-                                const rmkeys::RecordKeys& keys) {
-    return set_scene_gpu_tree(c, s, rebuild_request, &keys);
+                                const rmkeys::RecordKeys& keys, bool staged) {
+    return set_scene_gpu_tree(c, s, rebuild_request, &keys, staged);
+}
+
+int rm_geo_stage_begin(rm_ctx* c, size_t ntri) {
+    if (!c || ntri == 0 || ntri > size_t(INT32_MAX / 4)) return RM_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(c->mu);
+    if (int rc = check(c, hipSetDevice(c->device), "hipSetDevice")) return rc;
+    c->stage_ntri = 0;
+    if (!c->geo_stage.ensure(ntri * kTriGeoWords * sizeof(F4)) || !c->nrm_stage.ensure(ntri * kTriNrmWords * sizeof(F4)) ||
+        !c->nn_stage.ensure(ntri * sizeof(F4)))
+        return c->fail(RM_ERR_ALLOC, "HIP geometry staging allocation failed");
+    c->stage_ntri = ntri;
+    c->staged_tris.store(0);
+    c->stage_ok.store(true);
+    return RM_OK;
+}
+
+bool rm_geo_stage_put(rm_ctx* c, size_t first, size_t count, const F4* geo, const F4* nrm, const F4* nn) {
+    if (!c || first > c->stage_ntri || count > c->stage_ntri - first || !geo || !nrm || !nn) {
+        if (c) c->stage_ok.store(false);
+        return false;
+    }
+    auto copy = [&](DevBuf& d, const F4* src, size_t per_tri) {
+        const size_t bytes = per_tri * sizeof(F4);
+        return hipMemcpyAsync(static_cast<char*>(d.ptr) + first * bytes, src, count * bytes, hipMemcpyHostToDevice,
+                              c->stream) == hipSuccess;
+    };
+    // the calling thread may be a worker whose current device is another one
+    const bool ok = hipSetDevice(c->device) == hipSuccess && copy(c->geo_stage, geo, kTriGeoWords) &&
+                    copy(c->nrm_stage, nrm, kTriNrmWords) && copy(c->nn_stage, nn, 1);
+    if (!ok) c->stage_ok.store(false);
+    c->staged_tris.fetch_add(count);
+    return ok;
 }
 
 namespace {

@@ -273,18 +273,9 @@ bool pack_geometry_sliced(const rm_scene_desc& s, PackedScene& out, size_t slice
         const size_t a = t * kTask, b = std::min(s.ntri, a + kTask);
         for (size_t k = a; k < b; ++k) {  // vertices in leaf order for traversal
             const int32_t j = s.nnode > 0 ? s.bvh_tri[k] : int32_t(k);
-            const rm_triangle& tr = s.tris[j];
-            out.geo[3 * k] = {tr.v0.x, tr.v0.y, tr.v0.z, tr.v1.x};
-            out.geo[3 * k + 1] = {tr.v1.y, tr.v1.z, tr.v2.x, tr.v2.y};
-            out.geo[3 * k + 2] = {tr.v2.z, ibf(j), 0.0f, 0.0f};
+            pack_tri_geo(s.tris[j], j, &out.geo[3 * k]);
         }
-        for (size_t i = a; i < b; ++i) {  // shading normals in scene order
-            const rm_triangle& tr = s.tris[i];
-            out.nrm[3 * i] = {tr.n0.x, tr.n0.y, tr.n0.z, tr.n.x};
-            out.nrm[3 * i + 1] = {tr.n1.x, tr.n1.y, tr.n1.z, tr.n.y};
-            out.nrm[3 * i + 2] = {tr.n2.x, tr.n2.y, tr.n2.z, tr.n.z};
-            out.nn[i] = {tr.nn.x, tr.nn.y, tr.nn.z, 0.0f};
-        }
+        for (size_t i = a; i < b; ++i) pack_tri_nrm(s.tris[i], &out.nrm[3 * i], &out.nn[i]);  // scene order
         const size_t sl = t / per_slice, t0 = sl * per_slice, nt = std::min(per_slice, tasks - t0);
         if (done[sl].fetch_add(1) + 1 == nt && sink) {
             const size_t first = t0 * kTask, last = std::min(s.ntri, (t0 + nt) * kTask);

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <functional>
 #include <new>
 #include <utility>
@@ -58,6 +59,24 @@ struct PinnedAllocator {
 };
 template <class T>
 using PinnedVector = std::vector<T, PinnedAllocator<T>>;
+
+// Device records of one triangle record t: its kTriGeoWords vertex words
+// ([v0.xyz v1.x] [v1.yz v2.xy] [v2.z id - -], id = the triangle's index,
+// stored as int bits), and its kTriNrmWords shading-normal words plus the
+// face normal ([n0.xyz n.x] [n1.xyz n.y] [n2.xyz n.z], [nn.xyz -]).
+inline void pack_tri_geo(const rm_triangle& t, int32_t id, F4* g) {
+    float idf;
+    std::memcpy(&idf, &id, 4);
+    g[0] = {t.v0.x, t.v0.y, t.v0.z, t.v1.x};
+    g[1] = {t.v1.y, t.v1.z, t.v2.x, t.v2.y};
+    g[2] = {t.v2.z, idf, 0.0f, 0.0f};
+}
+inline void pack_tri_nrm(const rm_triangle& t, F4* n, F4* nn) {
+    n[0] = {t.n0.x, t.n0.y, t.n0.z, t.n.x};
+    n[1] = {t.n1.x, t.n1.y, t.n1.z, t.n.y};
+    n[2] = {t.n2.x, t.n2.y, t.n2.z, t.n.z};
+    *nn = {t.nn.x, t.nn.y, t.nn.z, 0.0f};
+}
 
 struct PackedScene {
     PinnedVector<F4> nodes;        // kNodeWords per 4-wide node, preorder (parents before children)
