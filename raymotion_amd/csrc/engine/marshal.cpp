@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <new>
 #include <cstdlib>
 #include <string>
 #include <unordered_map>
@@ -199,11 +200,17 @@ struct StageSink final : GeometrySink {
 int marshal_and_upload(rm_ctx* ctx, const Scene& scene, MarshalledScene& m, bool gpu_tree, int rebuild) {
     if (!ctx) return RM_ERR_INVALID;
     bool staged = false;
-    if (gpu_tree && m.changed && !scene.tris.empty() && rm_geo_stage_begin(ctx, scene.tris.size()) == RM_OK) {
-        StageSink sink(ctx);
-        staged = marshal_scene(scene, m, &sink);
-    } else {
-        marshal_scene(scene, m);
+    try {
+        if (gpu_tree && m.changed && !scene.tris.empty() && rm_geo_stage_begin(ctx, scene.tris.size()) == RM_OK) {
+            StageSink sink(ctx);
+            staged = marshal_scene(scene, m, &sink);
+        } else {
+            marshal_scene(scene, m);
+        }
+    } catch (const std::bad_alloc&) {
+        // Host or pinned memory for the records. `m` stays allocated (staging
+        // copies from it may be queued); the next marshal rewrites it.
+        return RM_ERR_ALLOC;
     }
     const rm_scene_desc d = m.desc(scene);
     return gpu_tree ? rm_set_scene_gpu_tree_keyed(ctx, &d, rebuild, m.keys(), staged)
