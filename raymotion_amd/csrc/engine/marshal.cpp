@@ -50,7 +50,7 @@ rmkeys::RecordKeys MarshalledScene::keys() const {
     return k;
 }
 
-bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink) {
+bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink, bool with_tree) {
     const size_t n = scene.tris.size(), old_n = out.tris.size();
     const size_t kept = std::min(old_n, n);  // records of the last marshal still in place
     out.tris.resize(n);
@@ -154,11 +154,12 @@ bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink)
     });
     // Preorder with escape links: a node is followed by its left subtree, then
     // its right subtree; `right` holds the index just past its own subtree.
+    // Not needed for an upload that builds its own tree.
     out.nodes.clear();
-    out.nodes.reserve(scene.nodes.size());
+    if (with_tree) out.nodes.reserve(scene.nodes.size());
     struct Item { int index; bool exit; };
     std::vector<Item> todo;
-    if (!scene.nodes.empty()) todo.push_back({0, false});
+    if (with_tree && !scene.nodes.empty()) todo.push_back({0, false});
     while (!todo.empty()) {
         const Item it = todo.back();
         todo.pop_back();
@@ -203,9 +204,9 @@ int marshal_and_upload(rm_ctx* ctx, const Scene& scene, MarshalledScene& m, bool
     try {
         if (gpu_tree && m.changed && !scene.tris.empty() && rm_geo_stage_begin(ctx, scene.tris.size()) == RM_OK) {
             StageSink sink(ctx);
-            staged = marshal_scene(scene, m, &sink);
+            staged = marshal_scene(scene, m, &sink, false);
         } else {
-            marshal_scene(scene, m);
+            marshal_scene(scene, m, nullptr, !gpu_tree);
         }
     } catch (const std::bad_alloc&) {
         // Host or pinned memory for the records. `m` stays allocated (staging

@@ -66,8 +66,9 @@ struct GeometrySink {
 // (metal.mm:129-144), light CDF -> FP32 (metal.mm:145). `out` may hold the
 // previous marshal's arrays: records that did not change are not rewritten.
 // With a sink, each chunk's device-format geometry is also written and handed
-// to it; returns false if the sink refused a chunk.
-bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink = nullptr);
+// to it; returns false if the sink refused a chunk. Without `with_tree` the
+// node array is left empty (for uploads that build their own tree).
+bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink = nullptr, bool with_tree = true);
 
 // Marshals `scene` into `m` and uploads it (rm_set_scene_keyed, or
 // rm_set_scene_gpu_tree_keyed with `gpu_tree`). For a GPU-tree upload of a
