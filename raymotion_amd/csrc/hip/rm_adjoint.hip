@@ -23,6 +23,10 @@
 #include "rm_device.h"
 #include "rm_kernels.h"
 
+#ifndef RM_ADJ_BLOCKS
+#define RM_ADJ_BLOCKS 3
+#endif
+
 namespace rmh {
 namespace {
 
@@ -126,8 +130,13 @@ __device__ __forceinline__ uint64_t stream_seed(const FrameParams& P, uint32_t x
     return s;
 }
 
+// Resident blocks per CU both kernels are register-limited to: 3 (3 waves
+// per SIMD, at most 168 VGPRs), as the forward megakernel runs; the LDS
+// stacks of 3 blocks fit up to S = 48.
+constexpr int kAdjointBlocksPerCU = RM_ADJ_BLOCKS;
+
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_adjoint_primal(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
+__global__ __launch_bounds__(kBlock, kAdjointBlocksPerCU) void k_adjoint_primal(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
                                                             int round, unsigned long long* cnt) {
     __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_primal(DevScene sc, FramePar
 }
 
 template <int S>
-__global__ __launch_bounds__(kBlock) void k_adjoint_replay(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
+__global__ __launch_bounds__(kBlock, kAdjointBlocksPerCU) void k_adjoint_replay(DevScene sc, FrameParams P, AdjointArgs A, int lanes,
                                                             int round, unsigned long long* cnt) {
     __shared__ uint32_t stack[lds_stack_entries<S>() * kBlock];
     // The block's gradient table, 5 per group, sized at launch to the scene's
