@@ -4,7 +4,8 @@ marshalled, then take the staged buffers; uploads of the same records through
 the public rm_set_scene_gpu_tree pack and copy the geometry after comparing
 keys. Both must leave the same scene resident: frame by frame the images are
 equal bit for bit, through frames with moved vertices (staged), unchanged
-frames (nothing staged), and the rebuild after 31 refits."""
+frames (nothing staged), the rebuild after 31 refits, and a rebuild for a
+larger triangle count (staged geometry of the new count)."""
 import numpy as np
 import pytest
 
@@ -36,5 +37,20 @@ def test_staged_geometry_equals_packed(backend):
             assert np.array_equal(ra, rb), k
         print("refit codes:", refits, flush=True)
         assert 3 in refits[1:] and 1 in refits and 0 in refits  # a rebuild, refits and unchanged frames
+        # More triangles (the staged geometry then feeds a rebuild), then a
+        # frame that moves the original vertices again.
+        rng = np.random.default_rng(4)
+        a = rng.uniform(-6, 6, (40, 3)) + [0.0, 2.0, 0.0]
+        extra = np.stack([a, a + [0.4, 0.0, 0.0], a + [0.0, 0.3, 0.4]], axis=1)
+        s.add_triangles(extra, scenes.material_rows(len(extra), albedo=(0.6, 0.5, 0.4)))
+        for k in (40, 41):
+            if k == 41:
+                s.set_positions(np.concatenate([anim.positions(4.1), extra]))
+            backend.set_scene(s, gpu_tree=True)
+            other.set_scene(s.export(), gpu_tree=True)
+            assert backend.stats().refit == other.stats().refit == (3 if k == 40 else 1), k
+            ra, ha = backend.render(f, 100 + k)
+            rb, hb = other.render(f, 100 + k)
+            assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32)), k
     finally:
         other.close()
