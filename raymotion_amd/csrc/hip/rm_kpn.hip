@@ -11,6 +11,10 @@
 
 #include <algorithm>
 
+#ifndef RM_CONV16_BLOCKS
+#define RM_CONV16_BLOCKS 3
+#endif
+
 namespace rmh {
 
 KpnNet kpn_net() {
@@ -453,6 +457,22 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
         bf16x8 rc[4][2], re[4][2];
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) load_row(y0 - 1 + rr, rc[rr], re[rr]);
+        // MODE 3: the radiance window of the filter, (16 + 4) x (4 + 4) pixels
+        // around the tile, edge-clamped, loaded now so that the loads complete
+        // under the convolution (window position lane + 64 w, w = 0..2).
+        float rwin[3][3];
+        if constexpr (MODE == 3) {
+#pragma unroll
+            for (int w = 0; w < 3; ++w) {
+                const int q = min(lane + 64 * w, kWinW * kWinH - 1);
+                const int xx = min(max(x0 - 2 + q % kWinW, 0), W - 1);
+                const int yy = min(max(y0 - 2 + q / kWinW, 0), H - 1);
+                const float* src = rad + ((img + size_t(yy)) * W + xx) * 3;
+                rwin[w][0] = src[0];
+                rwin[w][1] = src[1];
+                rwin[w][2] = src[2];
+            }
+        }
         {
             f32x4 acc[4][NT];
 #pragma unroll
@@ -502,13 +522,14 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
                 // as k_kpn_apply does from the fp32 logits.
                 float* lw = lgs + wave * 64 * kKpnTaps;
                 float* rw = rws + wave * kWinW * kWinH * 3;
-                for (int q = lane; q < kWinW * kWinH; q += 64) {
-                    const int xx = min(max(x0 - 2 + q % kWinW, 0), W - 1);
-                    const int yy = min(max(y0 - 2 + q / kWinW, 0), H - 1);
-                    const float* src = rad + ((img + size_t(yy)) * W + xx) * 3;
-                    rw[3 * q] = src[0];
-                    rw[3 * q + 1] = src[1];
-                    rw[3 * q + 2] = src[2];
+#pragma unroll
+                for (int w = 0; w < 3; ++w) {
+                    const int q = lane + 64 * w;
+                    if (q < kWinW * kWinH) {
+                        rw[3 * q] = rwin[w][0];
+                        rw[3 * q + 1] = rwin[w][1];
+                        rw[3 * q + 2] = rwin[w][2];
+                    }
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
@@ -591,7 +612,7 @@ __global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv64(const __bf16* 
 // a k-step's fragment for row tile j is the input row j + dy + 1 shifted by
 // dx across lanes, formed for both of the k-step's taps and selected by kq.
 template <int COUT, int MODE>
-__global__ __launch_bounds__(256, kConvBlocksPerCU) void k_conv16(const __bf16* __restrict__ X,
+__global__ __launch_bounds__(256, RM_CONV16_BLOCKS) void k_conv16(const __bf16* __restrict__ X,
                                                                 const __bf16* __restrict__ Wt,
                                                                 const float* __restrict__ bias,
                                                                 void* __restrict__ out, int N, int H, int W) {
@@ -950,7 +971,10 @@ void launch_conv(int cin, int cout, int mode, const __bf16* x, const __bf16* w, 
     const bool tiles2d = cin == 64 || (cin == 16 && mode == 0);
     const size_t tiles = tiles2d ? size_t(n) * size_t((h + 3) / 4) * size_t((wdt + 15) / 16)
                                  : (P + 16 * kConvJ - 1) / (16 * kConvJ);
-    const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * kConvBlocksPerCU))), block(256);
+    // The first layer (161 VGPRs, 28 KiB of LDS) fits 3 blocks per CU; more
+    // waves keep more of its rows in flight (it is bound by HBM traffic).
+    const int per_cu = cin == 16 && mode == 0 ? RM_CONV16_BLOCKS : kConvBlocksPerCU;
+    const dim3 grid(unsigned(std::min<size_t>(blocks_of(tiles, 4), size_t(cus) * per_cu))), block(256);
 #define RM_CONV(CI, CO, MO) \
     hipLaunchKernelGGL((k_conv<CI, CO, MO>), grid, block, 0, st, x, w, b, act, out, n, h, wdt)
 #define RM_CONV64(CO, MO) \
