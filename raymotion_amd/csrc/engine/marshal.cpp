@@ -143,7 +143,7 @@ bool marshal_scene(const Scene& scene, MarshalledScene& out, GeometrySink* sink,
             t.texture_height = s.texture->height;
             if (store(i, t)) changed.store(true);
         }
-    rmpar::parallel_chunks(chunks, [&](size_t c) {
+    if (!atlas.empty()) rmpar::parallel_chunks(chunks, [&](size_t c) {
         if (textured[c].empty()) return;
         const size_t a = c * kChunk, b = std::min(n, a + kChunk);
         rmkeys::RecordMix mix(c);
