@@ -107,6 +107,114 @@ template <class R> struct HitInfo {  // pathtrace.metal:11-23
 
 struct Stats { uint64_t nodes = 0, tris = 0, rays = 0; };
 
+// A 4-wide traversal tree in the backend's layout (nodes: 8 float4 words
+// each, the lower x, upper x, lower y, upper y, lower z and upper z bounds of
+// the four children, then their references and kinds as int bits: -1 empty,
+// 0 inner node, k > 0 leaf of k triangles from leaf-order index ref; geo: the
+// triangles in leaf order, 3 float4 words each, v0.xyz v1.x | v1.yz v2.xy |
+// v2.z id).
+struct WideTree {
+    const float* nodes = nullptr;
+    size_t nnode = 0;
+    const float* geo = nullptr;
+};
+
+// Set by orc_set_wide_tree: FP32 frames then traverse this tree with
+// wide_walk in place of the binary walk of the scene (the same hits, and on
+// the backend's own tree the same slab arithmetic, so also the same results
+// where a ray grazes a box: tight boxes make such rays tree-dependent).
+WideTree g_wide;
+
+// The device's traversal (Tracer::intersect of the backend) over a wide
+// tree: slab tests with fused multiply-adds, children sorted by entry
+// distance, hit leaves intersected nearest-first within the current hit plus
+// 2^-12, inner children descended nearest-first with the rest stacked; any:
+// stop at the first hit. Returns the hit triangle (-1: none) and sets t (the
+// hit distance, tmax without a hit), u, v; counts node visits and triangle
+// tests into st.
+inline int wide_walk(const WideTree& w, V3<float> ro, V3<float> rd, float tmax, bool any, Stats& st, float& t,
+                     float& hu, float& hv) {
+    auto ibits = [](float f) { int32_t i; std::memcpy(&i, &f, 4); return i; };
+    float ht = tmax;
+    int hid = -1;
+    hu = hv = 0.0f;
+    if (w.nnode > 0) {
+        const float ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x), iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y),
+                    iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
+        const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
+        std::vector<int> stack;
+        int node = 0;
+        bool done = false;
+        while (!done) {
+            const float* nw = w.nodes + size_t(node) * 32;
+            ++st.nodes;
+            const float tc = ht + ht * (1.0f / 4096.0f);
+            float tn[4];
+            int ref[4], kind[4];
+            for (int c = 0; c < 4; ++c) {
+                const float ax = std::fma(nw[0 + c], ix, ox), bx = std::fma(nw[4 + c], ix, ox);
+                const float ay = std::fma(nw[8 + c], iy, oy), by = std::fma(nw[12 + c], iy, oy);
+                const float az = std::fma(nw[16 + c], iz, oz), bz = std::fma(nw[20 + c], iz, oz);
+                const float nr = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)),
+                                           std::fmax(std::fmin(az, bz), 0.0f));
+                const float fr = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
+                                           std::fmin(std::fmax(az, bz), tc));
+                const int k = ibits(nw[28 + c]);
+                const bool ok = k >= 0 && nr <= fr;
+                tn[c] = ok ? nr : INFINITY;
+                ref[c] = ibits(nw[24 + c]);
+                kind[c] = ok ? k : -1;
+            }
+            auto cx = [&](int a, int b) {
+                if (tn[b] < tn[a]) { std::swap(tn[a], tn[b]); std::swap(ref[a], ref[b]); std::swap(kind[a], kind[b]); }
+            };
+            cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
+            int next = -1;
+            for (int c = 3; c >= 0; --c)
+                if (kind[c] == 0) {
+                    if (next >= 0) stack.push_back(next);
+                    next = ref[c];
+                }
+            if (next < 0) {
+                if (stack.empty()) done = true;
+                else { next = stack.back(); stack.pop_back(); }
+            }
+            bool found = false;
+            for (int c = 0; c < 4 && !found; ++c) {
+                if (!(kind[c] > 0 && tn[c] <= ht + ht * (1.0f / 4096.0f))) continue;
+                for (int k = 0; k < kind[c] && !found; ++k) {
+                    const float* g = w.geo + size_t(ref[c] + k) * 12;
+                    ++st.tris;
+                    const V3<float> v0{g[0], g[1], g[2]}, v1{g[3], g[4], g[5]}, v2{g[6], g[7], g[8]};
+                    const int id = ibits(g[9]);
+                    const V3<float> e1 = v1 - v0, e2 = v2 - v0;
+                    const V3<float> p = cross3(rd, e2);
+                    const float det = dot3(e1, p);
+                    if (std::fabs(det) < 1e-12f) continue;
+                    const V3<float> tv = ro - v0;
+                    const float u = dot3(tv, p) / det;
+                    if (u < 0.0f || u > 1.0f) continue;
+                    const V3<float> q = cross3(tv, e1);
+                    const float v = dot3(rd, q) / det;
+                    if (v < 0.0f || u + v > 1.0f) continue;
+                    const float dd = dot3(e2, q) / det;
+                    if (dd > 1e-8f && (dd < ht || (dd == ht && hid >= 0 && id < hid))) {
+                        ht = dd;
+                        hid = id;
+                        hu = u;
+                        hv = v;
+                        if (any) found = true;
+                    }
+                }
+            }
+            if (found) break;
+            node = next;
+        }
+    }
+    t = ht;
+    return hid;
+}
+
 // ---------------------------------------------------------------- tracer
 template <class R> struct Tracer {
     const orc_scene* sc;
@@ -131,6 +239,19 @@ template <class R> struct Tracer {
         HitInfo<R> hit;
         hit.t = dist;
         if (stats) ++stats->rays;
+        if constexpr (std::is_same<R, float>::value) {
+            if (g_wide.nodes) {
+                Stats local;
+                float t, u, v;
+                hit.tri = wide_walk(g_wide, ro, rd, dist, shadow, stats ? *stats : local, t, u, v);
+                hit.t = t;
+                hit.hit = hit.tri >= 0;
+                hit.u = u;
+                hit.v = v;
+                hit.p = ro + rd * hit.t;
+                return hit;
+            }
+        }
         if (P.nodes == 0) return hit;
         const V3<R> safe{rd.x == 0 ? R(1e-30) : rd.x, rd.y == 0 ? R(1e-30) : rd.y, rd.z == 0 ? R(1e-30) : rd.z};
         const V3<R> inv{R(1) / safe.x, R(1) / safe.y, R(1) / safe.z};
@@ -1898,90 +2019,25 @@ int orc_trace_wide(const float* nodes, size_t nnode, const float* geo, const flo
                    const float* tmax, size_t n, int any, int threads, int32_t* tri, float* t, uint64_t* stats) {
     using namespace orc;
     std::vector<Stats> per(n);
-    auto ibits = [](float f) { int32_t i; std::memcpy(&i, &f, 4); return i; };
+    const WideTree w{nodes, nnode, geo};
     parallel_for(uint32_t(n), threads, [&](uint32_t i) {
         const V3<float> ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]}, rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-        float ht = tmax ? tmax[i] : 1e30f;
-        int hid = -1;
         Stats& st = per[i];
         ++st.rays;
-        if (nnode > 0) {
-            const float ix = 1.0f / (rd.x == 0.0f ? 1e-30f : rd.x), iy = 1.0f / (rd.y == 0.0f ? 1e-30f : rd.y),
-                        iz = 1.0f / (rd.z == 0.0f ? 1e-30f : rd.z);
-            const float ox = -ro.x * ix, oy = -ro.y * iy, oz = -ro.z * iz;
-            std::vector<int> stack;
-            int node = 0;
-            bool done = false;
-            while (!done) {
-                const float* w = nodes + size_t(node) * 32;
-                ++st.nodes;
-                const float tc = ht + ht * (1.0f / 4096.0f);
-                float tn[4];
-                int ref[4], kind[4];
-                for (int c = 0; c < 4; ++c) {
-                    const float ax = std::fma(w[0 + c], ix, ox), bx = std::fma(w[4 + c], ix, ox);
-                    const float ay = std::fma(w[8 + c], iy, oy), by = std::fma(w[12 + c], iy, oy);
-                    const float az = std::fma(w[16 + c], iz, oz), bz = std::fma(w[20 + c], iz, oz);
-                    const float nr = std::fmax(std::fmax(std::fmin(ax, bx), std::fmin(ay, by)),
-                                               std::fmax(std::fmin(az, bz), 0.0f));
-                    const float fr = std::fmin(std::fmin(std::fmax(ax, bx), std::fmax(ay, by)),
-                                               std::fmin(std::fmax(az, bz), tc));
-                    const int k = ibits(w[28 + c]);
-                    const bool ok = k >= 0 && nr <= fr;
-                    tn[c] = ok ? nr : INFINITY;
-                    ref[c] = ibits(w[24 + c]);
-                    kind[c] = ok ? k : -1;
-                }
-                auto cx = [&](int a, int b) {
-                    if (tn[b] < tn[a]) { std::swap(tn[a], tn[b]); std::swap(ref[a], ref[b]); std::swap(kind[a], kind[b]); }
-                };
-                cx(0, 1); cx(2, 3); cx(0, 2); cx(1, 3); cx(1, 2);
-                int next = -1;
-                for (int c = 3; c >= 0; --c)
-                    if (kind[c] == 0) {
-                        if (next >= 0) stack.push_back(next);
-                        next = ref[c];
-                    }
-                if (next < 0) {
-                    if (stack.empty()) done = true;
-                    else { next = stack.back(); stack.pop_back(); }
-                }
-                bool found = false;
-                for (int c = 0; c < 4 && !found; ++c) {
-                    if (!(kind[c] > 0 && tn[c] <= ht + ht * (1.0f / 4096.0f))) continue;
-                    for (int k = 0; k < kind[c] && !found; ++k) {
-                        const float* g = geo + size_t(ref[c] + k) * 12;
-                        ++st.tris;
-                        const V3<float> v0{g[0], g[1], g[2]}, v1{g[3], g[4], g[5]}, v2{g[6], g[7], g[8]};
-                        const int id = ibits(g[9]);
-                        const V3<float> e1 = v1 - v0, e2 = v2 - v0;
-                        const V3<float> p = cross3(rd, e2);
-                        const float det = dot3(e1, p);
-                        if (std::fabs(det) < 1e-12f) continue;
-                        const V3<float> tv = ro - v0;
-                        const float u = dot3(tv, p) / det;
-                        if (u < 0.0f || u > 1.0f) continue;
-                        const V3<float> q = cross3(tv, e1);
-                        const float v = dot3(rd, q) / det;
-                        if (v < 0.0f || u + v > 1.0f) continue;
-                        const float dd = dot3(e2, q) / det;
-                        if (dd > 1e-8f && (dd < ht || (dd == ht && hid >= 0 && id < hid))) {
-                            ht = dd;
-                            hid = id;
-                            if (any) found = true;
-                        }
-                    }
-                }
-                if (found) break;
-                node = next;
-            }
-        }
-        tri[i] = hid;
+        float ht, u, v;
+        tri[i] = wide_walk(w, ro, rd, tmax ? tmax[i] : 1e30f, any != 0, st, ht, u, v);
         if (t) t[i] = ht;
     });
     stats[0] = stats[1] = 0;
     for (const auto& s : per) { stats[0] += s.nodes; stats[1] += s.tris; }
     return 0;
+}
+
+// FP32 frames (orc_render_frame, orc_restir_merge, orc_render_adjoint) trace
+// the given 4-wide tree with the backend's traversal until this is called
+// again with nodes = NULL (process-wide; not for concurrent callers).
+void orc_set_wide_tree(const float* nodes, size_t nnode, const float* geo) {
+    orc::g_wide = orc::WideTree{nodes, nodes ? nnode : 0, nodes ? geo : nullptr};
 }
 
 // Brute force over every triangle with the same Moller-Trumbore arithmetic and

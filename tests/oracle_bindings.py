@@ -3,6 +3,7 @@ own CPU engine built from its sources (oracle/_ref/libref.so). Test
 infrastructure only."""
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 from pathlib import Path
@@ -52,6 +53,7 @@ def orc() -> ctypes.CDLL:
             ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
             ("orc_check_div_pi", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+            ("orc_set_wide_tree", None, [_P, _SZ, _P]),
             ("orc_trace_wide", ctypes.c_int, [_P, _SZ, _P, _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
                                               _P]),
             ("orc_restir_merge", ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Frame), ctypes.c_uint64,
@@ -213,6 +215,23 @@ def trace(exported, origins, dirs, tmax=None, any_hit=False, threads: int = THRE
     orc().orc_trace(ctypes.byref(d), ptr(o), ptr(dd), ptr(tm), n, int(any_hit), threads, ptr(tri), ptr(t), ptr(u),
                     ptr(v), ptr(stats))
     return tri, t, u, v, stats
+
+
+@contextlib.contextmanager
+def wide_tree(nodes, tris):
+    """Inside the block, FP32 oracle frames (render_frame, restir_merge,
+    render_adjoint) traverse the backend's own 4-wide tree (Backend.read_tree)
+    with the backend's traversal instead of the scene's binary tree: the same
+    boxes and slab arithmetic as the GPU, so also the same outcome for rays
+    that graze a box edge, where tight boxes make the result depend on the
+    tree."""
+    nd = np.ascontiguousarray(nodes, np.float32)
+    tr = np.ascontiguousarray(tris, np.float32)
+    orc().orc_set_wide_tree(ptr(nd), len(nd), ptr(tr))
+    try:
+        yield
+    finally:
+        orc().orc_set_wide_tree(None, 0, None)
 
 
 def trace_wide(nodes, tris, origins, dirs, tmax=None, any_hit=False, threads: int = THREADS):
