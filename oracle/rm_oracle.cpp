@@ -25,6 +25,7 @@
 // Everything is templated on the real type R: R=float is the FP32 parity
 // oracle, R=double the FP64 reference renderer.
 #include "oracle_abi.h"
+#include "../raymotion_amd/csrc/common/rm_exp.h"
 #include "../raymotion_amd/csrc/common/rm_fdiv.h"
 #include "../raymotion_amd/csrc/common/rm_sincos.h"
 
@@ -1654,10 +1655,11 @@ void denoise_atrous(const orc_params& P, uint32_t id, int step, const orc_gbuffe
             float wn = std::fmax(nd, 0.0f);
             wn = wn * wn; wn = wn * wn; wn = wn * wn; wn = wn * wn;  // ^16
             const float ex = pp.x - p0.x, ey = pp.y - p0.y, ez = pp.z - p0.z;
-            const float wz = std::exp(-std::sqrt(ex * ex + ey * ey + ez * ez) / (0.1f * dc + 1e-4f));
-            const float wl = std::exp(-std::fabs(luma(c) - l0) / (0.3f * std::fmax(l0, 0.5f) + 1e-3f));
+            // exp as rm_expf (one deterministic rounding, shared with the device)
+            const float wz = rm_expf(-std::sqrt(ex * ex + ey * ey + ez * ez) / (0.1f * dc + 1e-4f));
+            const float wl = rm_expf(-std::fabs(luma(c) - l0) / (0.3f * std::fmax(l0, 0.5f) + 1e-3f));
             const float ax = aa.x - a0.x, ay = aa.y - a0.y, az = aa.z - a0.z;
-            const float wa = std::exp(-std::sqrt(ax * ax + ay * ay + az * az) / 0.25f);
+            const float wa = rm_expf(-std::sqrt(ax * ax + ay * ay + az * az) / 0.25f);
             const float w = wn * wz * wl * wa;
             sum = sum + c * w;
             wsum += w;
@@ -2113,6 +2115,41 @@ uint64_t orc_check_div_pi(uint32_t lo, uint32_t hi, int threads) {
         bad += local;
     });
     return bad.load();
+}
+
+// rm_expf against libm's double exp rounded to float, over the float bit
+// patterns [lo, hi) of both signs: returns the largest difference in ulp
+// (2^31 where one result is NaN and the other is not), and stores how many
+// results differ at all.
+uint64_t orc_check_exp(uint32_t lo, uint32_t hi, int threads, uint64_t* differ) {
+    std::atomic<uint64_t> worst{0}, ndiff{0};
+    const uint32_t n = hi > lo ? hi - lo : 0, chunk = 1u << 20;
+    orc::parallel_for((n + chunk - 1) / chunk, threads, [&](uint32_t c) {
+        uint64_t lw = 0, ld = 0;
+        const uint32_t a = lo + c * chunk, b = std::min(hi, a + chunk);
+        for (uint32_t u = a; u < b; ++u)
+            for (uint32_t sign = 0; sign < 2; ++sign) {
+                const uint32_t bits = u | (sign << 31);
+                float x;
+                std::memcpy(&x, &bits, 4);
+                const float want = float(std::exp(double(x))), got = rm_expf(x);
+                if (want != want || got != got) {
+                    if ((want != want) != (got != got)) lw = uint64_t(1) << 31, ++ld;
+                    continue;
+                }
+                int32_t wb, gb;  // both results are >= 0: bit patterns order like values
+                std::memcpy(&wb, &want, 4);
+                std::memcpy(&gb, &got, 4);
+                const uint64_t d = uint64_t(wb > gb ? wb - gb : gb - wb);
+                lw = std::max(lw, d);
+                ld += d != 0;
+            }
+        uint64_t cur = worst.load();
+        while (lw > cur && !worst.compare_exchange_weak(cur, lw)) {}
+        ndiff += ld;
+    });
+    if (differ) *differ = ndiff.load();
+    return worst.load();
 }
 
 // Closed-form test hooks (FP32 oracle functions).

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../common/rm_exp.h"
 #include "../common/rm_fdiv.h"
 #include "../common/rm_sincos.h"
 #include "rm_layout.h"

@@ -269,8 +269,9 @@ __global__ __launch_bounds__(kBlock) void k_temporal(FrameParams P, GBufferSoA g
 }
 
 // denoise_atrous (pathtrace.metal:1315-1354): 5x5 taps at stride 2^step with
-// normal, depth, luminance and albedo edge-stopping weights. The three
-// exponentials of a tap are evaluated as one exp of their summed arguments.
+// normal, depth, luminance and albedo edge-stopping weights, each exponential
+// weight as the reference writes it (quotient, then exp), with rm_expf: the
+// host denoiser gives the same bits.
 __global__ __launch_bounds__(kBlock) void k_atrous(FrameParams P, GBufferSoA gb, const float4* __restrict__ in,
                                                     float4* __restrict__ out, int step) {
     const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
@@ -286,8 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_atrous(FrameParams P, GBufferSoA gb,
     const float l0 = luma(c0);
     const float dcx = p0.x - P.cam.x, dcy = p0.y - P.cam.y, dcz = p0.z - P.cam.z;
     const float dc = sqrtf(dcx * dcx + dcy * dcy + dcz * dcz);
-    const float inv_z = 1.0f / (0.1f * dc + 1e-4f);
-    const float inv_l = 1.0f / (0.3f * fmaxf(l0, 0.5f) + 1e-3f);
+    const float sz = 0.1f * dc + 1e-4f, sl = 0.3f * fmaxf(l0, 0.5f) + 1e-3f;
     const int stride = 1 << step;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f, wsum = 0.0f;
     for (int dy = -2; dy <= 2; ++dy) {
@@ -305,11 +305,12 @@ __global__ __launch_bounds__(kBlock) void k_atrous(FrameParams P, GBufferSoA gb,
             float wn = fmaxf(nd, 0.0f);
             wn = wn * wn; wn = wn * wn; wn = wn * wn; wn = wn * wn;
             const float ex = q0.x - p0.x, ey = q0.y - p0.y, ez = q0.z - p0.z;
+            const float wz = rm_expf(-sqrtf(ex * ex + ey * ey + ez * ez) / sz);
+            const float wl = rm_expf(-fabsf(luma(c) - l0) / sl);
             const float ax = aa.x - a0.x, ay = aa.y - a0.y, az = aa.z - a0.z;
-            const float arg = sqrtf(ex * ex + ey * ey + ez * ez) * inv_z + fabsf(luma(c) - l0) * inv_l +
-                              sqrtf(ax * ax + ay * ay + az * az) * 4.0f;
-            const float w = wn * __expf(-arg);
-            sr += w * c.x; sg += w * c.y; sb += w * c.z;
+            const float wa = rm_expf(-sqrtf(ax * ax + ay * ay + az * az) * 4.0f);  // / 0.25, exactly
+            const float w = wn * wz * wl * wa;
+            sr += c.x * w; sg += c.y * w; sb += c.z * w;
             wsum += w;
         }
     }

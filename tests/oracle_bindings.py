@@ -53,6 +53,7 @@ def orc() -> ctypes.CDLL:
             ("orc_bsdf_grad_f64", None, [_P, _P, _P, ctypes.c_double, ctypes.c_double, _P, _P, _P, _P]),
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
             ("orc_check_div_pi", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+            ("orc_check_exp", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _P]),
             ("orc_set_wide_tree", None, [_P, _SZ, _P]),
             ("orc_trace_wide", ctypes.c_int, [_P, _SZ, _P, _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
                                               _P]),
@@ -281,6 +282,15 @@ def div_pi_mismatches(lo_bits: int, hi_bits: int, threads: int = 8) -> int:
     """Floats with |x|'s bits in [lo_bits, hi_bits), both signs, for which the
     shared division by pi differs from the IEEE quotient x / pi."""
     return int(orc().orc_check_div_pi(lo_bits, hi_bits, threads))
+
+
+def exp_ulp(lo_bits: int, hi_bits: int, threads: int = 8):
+    """Floats with |x|'s bits in [lo_bits, hi_bits), both signs: the largest
+    difference in ulp between the shared exponential and libm's double exp
+    rounded to float (2^31 for a NaN on one side only), and how many differ."""
+    n = ctypes.c_uint64(0)
+    worst = int(orc().orc_check_exp(lo_bits, hi_bits, threads, ctypes.addressof(n)))
+    return worst, int(n.value)
 
 
 class RefScene:

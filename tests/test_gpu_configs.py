@@ -11,11 +11,9 @@ history rules, metal.mm:232-254):
   in 16 px and 2 a-trous passes, before and after one animation step.
 
 The oracle traverses the backend's own 4-wide tree (oracle_bindings.wide_tree):
-the G-buffer and the initial and merged reservoirs equal the oracle's bit for
-bit, and so do the ray, node-visit and triangle-test counts. Bar for the
-output: per-pixel FP32 radiance within 1e-3 relative (floor 1e-3 absolute; the
-a-trous weights use a fast exponential), and identical 8-bit output wherever
-the radiance agrees bit for bit."""
+the G-buffer, the initial and merged reservoirs, the denoised FP32 radiance
+and the 8-bit output equal the oracle's bit for bit, and so do the ray,
+node-visit and triangle-test counts."""
 import numpy as np
 import pytest
 
@@ -46,13 +44,14 @@ def _check(backend, exported, f, seed, hist, expect_history):
     assert _fields_equal(g_rmerged, info["rmerged"])
     assert (st.node_visits[5], st.tri_tests[5]) == (info["node_visits"], info["tri_tests"])
     err = rel_err(g_hdr, c_hdr)
-    same = g_hdr == c_hdr
+    same = (np.ascontiguousarray(g_hdr, np.float32).view(np.uint32).reshape(-1) ==
+            np.ascontiguousarray(c_hdr, np.float32).view(np.uint32).reshape(-1))
     print(f"{f.width}x{f.height} {f.spp} spp seed {seed}: path {info['path']}, history {info['has_history']}, "
           f"{info['rays']} oracle rays, max rel {err.max():.3g}, exact {np.mean(same):.6f}", flush=True)
     assert info["path"] == 1
     assert info["has_history"] == expect_history
-    assert err.max() <= 1e-3
-    assert np.array_equal(g_rgb[same], c_rgb[same])
+    assert same.all()
+    assert np.array_equal(np.asarray(g_rgb).reshape(-1), np.asarray(c_rgb).reshape(-1))
     gpu = {"megakernel": st.rays[0], "generate": st.rays[1], "spatial": st.rays[2], "shade": st.rays[3]}
     print(f"  rays per pass: GPU {gpu}, oracle {info['pass_rays']}", flush=True)
     assert gpu == info["pass_rays"]
