@@ -188,3 +188,31 @@ def test_restir_pipeline_cornell(backend):
         err = rel_err(g_hdr, c_hdr)
         print(f"frame {k}: shade exact {np.mean(gshade == info['shade']):.6f}, final max rel {err.max():.3g}")
         assert err.max() <= 1e-3
+
+
+def test_restir_pipeline_cornell_same_tree(backend):
+    """The same three frames with the oracle traversing the backend's own
+    4-wide tree: shading, denoised radiance and 8-bit output bit-identical.
+    The first frame starts without history (the backend holds some from the
+    test above)."""
+    e = cornell().export()
+    hist = ob.History()
+    backend.set_scene(e)
+    nodes, tris, _ = backend.read_tree()
+    for k in range(3):
+        cam = scenes.cornell_camera()
+        cam.focal = 32 / np.tan(np.radians(20.0))
+        cam.x += 0.01 * k
+        f = frame(64, 64, 4, 5, cam=cam, candidates=8, spatial=3, radius=8.0, temporal=1 if k else 0)
+        seed = 100 + k
+        g_rgb, g_hdr = backend.render(f, seed)
+        gshade = backend.read_restir(64, 64)[3]
+        with ob.wide_tree(nodes, tris):
+            c_rgb, c_hdr, info = ob.render_frame(e, f, seed, history=hist, debug=True)
+        exact = [np.mean(np.asarray(a, np.float32).view(np.uint32).reshape(-1) ==
+                         np.ascontiguousarray(b, np.float32).view(np.uint32).reshape(-1))
+                 for a, b in ((gshade, info["shade"]), (g_hdr, c_hdr))]
+        print(f"frame {k}: shade exact {exact[0]:.6f}, final exact {exact[1]:.6f}")
+        assert exact == [1.0, 1.0]
+        assert np.array_equal(np.asarray(g_rgb).reshape(-1), np.asarray(c_rgb).reshape(-1))
+        check_pass_rays(backend.stats(), info)
