@@ -25,6 +25,7 @@
 // Everything is templated on the real type R: R=float is the FP32 parity
 // oracle, R=double the FP64 reference renderer.
 #include "oracle_abi.h"
+#include "../raymotion_amd/csrc/common/rm_atan.h"
 #include "../raymotion_amd/csrc/common/rm_exp.h"
 #include "../raymotion_amd/csrc/common/rm_fdiv.h"
 #include "../raymotion_amd/csrc/common/rm_sincos.h"
@@ -316,7 +317,7 @@ template <class R> struct Tracer {
             hit.n_geom = cvt<R>(t.n);
             if (hit.shader == 1) {
                 const R kpi = R(3.141592653589793);
-                const R az = std::atan(hit.p.x / hit.p.z) * R(180) / kpi + ((hit.p.z > 0) ? R(0) : R(180));
+                const R az = atan_r(hit.p.x / hit.p.z) * R(180) / kpi + ((hit.p.z > 0) ? R(0) : R(180));
                 const R h = gmod(az, R(360)) / R(360);
                 // hsv2rgb(h, 0.8, 1) (pathtrace.metal:27-64); h lies in [0,1) here.
                 const R s = R(0.8), val = R(1);
@@ -384,6 +385,12 @@ template <class R> struct Tracer {
     static R div_pi(R x) {
         if constexpr (std::is_same<R, float>::value) return rm_div_pi(x);
         else return x / R(3.141592653589793);
+    }
+    // atan of the hue shader: in FP32 the deterministic rm_atanf (rm_atan.h,
+    // within one ulp of libm, the same bits as the kernels), in FP64 libm's.
+    static R atan_r(R x) {
+        if constexpr (std::is_same<R, float>::value) return rm_atanf(x);
+        else return std::atan(x);
     }
 
     static Bsdf evaluate_opaque_bsdf(R wi_x, R wi_y, R wi_z, R wo_x, R wo_y, R wo_z, R n_x, R n_y, R n_z,
@@ -2144,6 +2151,39 @@ uint64_t orc_check_exp(uint32_t lo, uint32_t hi, int threads, uint64_t* differ) 
                 lw = std::max(lw, d);
                 ld += d != 0;
             }
+        uint64_t cur = worst.load();
+        while (lw > cur && !worst.compare_exchange_weak(cur, lw)) {}
+        ndiff += ld;
+    });
+    if (differ) *differ = ndiff.load();
+    return worst.load();
+}
+
+// rm_atanf against libm's double atan rounded to float, over the float bit
+// patterns [lo, hi) of both signs: the largest difference in ulp (2^31 where
+// one result is NaN and the other is not); stores how many results differ.
+uint64_t orc_check_atan(uint32_t lo, uint32_t hi, int threads, uint64_t* differ) {
+    std::atomic<uint64_t> worst{0}, ndiff{0};
+    const uint32_t n = hi > lo ? hi - lo : 0, chunk = 1u << 20;
+    orc::parallel_for((n + chunk - 1) / chunk, threads, [&](uint32_t c) {
+        uint64_t lw = 0, ld = 0;
+        const uint32_t a = lo + c * chunk, b = std::min(hi, a + chunk);
+        for (uint32_t u = a; u < b; ++u) {
+            float x;
+            std::memcpy(&x, &u, 4);  // x >= 0; atan is odd and rm_atanf mirrors the sign exactly
+            const float want = float(std::atan(double(x))), got = rm_atanf(x), gneg = rm_atanf(-x);
+            if (want != want || got != got) {
+                if ((want != want) != (got != got)) lw = uint64_t(1) << 31, ++ld;
+                continue;
+            }
+            if (!(gneg == -got)) lw = uint64_t(1) << 31, ++ld;
+            int32_t wb, gb;
+            std::memcpy(&wb, &want, 4);
+            std::memcpy(&gb, &got, 4);
+            const uint64_t d = uint64_t(wb > gb ? wb - gb : gb - wb);
+            lw = std::max(lw, d);
+            ld += d != 0;
+        }
         uint64_t cur = worst.load();
         while (lw > cur && !worst.compare_exchange_weak(cur, lw)) {}
         ndiff += ld;

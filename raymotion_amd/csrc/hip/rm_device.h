@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../common/rm_atan.h"
 #include "../common/rm_exp.h"
 #include "../common/rm_fdiv.h"
 #include "../common/rm_sincos.h"
@@ -510,7 +511,7 @@ struct Tracer {
             if (s.shader == 1) {
                 // Hue from azimuth through hsv2rgb(h, 0.8, 1) (pathtrace.metal:27-64,193-201).
                 const float kpi = 3.14159265358979323846f;
-                const float az = atanf(s.p.x / s.p.z) * 180.0f / kpi + ((s.p.z > 0.0f) ? 0.0f : 180.0f);
+                const float az = rm_atanf(s.p.x / s.p.z) * 180.0f / kpi + ((s.p.z > 0.0f) ? 0.0f : 180.0f);
                 const float hh = (az - floorf(az / 360.0f) * 360.0f) / 360.0f;
                 const float cc = 0.8f * 1.0f;
                 const float q = hh / 60.0f;

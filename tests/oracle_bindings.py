@@ -54,6 +54,7 @@ def orc() -> ctypes.CDLL:
             ("orc_sincos", None, [_P, _SZ, _P, _P]),
             ("orc_check_div_pi", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
             ("orc_check_exp", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _P]),
+            ("orc_check_atan", ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _P]),
             ("orc_set_wide_tree", None, [_P, _SZ, _P]),
             ("orc_trace_wide", ctypes.c_int, [_P, _SZ, _P, _P, _P, _P, _SZ, ctypes.c_int, ctypes.c_int, _P, _P,
                                               _P]),
@@ -290,6 +291,16 @@ def exp_ulp(lo_bits: int, hi_bits: int, threads: int = 8):
     rounded to float (2^31 for a NaN on one side only), and how many differ."""
     n = ctypes.c_uint64(0)
     worst = int(orc().orc_check_exp(lo_bits, hi_bits, threads, ctypes.addressof(n)))
+    return worst, int(n.value)
+
+
+def atan_ulp(lo_bits: int, hi_bits: int, threads: int = 8):
+    """Floats x >= 0 with bits in [lo_bits, hi_bits): the largest difference
+    in ulp between the shared arctangent and libm's double atan rounded to
+    float (2^31 for a NaN on one side only, or atan(-x) != -atan(x)), and how
+    many differ."""
+    n = ctypes.c_uint64(0)
+    worst = int(orc().orc_check_atan(lo_bits, hi_bits, threads, ctypes.addressof(n)))
     return worst, int(n.value)
 
 

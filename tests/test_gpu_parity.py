@@ -2,8 +2,10 @@
 
 Bar: bit-identical closest-hit triangle ids (distances within 1e-6 relative);
 per-pixel FP32 radiance within 1e-3 relative (floor 1e-3 absolute) and
-identical 8-bit output wherever the radiance agrees. Sizes are ones the
-oracle finishes in seconds.
+identical 8-bit output wherever the radiance agrees; with the oracle on the
+backend's own 4-wide tree (adaptive lanes, hue shader, ReSTIR pipeline),
+radiance and 8-bit output bit-identical. Sizes are ones the oracle finishes
+in seconds.
 """
 import numpy as np
 import pytest
@@ -119,10 +121,24 @@ def test_render_cornell_lanes(backend, spp, lanes):
     assert info["lanes"] == lanes
 
 
+def _exact_on_backend_tree(backend, exported, f, seed):
+    """Radiance and 8-bit output bit-identical with the oracle traversing the
+    backend's own 4-wide tree (the scene is already set)."""
+    g_rgb, g_hdr = backend.render(f, seed)
+    nodes, tris, _ = backend.read_tree()
+    with ob.wide_tree(nodes, tris):
+        c_rgb, c_hdr, _ = ob.render_frame(exported, f, seed)
+    assert np.array_equal(np.asarray(g_hdr, np.float32).view(np.uint32).reshape(-1),
+                          np.ascontiguousarray(c_hdr, np.float32).view(np.uint32).reshape(-1))
+    assert np.array_equal(np.asarray(g_rgb).reshape(-1), np.asarray(c_rgb).reshape(-1))
+
+
 def test_render_adaptive_lanes8(backend):
     e = cornell().export()
-    info = _compare_render(backend, e, cornell_frame(40, 32, 64, 5, adapt_min=16, adapt_step=8), 4242)
+    f = cornell_frame(40, 32, 64, 5, adapt_min=16, adapt_step=8)
+    info = _compare_render(backend, e, f, 4242)
     assert info["lanes"] == 8
+    _exact_on_backend_tree(backend, e, f, 4242)
 
 
 def test_render_adaptive_one_lane(backend):
@@ -147,6 +163,7 @@ def test_render_hue_shader_scene(backend):
     e, cam = hue_scene(120.0)
     f = frame(48, 48, 16, 4, cam=cam, candidates=4, spatial=0)
     _compare_render(backend, e, f, 99)
+    _exact_on_backend_tree(backend, e, f, 99)
 
 
 def test_render_metal_scene_ris(backend):
